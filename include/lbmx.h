@@ -1,0 +1,246 @@
+/*
+ * lbmx.h -- C ABI of the MI355X lattice Boltzmann forward/adjoint engine.
+ *
+ * Every entry point returns an lbmx_status (0 = success), the counterpart of the
+ * reference's PetscErrorCode convention (CHKERRQ propagation). Arrays crossing the
+ * boundary are plain host pointers in the reference's single-rank PETSc DMDA order:
+ *   design / sensitivity : natural order  (k*ny + j)*nx + i           (1 value per node)
+ *   distributions        : array-of-structures [k][j][i][q]          (Q values per node)
+ * so a PETSc adapter maps Vec <-> pointer with VecGetArrayRead/VecGetArray and nothing else.
+ *
+ * A handle is thread-compatible, not thread-safe: calls on one handle must not overlap.
+ */
+#ifndef LBMX_H
+#define LBMX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBMX_VERSION_MAJOR 0
+#define LBMX_VERSION_MINOR 1
+
+/* ---- status codes (mirroring the PETSc error classes the reference raises) ---- */
+typedef int lbmx_status;
+#define LBMX_OK               0
+#define LBMX_ERR_ARG          62   /* PETSC_ERR_ARG_OUTOFRANGE / bad argument            */
+#define LBMX_ERR_SUP          56   /* PETSC_ERR_SUP: orientation/type not implemented    */
+#define LBMX_ERR_MEM          55   /* PETSC_ERR_MEM: device allocation failed            */
+#define LBMX_ERR_STATE        73   /* PETSC_ERR_ORDER: call sequence violated            */
+#define LBMX_ERR_DEVICE       76   /* HIP runtime error (message in lbmx_last_error)     */
+
+/* ---- lattices (src/latticeBoltzmann/lattices2d.hh:16-84, lattices3d.hh:34-53) ---- */
+typedef enum {
+  LBMX_D2Q9 = 0,        /* flow only, Q = 9                                   */
+  LBMX_D3Q19 = 1,       /* flow only, Q = 19                                  */
+  LBMX_D2Q9_D2Q5 = 2    /* thermal double distribution, 9 + 5 = 14 per node  */
+} lbmx_lattice;
+
+/* ---- flow collision operators ---- */
+typedef enum {
+  LBMX_BGK = 0,           /* IncompressibleBGK2d<D2Q9> (src/dynamics/incompressibleBGK2d.hh:94-153);
+                             3D: incompressible BGK with IncompressibleFeq3d                 */
+  LBMX_MRT = 1,           /* IncompressibleMRT2d<D2Q9> (src/dynamics/incompressibleMRT2d.hh:18-98),
+                             IncompressibleMRT3d<D3Q19> (src/dynamics/MRT3d.hh:28-200)       */
+  LBMX_CASCADED = 2,      /* IncompressibleCascaded2d<D2Q9> (src/dynamics/cascaded2d.hh:15-97) */
+  LBMX_MRT_STANDARD = 3,  /* StandardMRT3d<D3Q19> (src/dynamics/MRT3d.hh:206-383), forward only */
+  LBMX_BGK_STANDARD = 4   /* StandardBGK2d (src/dynamics/incompressibleBGK2d.hh:52-88),
+                             StandardBGK3d (src/dynamics/BGK3d.hh:10-38), forward only       */
+} lbmx_collision;
+
+/* ---- how the design field gamma (1 = fluid, 0 = solid) enters the collision ---- */
+typedef enum {
+  LBMX_DESIGN_NONE = 0,
+  LBMX_DESIGN_PBB = 1,      /* PartialBouncebackCollision<Base, BPInterpolation>
+                               (src/dynamics/partialBouncebackCollision.hh:53-65):
+                               imult = 0.5 * alpha_max * BP_q(gamma); alpha_max = 1 is the
+                               reference operator exactly                                   */
+  LBMX_DESIGN_BRINKMAN = 2, /* Brinkman drag F = -alpha_max * BP_q(gamma) * u with Guo forcing
+                               (no reference counterpart for BGK/MRT3d; defined by this engine) */
+  LBMX_DESIGN_DARCY = 3     /* DarcyMRT2d<D2Q9,BP> (src/dynamics/darcyMRT2d.hh:33-59), MRT only */
+} lbmx_design;
+
+typedef enum {
+  LBMX_FP64 = 0,            /* plain form: FP64 storage, FP64 collision                      */
+  LBMX_FP32 = 1,            /* FP32 storage, FP32 collision                                  */
+  LBMX_FP16_SHIFTED = 2,    /* FP16 storage of (f_q - w_q)/w_q, FP32 collision, FP32 adjoint  */
+  LBMX_BF16_SHIFTED = 3     /* BF16 storage of (f_q - w_q)/w_q, FP32 collision, FP32 adjoint  */
+} lbmx_precision;
+
+typedef struct {
+  int nx, ny, nz;             /* nz = 1 for 2D lattices                                     */
+  int lattice;                /* lbmx_lattice                                               */
+  int collision;              /* lbmx_collision                                             */
+  int design;                 /* lbmx_design                                                */
+  int precision;              /* lbmx_precision                                             */
+  int reserved0;
+  double omega;               /* flow relaxation rate (1/tau)                               */
+  double interp_q;            /* BPInterpolation penalisation q (src/core/interpolationFunctions.hh:7-28) */
+  double alpha_max;           /* design coupling strength                                   */
+  double body_force[3];       /* DarcyMRT2d constant body force (cfx, cfy)                  */
+  /* thermal part (LBMX_D2Q9_D2Q5): ThermalObstacleCollision<ThermalMRT2d<D2Q5_TMRT>, Ramp> */
+  double thermal_diffusivity; /* reference diffusivity D_ref (src/dynamics/thermalObstacleCollision.hh:47-53) */
+  double ramp_p;              /* RampInterpolation penal (src/core/interpolationFunctions.hh:30-54) */
+  double ramp_ck;             /* RampInterpolation Ck (k_solid/k_fluid = 1/Ck)               */
+} lbmx_desc;
+
+/* Inclusive index box, the reference's Box2d/Box3d (src/core/globalDefinitions.hh:25-71). */
+typedef struct { int x0, x1, y0, y1, z0, z1; } lbmx_box;
+
+/* ---- boundary conditions ---- */
+typedef enum {
+  LBMX_BC_BOUNCEBACK = 0,         /* OnGridBounceback2d/3d (src/boundaryConditions/bounceback2d.cc:7-193,
+                                     bounceback3d.cc:8-511), all orientations                    */
+  LBMX_BC_ZOUHE_VELOCITY = 1,     /* {Incompressible,Standard}ZouHeVelocity{West,North,East}2d
+                                     (src/boundaryConditions/ZouHe2d.hh:23-195)                  */
+  LBMX_BC_ZOUHE_PRESSURE = 2,     /* {Incompressible,Standard}ZouHePressure{East,West}2d
+                                     (src/boundaryConditions/ZouHe2d.hh:240-390)                 */
+  LBMX_BC_ZOUHE_NEUMANN = 3,      /* IncompressibleZouHeNeumann{East,West}2d (forward only)      */
+  LBMX_BC_HECHT_VELOCITY = 4,     /* {Incompressible,Standard}HechtVelocityWest3d
+                                     (src/boundaryConditions/Hecht3d.hh:26-110)                  */
+  LBMX_BC_HECHT_PRESSURE = 5,     /* {Incompressible,Standard}HechtPressureEast3d
+                                     (src/boundaryConditions/Hecht3d.hh:120-238)                 */
+  LBMX_BC_THERMAL_BOUNCEBACK = 6, /* ThermalBounceback2d<D2Q9,D2Q5> (adiabatic),
+                                     (src/boundaryConditions/thermalBounceback2d.hh:260-300)     */
+  LBMX_BC_THERMAL_DIRICHLET = 7   /* ThermalZouHeWest2d<D2Q9,D2Q5> (thermalZouHe2d.hh:62-93); the
+                                     N/S/E orientations are the same construction, defined here  */
+} lbmx_bc_type;
+
+typedef enum {
+  LBMX_PROFILE_UNIFORM = 0,       /* value                                                      */
+  LBMX_PROFILE_PARABOLIC = 1,     /* value*(R^2-(s-c0)^2)/R^2, s = tangential index (2D), as the
+                                     InflowVelocity functors of applications/diffuser/diffuser.cc:21-31 */
+  LBMX_PROFILE_PARABOLIC_DISK = 2,/* value*(1-r^2/R^2) inside r<=R, else 0; r^2=(s-c0)^2+(t-c1)^2
+                                     (DevFlowYZ, applications/Diffuser3d/diffuser3d.cc:22-46)   */
+  LBMX_PROFILE_PARABOLIC_DUCT = 3 /* value*(1-((s-c0)/R)^2)*(1-((t-c1)/R1)^2)                  */
+} lbmx_profile_kind;
+
+typedef enum {
+  LBMX_TIME_CONSTANT = 0,
+  LBMX_TIME_SINE = 1              /* factor = t_offset + t_amp*sin(2*pi*t/t_period + t_phase),
+                                     as applications/RobustPump2d/robustPump2d.cc:14-19           */
+} lbmx_time_kind;
+
+typedef struct {
+  int type;           /* lbmx_bc_type                                                      */
+  int profile;        /* lbmx_profile_kind                                                 */
+  int time_mode;      /* lbmx_time_kind                                                    */
+  int reserved0;
+  double value;       /* u_max (velocity), rho0 (pressure), T0 (thermal Dirichlet)         */
+  double dir[3];      /* velocity direction multipliers (ux, uy, uz)                       */
+  double center[2];   /* profile centre in the tangential index coordinates (s, t)         */
+  double radius[2];   /* profile radii                                                     */
+  double t_period, t_amp, t_offset, t_phase;
+} lbmx_bc_desc;
+
+/* ---- objective functionals: J = sum_t timeConst * sum_terms ---- */
+typedef enum {
+  LBMX_OBJ_NONE = 0,
+  LBMX_OBJ_PRESSURE_DROP = 1,     /* PressureDrop2d (src/functionals/pressureDrop2d.hh:14-52):
+                                     mean(rho, box_a) - mean(rho, box_b); also used in 3D       */
+  LBMX_OBJ_OUTFLOW = 2,           /* OutflowEast2d (src/functionals/outflow2d.hh:15-54): -mean(u_x, box_a) */
+  LBMX_OBJ_DISSIPATED_POWER = 3,  /* total-pressure flux through the ports, defined here:
+                                     sum_box_a (p+|u|^2/2) u_x - sum_box_b (p+|u|^2/2) u_x      */
+  LBMX_OBJ_HEAT_FLUX = 4          /* ThermalOutflowEast2d (src/functionals/temperatureFlow2d.hh:18-54):
+                                     -sum_box_a u_x * T                                          */
+} lbmx_objective_kind;
+
+typedef struct {
+  int kind;                 /* lbmx_objective_kind                                       */
+  int reserved0;
+  double time_const;        /* the reference's timeConst (e.g. 1/numSteps)               */
+  lbmx_box box_a;
+  lbmx_box box_b;
+} lbmx_objective;
+
+typedef struct lbmx_solver lbmx_solver;
+
+/* ---- lifetime ---- */
+/* IsothermalLBSolver2d/3d::make (src/latticeBoltzmann/isothermalLBSolver2d.hh:20-46),
+   ThermalLBSolver2d::make (src/latticeBoltzmann/thermalLBSolver2d.hh:15-110). The design
+   field defaults to 1 (fluid), as VecSet(materialGlobal,1.) (src/latticeBoltzmann/NewLBSolver2d.cc:25). */
+lbmx_status lbmx_create(const lbmx_desc* desc, lbmx_solver** out);
+lbmx_status lbmx_destroy(lbmx_solver* s);
+/* Bind a HIP stream (hipStream_t passed as void*); NULL = the legacy default stream. */
+lbmx_status lbmx_set_stream(lbmx_solver* s, void* hip_stream);
+/* Message of the last failing call on this thread. */
+const char* lbmx_last_error(void);
+
+/* ---- setup ---- */
+/* NewLBSolver2d::addBoundaryCondition (src/latticeBoltzmann/NewLBSolver2d.cc:106-155),
+   LBSolver3d::addBoundaryCondition (src/latticeBoltzmann/LBSolver3d.cc): the box is
+   intersected with the 4 edges + 4 corners (2D) or 6 faces + 12 edges + 8 corners (3D)
+   of the domain; each piece becomes one boundary loop, applied in insertion order. */
+lbmx_status lbmx_add_boundary(lbmx_solver* s, const lbmx_box* box, const lbmx_bc_desc* bc);
+/* LBSolver::setFieldFromDesignVec (src/latticeBoltzmann/LBSolver.cc:333-348). */
+lbmx_status lbmx_set_design(lbmx_solver* s, const double* gamma, size_t n);
+/* LBSolver::createDesignVec (src/latticeBoltzmann/LBSolver.cc:318-331): copy out the field. */
+lbmx_status lbmx_get_design(lbmx_solver* s, double* gamma, size_t n);
+/* The functional later used by lbmx_forward/lbmx_gradient. */
+lbmx_status lbmx_set_objective(lbmx_solver* s, const lbmx_objective* obj);
+
+/* ---- forward ---- */
+/* LBSolver::initializeAtEquilibrium (src/latticeBoltzmann/LBSolver.cc:234-249) with uniform
+   macros (rho0, u0) (uniformMacroInitialization), thermal part at temperature T0; t := 0. */
+lbmx_status lbmx_init_equilibrium(lbmx_solver* s, double rho0, const double u0[3], double T0);
+/* Per-node initial macros: macros[(k*ny+j)*nx+i][0..3] = (rho, ux, uy, uz) (getInitialMacroArray);
+   thermal: T in [4]; stride = 5 doubles per node. */
+lbmx_status lbmx_init_equilibrium_field(lbmx_solver* s, const double* macros, size_t n);
+/* nsteps x LBSolver::collideAndStream (src/latticeBoltzmann/LBSolver.cc:108-115), with the
+   objective of each new state accumulated as computeMacroFunctional does after the next
+   collision; *J (may be NULL) receives the sum over these steps. */
+lbmx_status lbmx_forward(lbmx_solver* s, int nsteps, double* J);
+/* distributionsGlobal: the pre-collision state after the last stream+BC, AoS [node][q]. */
+lbmx_status lbmx_get_state(lbmx_solver* s, double* f, size_t n);
+/* LBSolver::setDistributions + setCurrentTimestep (src/latticeBoltzmann/LBSolver.cc:65-97). */
+lbmx_status lbmx_set_state(lbmx_solver* s, const double* f, size_t n, int timestep);
+/* Macros of the current state: (rho, ux, uy, uz[, T]) per node, 5 doubles per node. */
+lbmx_status lbmx_get_macros(lbmx_solver* s, double* macros, size_t n);
+lbmx_status lbmx_get_timestep(lbmx_solver* s, int* t);
+
+/* ---- adjoint (AdjointLBSolver, src/adjointLatticeBoltzmann/adjointLBSolver.hh:11-46) ---- */
+/* Wang dynamic checkpointing with nslots HBM slots (LBDynamicCheckpointing,
+   src/topOpt/LBCheckpointing.cc:37-65); nslots <= 0 stores every step (NaiveCheckpointing). */
+lbmx_status lbmx_checkpoint_config(lbmx_solver* s, int nslots);
+/* One design-gradient evaluation, the loop of applications/RobustPump2d/robustPump2d.cc:233-277:
+   initializeAtEquilibrium(rho0,u0,T0), nsteps recorded forward steps, resetAdjoints, then
+   nsteps reverse steps (adjointCollide, computeCollideSource, adjointStream) with checkpoint
+   recomputation. J (may be NULL) = objective; sens (may be NULL, n = nodes) = dJ/dgamma. */
+lbmx_status lbmx_gradient(lbmx_solver* s, int nsteps, double rho0, const double u0[3], double T0,
+                          double* J, double* sens, size_t n);
+/* AdjointLBSolver::getSensitivities after the last lbmx_gradient (the sensitivityGlobal Vec). */
+lbmx_status lbmx_get_sensitivity(lbmx_solver* s, double* sens, size_t n);
+/* The adjoint distributions after the last lbmx_gradient (adjointGlobal), AoS [node][q]. */
+lbmx_status lbmx_get_adjoint(lbmx_solver* s, double* adj, size_t n);
+
+/* ---- instrumentation ---- */
+typedef struct {
+  double fwd_kernel_ms;       /* device time of forward sweeps in the last call          */
+  double adj_kernel_ms;       /* device time of adjoint sweeps in the last call          */
+  double forward_phase_ms;    /* wall time of the recorded forward phase                 */
+  double reverse_phase_ms;    /* wall time of the reverse phase incl. recomputation      */
+  long long fwd_sweeps;       /* forward sweeps launched (incl. recomputation)           */
+  long long adj_sweeps;
+  long long checkpoint_copies;
+  long long recompute_sweeps;
+} lbmx_stats;
+lbmx_status lbmx_get_stats(lbmx_solver* s, lbmx_stats* st);
+/* Bytes of device memory held by the handle (state, adjoint, checkpoints). */
+lbmx_status lbmx_device_bytes(lbmx_solver* s, size_t* bytes);
+
+/* ---- host-only diagnostics (no device access) ---- */
+/* Boundary pieces after box splitting, in application order: per piece 10 ints
+   (type, nx, ny, nz, x0, x1, y0, y1, z0, z1). *count receives the number of pieces. */
+lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, int* count);
+/* Replays the checkpoint schedule of lbmx_gradient for (nslots, nsteps) on labelled host
+   buffers and verifies that every reverse step receives the right state. *advances = forward
+   steps run in the reverse phase (recomputation), *max_slots = slots in use at most. */
+lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances, int* max_slots);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBMX_H */

@@ -1,0 +1,915 @@
+// Host engine behind the C ABI of include/lbmx.h. One engine = one lattice on the current
+// HIP device: state buffers in HBM, boundary pieces, functional, design field, adjoint and
+// the checkpoint schedule. Setup calls are host-only; device memory is allocated on the
+// first call that computes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "checkpoint.hpp"
+#include "lbmx.h"
+#include "lbmx_common.h"
+#include "lbmx_launch.h"
+
+namespace lbmx {
+
+static thread_local std::string g_last_error;
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define HIP_OK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      throw Error{LBMX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)};       \
+  } while (0)
+
+static size_t prec_state_bytes(int prec) {
+  switch (prec) {
+    case LBMX_FP64: return 8;
+    case LBMX_FP32: return 4;
+    default: return 2;
+  }
+}
+static size_t prec_lam_bytes(int prec) { return prec == LBMX_FP64 ? 8 : 4; }
+static size_t prec_gamma_bytes(int prec) { return prec == LBMX_FP64 ? 8 : 4; }
+
+struct DeviceBuffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t b) {
+    if (bytes >= b && p) return;
+    reset();
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error{LBMX_ERR_MEM, "hipMalloc of " + std::to_string(b) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    bytes = b;
+  }
+};
+
+// zero-fill with a kernel-free path that is safe outside graph capture
+static void zero_async(void* p, size_t bytes, hipStream_t s) { HIP_OK(hipMemsetAsync(p, 0, bytes, s)); }
+
+struct Engine {
+  lbmx_desc d{};
+  int Q = 0, D = 0;
+  long long N = 0;
+  bool inc = true;
+  hipStream_t stream = nullptr;
+
+  std::vector<DevBC> bcs;           // pieces in insertion order
+  std::vector<double> gamma_host;   // design field
+  bool gamma_dirty = true;
+  ObjArgs obj{};                    // acc pointer filled at launch
+  bool obj_set = false;
+
+  // device
+  DeviceBuffer gamma_dev, bc_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
+  std::vector<std::unique_ptr<DeviceBuffer>> pool;  // state buffers: [0],[1] working; rest slots
+  bool bc_dirty = true;
+  void* cur = nullptr;   // p_t
+  void* prev = nullptr;  // p_{t-1} (valid when state_kind == 0)
+  int t = 0;
+  int state_kind = -1;   // -1 none, 0 p_{t-1} valid, 1 uniform equilibrium, 2 raw AoS in raw_aos
+  double init_rho = 1, init_u[3] = {0, 0, 0}, init_T = 0;
+  int nslots = 64;
+  bool grad_valid = false;
+  lbmx_stats stats{};
+
+  explicit Engine(const lbmx_desc& desc) : d(desc) {
+    if (d.nx < 1 || d.ny < 1 || d.nz < 1) throw Error{LBMX_ERR_ARG, "grid dimensions must be positive"};
+    switch (d.lattice) {
+      case LBMX_D2Q9: Q = 9; D = 2; break;
+      case LBMX_D3Q19: Q = 19; D = 3; break;
+      case LBMX_D2Q9_D2Q5: Q = 14; D = 2; break;
+      default: throw Error{LBMX_ERR_ARG, "unknown lattice"};
+    }
+    if (D == 2 && d.nz != 1) throw Error{LBMX_ERR_ARG, "2D lattices need nz = 1"};
+    if (d.precision < LBMX_FP64 || d.precision > LBMX_BF16_SHIFTED) throw Error{LBMX_ERR_ARG, "unknown precision"};
+    if (d.design < LBMX_DESIGN_NONE || d.design > LBMX_DESIGN_DARCY) throw Error{LBMX_ERR_ARG, "unknown design coupling"};
+    bool ok = false;
+    switch (d.collision) {
+      case LBMX_BGK: case LBMX_MRT: ok = true; break;
+      case LBMX_BGK_STANDARD: ok = d.lattice != LBMX_D2Q9_D2Q5; break;
+      case LBMX_MRT_STANDARD: ok = d.lattice == LBMX_D3Q19; break;
+      default: ok = false;
+    }
+    if (!ok) throw Error{LBMX_ERR_SUP, "collision operator not available for this lattice"};
+    if (d.design == LBMX_DESIGN_DARCY) throw Error{LBMX_ERR_SUP, "DarcyMRT2d coupling is not available"};
+    inc = !(d.collision == LBMX_MRT_STANDARD || d.collision == LBMX_BGK_STANDARD);
+    N = (long long)d.nx * d.ny * d.nz;
+    gamma_host.assign(N, 1.0);
+    obj.nterm = 0;
+  }
+
+  ~Engine() {
+    if (touched) (void)hipStreamSynchronize(stream);
+  }
+  bool touched = false;
+
+  Geometry geom() const { return Geometry{d.nx, d.ny, d.nz, 0, N}; }
+
+  ModelParams params() const {
+    ModelParams m{};
+    m.omega = d.omega;
+    m.alpha_max = d.alpha_max;
+    m.interp_q = d.interp_q;
+    m.body[0] = d.body_force[0]; m.body[1] = d.body_force[1]; m.body[2] = d.body_force[2];
+    m.tdiff = d.thermal_diffusivity;
+    m.ramp_p = d.ramp_p;
+    m.ramp_ck = d.ramp_ck;
+    m.design = d.design;
+    return m;
+  }
+
+  size_t state_bytes() const { return (size_t)N * Q * prec_state_bytes(d.precision); }
+  size_t lam_bytes() const { return (size_t)N * Q * prec_lam_bytes(d.precision); }
+  int grid_blocks() const {
+    return ((d.nx + kTileX - 1) / kTileX) * ((d.ny + kTileY - 1) / kTileY) * d.nz;
+  }
+
+  // ---- boundary pieces: NewLBSolver2d::addBoundaryCondition (src/latticeBoltzmann/NewLBSolver2d.cc:106-155),
+  //      getBoxBoundaries (src/core/geometry2d.cc:31-53, src/core/geometry3d.cc:36-85) ----
+  struct Piece {
+    int box[6];
+    int n[3];
+  };
+  std::vector<Piece> pieces() const {
+    const int X = d.nx - 1, Y = d.ny - 1, Z = d.nz - 1;
+    std::vector<Piece> p;
+    auto add = [&](int x0, int x1, int y0, int y1, int z0, int z1, int a, int b, int c) {
+      p.push_back(Piece{{x0, x1, y0, y1, z0, z1}, {a, b, c}});
+    };
+    if (D == 2) {
+      add(1, X - 1, Y, Y, 0, 0, 0, 1, 0);
+      add(1, X - 1, 0, 0, 0, 0, 0, -1, 0);
+      add(0, 0, 1, Y - 1, 0, 0, -1, 0, 0);
+      add(X, X, 1, Y - 1, 0, 0, 1, 0, 0);
+      add(0, 0, Y, Y, 0, 0, -1, 1, 0);
+      add(X, X, Y, Y, 0, 0, 1, 1, 0);
+      add(0, 0, 0, 0, 0, 0, -1, -1, 0);
+      add(X, X, 0, 0, 0, 0, 1, -1, 0);
+    } else {
+      add(1, X - 1, Y, Y, 1, Z - 1, 0, 1, 0);
+      add(1, X - 1, 0, 0, 1, Z - 1, 0, -1, 0);
+      add(0, 0, 1, Y - 1, 1, Z - 1, -1, 0, 0);
+      add(X, X, 1, Y - 1, 1, Z - 1, 1, 0, 0);
+      add(1, X - 1, 1, Y - 1, Z, Z, 0, 0, 1);
+      add(1, X - 1, 1, Y - 1, 0, 0, 0, 0, -1);
+      add(0, 0, Y, Y, 1, Z - 1, -1, 1, 0);
+      add(X, X, Y, Y, 1, Z - 1, 1, 1, 0);
+      add(1, X - 1, Y, Y, Z, Z, 0, 1, 1);
+      add(1, X - 1, Y, Y, 0, 0, 0, 1, -1);
+      add(0, 0, 0, 0, 1, Z - 1, -1, -1, 0);
+      add(X, X, 0, 0, 1, Z - 1, 1, -1, 0);
+      add(1, X - 1, 0, 0, Z, Z, 0, -1, 1);
+      add(1, X - 1, 0, 0, 0, 0, 0, -1, -1);
+      add(0, 0, 1, Y - 1, Z, Z, -1, 0, 1);
+      add(0, 0, 1, Y - 1, 0, 0, -1, 0, -1);
+      add(X, X, 1, Y - 1, Z, Z, 1, 0, 1);
+      add(X, X, 1, Y - 1, 0, 0, 1, 0, -1);
+      add(0, 0, Y, Y, 0, 0, -1, 1, -1);
+      add(X, X, Y, Y, 0, 0, 1, 1, -1);
+      add(0, 0, Y, Y, Z, Z, -1, 1, 1);
+      add(X, X, Y, Y, Z, Z, 1, 1, 1);
+      add(0, 0, 0, 0, 0, 0, -1, -1, -1);
+      add(X, X, 0, 0, 0, 0, 1, -1, -1);
+      add(0, 0, 0, 0, Z, Z, -1, -1, 1);
+      add(X, X, 0, 0, Z, Z, 1, -1, 1);
+    }
+    return p;
+  }
+
+  bool supported(int type, const int* n) const {
+    int nn = std::abs(n[0]) + std::abs(n[1]) + std::abs(n[2]);
+    switch (type) {
+      case LBMX_BC_BOUNCEBACK: return true;
+      case LBMX_BC_ZOUHE_VELOCITY:
+        if (D != 2 || nn != 1) return false;
+        return inc ? (n[0] == -1 || n[1] == 1 || n[0] == 1) : n[0] == -1;
+      case LBMX_BC_ZOUHE_PRESSURE: return D == 2 && nn == 1 && n[1] == 0;
+      case LBMX_BC_ZOUHE_NEUMANN: return false;  // needs neighbour data: not on the device path
+      case LBMX_BC_HECHT_VELOCITY: return D == 3 && nn == 1 && n[0] == -1;
+      case LBMX_BC_HECHT_PRESSURE: return D == 3 && nn == 1 && n[0] == 1;
+      case LBMX_BC_THERMAL_BOUNCEBACK: return d.lattice == LBMX_D2Q9_D2Q5;
+      case LBMX_BC_THERMAL_DIRICHLET: return d.lattice == LBMX_D2Q9_D2Q5 && nn == 1;
+    }
+    return false;
+  }
+
+  static bool intersect(const int* a, const int* b, int* o) {
+    for (int k = 0; k < 3; ++k) {
+      o[2 * k] = std::max(a[2 * k], b[2 * k]);
+      o[2 * k + 1] = std::min(a[2 * k + 1], b[2 * k + 1]);
+      if (o[2 * k + 1] < o[2 * k]) return false;
+    }
+    return true;
+  }
+
+  void add_boundary(const lbmx_box& box, const lbmx_bc_desc& bc) {
+    if (bc.type < LBMX_BC_BOUNCEBACK || bc.type > LBMX_BC_THERMAL_DIRICHLET)
+      throw Error{LBMX_ERR_ARG, "unknown boundary type"};
+    int dom[6] = {0, d.nx - 1, 0, d.ny - 1, 0, d.nz - 1};
+    int ub[6] = {box.x0, box.x1, box.y0, box.y1, box.z0, box.z1};
+    int b[6];
+    if (!intersect(ub, dom, b)) return;
+    auto ps = pieces();
+    for (auto& pc : ps) {
+      int o[6];
+      if (intersect(b, pc.box, o) && !supported(bc.type, pc.n))
+        throw Error{LBMX_ERR_SUP, "boundary type " + std::to_string(bc.type) +
+                                      " is not implemented for orientation (" + std::to_string(pc.n[0]) + "," +
+                                      std::to_string(pc.n[1]) + "," + std::to_string(pc.n[2]) + ")"};
+    }
+    if ((bc.type == LBMX_BC_ZOUHE_VELOCITY || bc.type == LBMX_BC_HECHT_VELOCITY) && bc.profile != LBMX_PROFILE_UNIFORM &&
+        (bc.radius[0] == 0.0))
+      throw Error{LBMX_ERR_ARG, "profile radius must be non-zero"};
+    if (bc.time_mode == LBMX_TIME_SINE && bc.t_period == 0.0) throw Error{LBMX_ERR_ARG, "sine period must be non-zero"};
+    for (auto& pc : ps) {
+      int o[6];
+      if (!intersect(b, pc.box, o)) continue;
+      DevBC e{};
+      e.type = bc.type;
+      for (int k = 0; k < 3; ++k) e.nrm[k] = pc.n[k];
+      for (int k = 0; k < 6; ++k) e.box[k] = o[k];
+      e.profile = bc.profile;
+      e.time_mode = bc.time_mode;
+      e.inc = inc ? 1 : 0;
+      e.value = bc.value;
+      for (int k = 0; k < 3; ++k) e.dir[k] = bc.dir[k];
+      e.center[0] = bc.center[0]; e.center[1] = bc.center[1];
+      e.radius[0] = bc.radius[0]; e.radius[1] = bc.radius[1];
+      e.t_period = bc.t_period; e.t_amp = bc.t_amp; e.t_offset = bc.t_offset; e.t_phase = bc.t_phase;
+      bcs.push_back(e);
+    }
+    bc_dirty = true;
+  }
+
+  // ---- functional terms (src/functionals/*.hh) ----
+  void set_objective(const lbmx_objective& o) {
+    ObjArgs a{};
+    a.time_const = o.time_const;
+    auto size = [](const lbmx_box& b) {
+      return (double)(b.x1 - b.x0 + 1) * (b.y1 - b.y0 + 1) * (b.z1 - b.z0 + 1);
+    };
+    auto term = [&](const lbmx_box& b, double w, int kind) {
+      if (b.x1 < b.x0 || b.y1 < b.y0 || b.z1 < b.z0) throw Error{LBMX_ERR_ARG, "empty objective box"};
+      if (b.x0 < 0 || b.y0 < 0 || b.z0 < 0 || b.x1 >= d.nx || b.y1 >= d.ny || b.z1 >= d.nz)
+        throw Error{LBMX_ERR_ARG, "objective box outside the grid"};
+      DevTerm& t = a.terms[a.nterm++];
+      t.box[0] = b.x0; t.box[1] = b.x1; t.box[2] = b.y0; t.box[3] = b.y1; t.box[4] = b.z0; t.box[5] = b.z1;
+      t.weight = w;
+      t.kind = kind;
+    };
+    switch (o.kind) {
+      case LBMX_OBJ_NONE: break;
+      case LBMX_OBJ_PRESSURE_DROP:
+        term(o.box_a, 1.0 / size(o.box_a), 0);
+        term(o.box_b, -1.0 / size(o.box_b), 0);
+        break;
+      case LBMX_OBJ_OUTFLOW: term(o.box_a, -1.0 / size(o.box_a), 1); break;
+      case LBMX_OBJ_DISSIPATED_POWER:
+        term(o.box_a, 1.0, 2);
+        term(o.box_b, -1.0, 2);
+        break;
+      case LBMX_OBJ_HEAT_FLUX:
+        if (d.lattice != LBMX_D2Q9_D2Q5) throw Error{LBMX_ERR_SUP, "heat flux needs the thermal lattice"};
+        term(o.box_a, -1.0, 3);
+        break;
+      default: throw Error{LBMX_ERR_ARG, "unknown objective"};
+    }
+    int bb[6] = {INT32_MAX, -1, INT32_MAX, -1, INT32_MAX, -1};
+    for (int m = 0; m < a.nterm; ++m)
+      for (int k = 0; k < 3; ++k) {
+        bb[2 * k] = std::min(bb[2 * k], a.terms[m].box[2 * k]);
+        bb[2 * k + 1] = std::max(bb[2 * k + 1], a.terms[m].box[2 * k + 1]);
+      }
+    for (int k = 0; k < 6; ++k) a.bbox[k] = bb[k];
+    obj = a;
+    obj_set = true;
+  }
+
+  void set_design(const double* g, size_t n) {
+    if ((long long)n != N) throw Error{LBMX_ERR_ARG, "design vector length != number of nodes"};
+    std::copy(g, g + n, gamma_host.begin());
+    gamma_dirty = true;
+  }
+
+  // ---- device preparation ----
+  void upload_static() {
+    if (gamma_dirty) {
+      gamma_dev.ensure((size_t)N * prec_gamma_bytes(d.precision));
+      if (d.precision == LBMX_FP64) {
+        HIP_OK(hipMemcpyAsync(gamma_dev.p, gamma_host.data(), N * 8, hipMemcpyHostToDevice, stream));
+      } else {
+        std::vector<float> gf(gamma_host.begin(), gamma_host.end());
+        HIP_OK(hipMemcpyAsync(gamma_dev.p, gf.data(), N * 4, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+      }
+      gamma_dirty = false;
+    }
+    if (bc_dirty) {
+      bc_dev.ensure(std::max<size_t>(1, bcs.size()) * sizeof(DevBC));
+      if (!bcs.empty())
+        HIP_OK(hipMemcpyAsync(bc_dev.p, bcs.data(), bcs.size() * sizeof(DevBC), hipMemcpyHostToDevice, stream));
+      HIP_OK(hipStreamSynchronize(stream));
+      bc_dirty = false;
+    }
+  }
+
+  void ensure_pool(size_t count) {
+    while (pool.size() < count) {
+      pool.emplace_back(new DeviceBuffer());
+      pool.back()->ensure(state_bytes());
+    }
+  }
+
+  FwdArgs fwd_args(const void* src, void* dst, int tnew, bool with_obj) {
+    FwdArgs a{};
+    a.src = src;
+    a.dst = dst;
+    a.gamma = gamma_dev.p;
+    a.bcs = static_cast<const DevBC*>(bc_dev.p);
+    a.nbc = (int)bcs.size();
+    a.t = tnew;
+    a.mode = 0;
+    a.g = geom();
+    a.m = params();
+    a.obj = obj;
+    a.obj.acc = (with_obj && obj_set && obj.nterm > 0) ? static_cast<double*>(acc_dev.p) : nullptr;
+    return a;
+  }
+
+  hipError_t run_fwd(const FwdArgs& a) {
+    switch (d.lattice) {
+      case LBMX_D2Q9: return launch_fwd_d2q9(d.collision, d.precision, a, stream);
+      case LBMX_D3Q19: return launch_fwd_d3q19(d.collision, d.precision, a, stream);
+      default: return launch_fwd_thermal(d.collision, d.precision, a, stream);
+    }
+  }
+  hipError_t run_init(const FwdArgs& a, const double* macros) {
+    switch (d.lattice) {
+      case LBMX_D2Q9: return launch_init_d2q9(d.collision, d.precision, a, macros, init_u, init_rho, init_T, stream);
+      case LBMX_D3Q19: return launch_init_d3q19(d.collision, d.precision, a, macros, init_u, init_rho, init_T, stream);
+      default: return launch_init_thermal(d.collision, d.precision, a, macros, init_u, init_rho, init_T, stream);
+    }
+  }
+  hipError_t run_adj(const AdjArgs& a) {
+    switch (d.lattice) {
+      case LBMX_D2Q9: return launch_adj_d2q9(d.collision, d.precision, a, stream);
+      case LBMX_D3Q19: return launch_adj_d3q19(d.collision, d.precision, a, stream);
+      default: return launch_adj_thermal(d.collision, d.precision, a, stream);
+    }
+  }
+
+  void prepare() {
+    touched = true;
+    upload_static();
+    ensure_pool(2);
+    acc_dev.ensure((size_t)grid_blocks() * sizeof(double));
+    red_dev.ensure(sizeof(double));
+  }
+
+  // ---- forward API ----
+  void init_equilibrium(double rho0, const double* u0, double T0, const double* macros_host, size_t n) {
+    prepare();
+    init_rho = rho0;
+    for (int k = 0; k < 3; ++k) init_u[k] = u0 ? u0[k] : 0.0;
+    init_T = T0;
+    DeviceBuffer mac;
+    const double* mac_dev = nullptr;
+    if (macros_host) {
+      if ((long long)n != 5 * N) throw Error{LBMX_ERR_ARG, "macro field must hold 5 values per node"};
+      mac.ensure(n * sizeof(double));
+      HIP_OK(hipMemcpyAsync(mac.p, macros_host, n * 8, hipMemcpyHostToDevice, stream));
+      mac_dev = static_cast<const double*>(mac.p);
+      raw_aos.ensure((size_t)N * Q * 8);
+    }
+    cur = pool[0]->p;
+    prev = pool[1]->p;
+    FwdArgs a = fwd_args(nullptr, cur, 0, false);
+    a.aos_out = macros_host ? raw_aos.p : nullptr;
+    HIP_OK(run_init(a, mac_dev));
+    if (macros_host) HIP_OK(hipStreamSynchronize(stream));
+    t = 0;
+    state_kind = macros_host ? 2 : 1;
+  }
+
+  void require_state() const {
+    if (state_kind < 0) throw Error{LBMX_ERR_STATE, "state not initialised (call lbmx_init_equilibrium)"};
+  }
+
+  void zero_acc() { zero_async(acc_dev.p, (size_t)grid_blocks() * sizeof(double), stream); }
+
+  double reduce_acc() {
+    std::vector<double> h(grid_blocks());
+    HIP_OK(hipMemcpyAsync(h.data(), acc_dev.p, h.size() * 8, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    double s = 0;
+    for (double v : h) s += v;  // fixed order: deterministic
+    return s;
+  }
+
+  void forward(int nsteps, double* J) {
+    require_state();
+    if (nsteps < 0) throw Error{LBMX_ERR_ARG, "nsteps must be >= 0"};
+    prepare();
+    bool with_obj = J && obj_set && obj.nterm > 0;
+    if (with_obj) zero_acc();
+    auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < nsteps; ++s) {
+      FwdArgs a = fwd_args(cur, prev, t + 1, with_obj);
+      HIP_OK(run_fwd(a));
+      std::swap(cur, prev);
+      ++t;
+      state_kind = 0;
+    }
+    stats.fwd_sweeps += nsteps;
+    double Jv = with_obj ? reduce_acc() : 0.0;
+    HIP_OK(hipStreamSynchronize(stream));
+    stats.forward_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (J) *J = Jv;
+  }
+
+  void get_state(double* f, size_t n) {
+    require_state();
+    if ((long long)n != N * Q) throw Error{LBMX_ERR_ARG, "state length != nodes * Q"};
+    prepare();
+    if (state_kind == 2) {
+      HIP_OK(hipMemcpy(f, raw_aos.p, n * 8, hipMemcpyDeviceToHost));
+      return;
+    }
+    DeviceBuffer out;
+    out.ensure(n * 8);
+    if (state_kind == 1) {
+      FwdArgs a = fwd_args(nullptr, nullptr, 0, false);
+      a.aos_out = out.p;
+      HIP_OK(run_init(a, nullptr));
+    } else {
+      FwdArgs a = fwd_args(prev, nullptr, t, false);
+      a.mode = 1;
+      a.aos_out = out.p;
+      HIP_OK(run_fwd(a));
+    }
+    HIP_OK(hipStreamSynchronize(stream));
+    HIP_OK(hipMemcpy(f, out.p, n * 8, hipMemcpyDeviceToHost));
+  }
+
+  void set_state(const double* f, size_t n, int timestep) {
+    if ((long long)n != N * Q) throw Error{LBMX_ERR_ARG, "state length != nodes * Q"};
+    prepare();
+    raw_aos.ensure(n * 8);
+    HIP_OK(hipMemcpyAsync(raw_aos.p, f, n * 8, hipMemcpyHostToDevice, stream));
+    cur = pool[0]->p;
+    prev = pool[1]->p;
+    FwdArgs a = fwd_args(raw_aos.p, cur, timestep, false);
+    a.mode = 2;
+    HIP_OK(run_fwd(a));
+    HIP_OK(hipStreamSynchronize(stream));
+    t = timestep;
+    state_kind = 2;
+  }
+
+  // ---- gradient: forward with checkpoints, reverse sweep with recomputation ----
+  void gradient(int nsteps, double rho0, const double* u0, double T0, double* J, double* sens, size_t n) {
+    if (!inc) throw Error{LBMX_ERR_SUP, "no adjoint for the standard (compressible) operators"};
+    if (nsteps < 1) throw Error{LBMX_ERR_ARG, "nsteps must be >= 1"};
+    if (sens && (long long)n != N) throw Error{LBMX_ERR_ARG, "sensitivity length != nodes"};
+    prepare();
+    const bool naive = nslots <= 0;
+    const int slots = naive ? nsteps : std::min(nslots, std::max(2, nsteps));
+    ensure_pool(2 + slots);
+    std::vector<void*> slotbuf(slots);
+    for (int s = 0; s < slots; ++s) slotbuf[s] = pool[2 + s]->p;
+    void* work[2] = {pool[0]->p, pool[1]->p};
+    auto other_work = [&](const void* src) -> void*& { return src == work[0] ? work[1] : work[0]; };
+
+    lam_a.ensure(lam_bytes());
+    lam_b.ensure(lam_bytes());
+    sens_dev.ensure((size_t)N * 8);
+
+    auto tf0 = std::chrono::steady_clock::now();
+    long long ncopies = 0, nrecompute = 0;
+    // initial state p_0 = C(f_eq)
+    init_rho = rho0;
+    for (int k = 0; k < 3; ++k) init_u[k] = u0 ? u0[k] : 0.0;
+    init_T = T0;
+    {
+      FwdArgs a = fwd_args(nullptr, work[0], 0, false);
+      HIP_OK(run_init(a, nullptr));
+    }
+    const bool with_obj = obj_set && obj.nterm > 0;
+    if (with_obj) zero_acc();
+    std::unique_ptr<WangSchedule> sched;
+    if (!naive) sched.reset(new WangSchedule(slots));
+    // place p_0
+    void* src = work[0];
+    auto store = [&](int step, void*& holder) -> void* {
+      int s = naive ? step : sched->make_checkpoint(step);
+      std::swap(slotbuf[s], holder);  // the slot now owns the state; holder gets a spare buffer
+      return slotbuf[s];
+    };
+    src = store(0, work[0]);
+    for (int k = 1; k <= nsteps; ++k) {
+      void*& dst = other_work(src);
+      FwdArgs a = fwd_args(src, dst, k, with_obj);
+      HIP_OK(run_fwd(a));
+      if (k < nsteps) src = store(k, dst);
+      else src = dst;
+    }
+    stats.fwd_sweeps += nsteps;
+    double Jv = with_obj ? reduce_acc() : 0.0;
+    auto tf1 = std::chrono::steady_clock::now();
+
+    // reverse
+    zero_async(lam_a.p, lam_bytes(), stream);
+    zero_async(sens_dev.p, (size_t)N * 8, stream);
+    void* lin = lam_a.p;
+    void* lout = lam_b.p;
+    ObjArgs oa = obj;
+    oa.acc = nullptr;
+    if (!with_obj) oa.nterm = 0;
+    for (int step = nsteps; step >= 1; --step) {
+      const int index = step - 1;  // the adjoint sweep at `step` needs p_{step-1}
+      const void* fsrc = nullptr;
+      if (naive) {
+        fsrc = slotbuf[index];
+      } else {
+        WangSchedule::Plan plan = sched->get_forward_solution(index);
+        if (plan.stored()) {
+          fsrc = slotbuf[plan.from_slot];
+        } else {
+          const void* s2 = slotbuf[plan.from_slot];
+          size_t next = 0;
+          for (int c = plan.from_step + 1; c <= plan.index; ++c) {
+            void*& dst = other_work(s2);
+            FwdArgs a = fwd_args(s2, dst, c, false);
+            HIP_OK(run_fwd(a));
+            ++nrecompute;
+            if (next < plan.stores.size() && plan.stores[next].first == c) {
+              int s = plan.stores[next].second;
+              std::swap(slotbuf[s], dst);
+              s2 = slotbuf[s];
+              ++next;
+            } else {
+              s2 = dst;
+            }
+          }
+          fsrc = s2;
+        }
+      }
+      AdjArgs a{};
+      a.fwd_src = fsrc;
+      a.lam_in = lin;
+      a.lam_out = lout;
+      a.gamma = gamma_dev.p;
+      a.sens = static_cast<double*>(sens_dev.p);
+      a.bcs = static_cast<const DevBC*>(bc_dev.p);
+      a.nbc = (int)bcs.size();
+      a.t = step;
+      a.g = geom();
+      a.m = params();
+      a.obj = oa;
+      HIP_OK(run_adj(a));
+      std::swap(lin, lout);
+      if (!naive) sched->release();
+    }
+    HIP_OK(hipStreamSynchronize(stream));
+    auto tr1 = std::chrono::steady_clock::now();
+    stats.adj_sweeps += nsteps;
+    stats.recompute_sweeps += nrecompute;
+    stats.fwd_sweeps += nrecompute;
+    stats.checkpoint_copies += ncopies;
+    stats.forward_phase_ms = std::chrono::duration<double, std::milli>(tf1 - tf0).count();
+    stats.reverse_phase_ms = std::chrono::duration<double, std::milli>(tr1 - tf1).count();
+    if (lin != lam_a.p) std::swap(lam_a.p, lam_b.p);  // final adjoint in lam_a
+    grad_valid = true;
+    state_kind = -1;  // the forward buffers were reused
+    if (J) *J = Jv;
+    if (sens) HIP_OK(hipMemcpy(sens, sens_dev.p, (size_t)N * 8, hipMemcpyDeviceToHost));
+  }
+
+  void get_sensitivity(double* s, size_t n) {
+    if (!grad_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
+    if ((long long)n != N) throw Error{LBMX_ERR_ARG, "sensitivity length != nodes"};
+    HIP_OK(hipMemcpy(s, sens_dev.p, n * 8, hipMemcpyDeviceToHost));
+  }
+
+  void get_adjoint(double* out, size_t n) {
+    if (!grad_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
+    if ((long long)n != N * Q) throw Error{LBMX_ERR_ARG, "adjoint length != nodes * Q"};
+    size_t eb = prec_lam_bytes(d.precision);
+    std::vector<char> h((size_t)N * Q * eb);
+    HIP_OK(hipMemcpy(h.data(), lam_a.p, h.size(), hipMemcpyDeviceToHost));
+    for (long long node = 0; node < N; ++node)
+      for (int q = 0; q < Q; ++q) {
+        size_t i = (size_t)q * N + node;
+        out[node * Q + q] = eb == 8 ? reinterpret_cast<double*>(h.data())[i] : reinterpret_cast<float*>(h.data())[i];
+      }
+  }
+
+  size_t device_bytes() const {
+    size_t b = gamma_dev.bytes + bc_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
+               lam_b.bytes + raw_aos.bytes;
+    for (auto& p : pool) b += p->bytes;
+    return b;
+  }
+};
+
+}  // namespace lbmx
+
+using lbmx::Engine;
+using lbmx::Error;
+
+struct lbmx_solver {
+  std::unique_ptr<Engine> e;
+};
+
+template <class F>
+static lbmx_status guarded(F&& f) {
+  try {
+    f();
+    return LBMX_OK;
+  } catch (const Error& e) {
+    lbmx::g_last_error = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    lbmx::g_last_error = "host allocation failed";
+    return LBMX_ERR_MEM;
+  } catch (const std::exception& e) {
+    lbmx::g_last_error = e.what();
+    return LBMX_ERR_STATE;
+  }
+}
+
+#define CHECK_HANDLE(s)                                      \
+  if (!(s) || !(s)->e) {                                     \
+    lbmx::g_last_error = "null solver handle";               \
+    return LBMX_ERR_ARG;                                     \
+  }
+
+extern "C" {
+
+lbmx_status lbmx_create(const lbmx_desc* desc, lbmx_solver** out) {
+  if (!desc || !out) {
+    lbmx::g_last_error = "null argument";
+    return LBMX_ERR_ARG;
+  }
+  return guarded([&] {
+    auto* s = new lbmx_solver;
+    try {
+      s->e.reset(new Engine(*desc));
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+lbmx_status lbmx_destroy(lbmx_solver* s) {
+  if (!s) return LBMX_OK;
+  delete s;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_set_stream(lbmx_solver* s, void* stream) {
+  CHECK_HANDLE(s);
+  s->e->stream = static_cast<hipStream_t>(stream);
+  return LBMX_OK;
+}
+
+const char* lbmx_last_error(void) { return lbmx::g_last_error.c_str(); }
+
+lbmx_status lbmx_add_boundary(lbmx_solver* s, const lbmx_box* box, const lbmx_bc_desc* bc) {
+  CHECK_HANDLE(s);
+  if (!box || !bc) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->add_boundary(*box, *bc); });
+}
+
+lbmx_status lbmx_set_design(lbmx_solver* s, const double* gamma, size_t n) {
+  CHECK_HANDLE(s);
+  if (!gamma) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->set_design(gamma, n); });
+}
+
+lbmx_status lbmx_get_design(lbmx_solver* s, double* gamma, size_t n) {
+  CHECK_HANDLE(s);
+  if (!gamma) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] {
+    if ((long long)n != s->e->N) throw Error{LBMX_ERR_ARG, "design vector length != number of nodes"};
+    std::copy(s->e->gamma_host.begin(), s->e->gamma_host.end(), gamma);
+  });
+}
+
+lbmx_status lbmx_set_objective(lbmx_solver* s, const lbmx_objective* obj) {
+  CHECK_HANDLE(s);
+  if (!obj) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->set_objective(*obj); });
+}
+
+lbmx_status lbmx_init_equilibrium(lbmx_solver* s, double rho0, const double u0[3], double T0) {
+  CHECK_HANDLE(s);
+  return guarded([&] { s->e->init_equilibrium(rho0, u0, T0, nullptr, 0); });
+}
+
+lbmx_status lbmx_init_equilibrium_field(lbmx_solver* s, const double* macros, size_t n) {
+  CHECK_HANDLE(s);
+  if (!macros) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->init_equilibrium(1.0, nullptr, 0.0, macros, n); });
+}
+
+lbmx_status lbmx_forward(lbmx_solver* s, int nsteps, double* J) {
+  CHECK_HANDLE(s);
+  return guarded([&] { s->e->forward(nsteps, J); });
+}
+
+lbmx_status lbmx_get_state(lbmx_solver* s, double* f, size_t n) {
+  CHECK_HANDLE(s);
+  if (!f) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->get_state(f, n); });
+}
+
+lbmx_status lbmx_set_state(lbmx_solver* s, const double* f, size_t n, int timestep) {
+  CHECK_HANDLE(s);
+  if (!f) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->set_state(f, n, timestep); });
+}
+
+lbmx_status lbmx_get_macros(lbmx_solver* s, double* macros, size_t n) {
+  CHECK_HANDLE(s);
+  if (!macros) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] {
+    Engine& e = *s->e;
+    if ((long long)n != 5 * e.N) throw Error{LBMX_ERR_ARG, "macro array must hold 5 values per node"};
+    std::vector<double> f((size_t)e.N * e.Q);
+    e.get_state(f.data(), f.size());
+    static const int ex9[9] = {0, 1, 0, -1, -1, -1, 0, 1, 1}, ey9[9] = {0, -1, -1, -1, 0, 1, 1, 1, 0};
+    static const int ex19[19] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+    static const int ey19[19] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+    static const int ez19[19] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+    const int QF = e.D == 3 ? 19 : 9;
+    for (long long node = 0; node < e.N; ++node) {
+      const double* fn = &f[node * e.Q];
+      double r = 0, jx = 0, jy = 0, jz = 0, T = 0;
+      for (int q = 0; q < QF; ++q) {
+        r += fn[q];
+        jx += (e.D == 3 ? ex19[q] : ex9[q]) * fn[q];
+        jy += (e.D == 3 ? ey19[q] : ey9[q]) * fn[q];
+        if (e.D == 3) jz += ez19[q] * fn[q];
+      }
+      for (int q = QF; q < e.Q; ++q) T += fn[q];
+      if (!e.inc) { jx /= r; jy /= r; jz /= r; }
+      macros[5 * node] = r; macros[5 * node + 1] = jx; macros[5 * node + 2] = jy;
+      macros[5 * node + 3] = jz; macros[5 * node + 4] = T;
+    }
+  });
+}
+
+lbmx_status lbmx_get_timestep(lbmx_solver* s, int* t) {
+  CHECK_HANDLE(s);
+  if (!t) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  *t = s->e->t;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_checkpoint_config(lbmx_solver* s, int nslots) {
+  CHECK_HANDLE(s);
+  if (nslots == 1) { lbmx::g_last_error = "dynamic checkpointing needs at least 2 slots"; return LBMX_ERR_ARG; }
+  s->e->nslots = nslots;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_gradient(lbmx_solver* s, int nsteps, double rho0, const double u0[3], double T0, double* J,
+                          double* sens, size_t n) {
+  CHECK_HANDLE(s);
+  return guarded([&] { s->e->gradient(nsteps, rho0, u0, T0, J, sens, n); });
+}
+
+lbmx_status lbmx_get_sensitivity(lbmx_solver* s, double* sens, size_t n) {
+  CHECK_HANDLE(s);
+  if (!sens) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->get_sensitivity(sens, n); });
+}
+
+lbmx_status lbmx_get_adjoint(lbmx_solver* s, double* adj, size_t n) {
+  CHECK_HANDLE(s);
+  if (!adj) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->get_adjoint(adj, n); });
+}
+
+lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, int* count) {
+  CHECK_HANDLE(s);
+  if (!count) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  const auto& b = s->e->bcs;
+  *count = (int)b.size();
+  for (int k = 0; k < (int)b.size() && k < max_pieces && out; ++k) {
+    int* o = out + 10 * k;
+    o[0] = b[k].type;
+    for (int c = 0; c < 3; ++c) o[1 + c] = b[k].nrm[c];
+    for (int c = 0; c < 6; ++c) o[4 + c] = b[k].box[c];
+  }
+  return LBMX_OK;
+}
+
+// Replays the schedule of Engine::gradient with integer-labelled buffers: a buffer holds the
+// step of the state it contains; advancing from label k writes label k+1.
+lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances, int* max_slots) {
+  if (!advances || !max_slots || nsteps < 1 || nslots < 2) {
+    lbmx::g_last_error = "bad argument";
+    return LBMX_ERR_ARG;
+  }
+  return guarded([&] {
+    const int slots = std::min(nslots, std::max(2, nsteps));
+    lbmx::WangSchedule sched(slots);
+    std::vector<int> bufs(slots + 2, -1);  // label per physical buffer
+    std::vector<int> slotbuf(slots);
+    for (int k = 0; k < slots; ++k) slotbuf[k] = 2 + k;
+    int work[2] = {0, 1};
+    auto other = [&](int src) -> int& { return src == work[0] ? work[1] : work[0]; };
+    bufs[work[0]] = 0;
+    int src;
+    {
+      int s0 = sched.make_checkpoint(0);
+      std::swap(slotbuf[s0], work[0]);
+      src = slotbuf[s0];
+    }
+    int maxused = 1;
+    for (int k = 1; k < nsteps; ++k) {
+      int& dst = other(src);
+      bufs[dst] = bufs[src] + 1;
+      int s = sched.make_checkpoint(k);
+      std::swap(slotbuf[s], dst);
+      src = slotbuf[s];
+      maxused = std::max(maxused, slots - sched.free_slots());
+    }
+    long long adv = 0;
+    for (int step = nsteps; step >= 1; --step) {
+      int index = step - 1;
+      auto plan = sched.get_forward_solution(index);
+      int fsrc;
+      if (plan.stored()) {
+        fsrc = slotbuf[plan.from_slot];
+      } else {
+        int s2 = slotbuf[plan.from_slot];
+        if (bufs[s2] != plan.from_step) throw Error{LBMX_ERR_STATE, "restart buffer holds the wrong state"};
+        size_t next = 0;
+        for (int c = plan.from_step + 1; c <= plan.index; ++c) {
+          int& dst = other(s2);
+          bufs[dst] = bufs[s2] + 1;
+          ++adv;
+          if (next < plan.stores.size() && plan.stores[next].first == c) {
+            std::swap(slotbuf[plan.stores[next].second], dst);
+            s2 = slotbuf[plan.stores[next].second];
+            ++next;
+          } else {
+            s2 = dst;
+          }
+        }
+        fsrc = s2;
+      }
+      if (bufs[fsrc] != index) throw Error{LBMX_ERR_STATE, "delivered the wrong state at step " + std::to_string(step)};
+      maxused = std::max(maxused, slots - sched.free_slots());
+      sched.release();
+    }
+    *advances = adv;
+    *max_slots = maxused;
+  });
+}
+
+lbmx_status lbmx_get_stats(lbmx_solver* s, lbmx_stats* st) {
+  CHECK_HANDLE(s);
+  if (!st) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  *st = s->e->stats;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_device_bytes(lbmx_solver* s, size_t* bytes) {
+  CHECK_HANDLE(s);
+  if (!bytes) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  *bytes = s->e->device_bytes();
+  return LBMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,8 @@
+// D2Q9 sweeps: incompressible BGK / MRT (forward + adjoint), standard BGK (forward).
+#include "lbmx_kernels.hpp"
+#include "lbmx_launch.h"
+#define LAT LatD2Q9
+#define SUFFIX d2q9
+#define FWD_OPS X(C_BGK) X(C_MRT) X(C_BGK_STD)
+#define ADJ_OPS X(C_BGK) X(C_MRT)
+#include "lbmx_dispatch.inc"

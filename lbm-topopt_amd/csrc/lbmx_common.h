@@ -1,0 +1,95 @@
+// Structures shared by the host engine and the HIP kernels (no device code here).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace lbmx {
+
+constexpr int kMaxTerms = 4;
+constexpr int kTileX = 64;   // threads along x per workgroup (one wave)
+constexpr int kTileY = 4;    // waves per workgroup, stacked along y
+
+// One boundary piece (a box on one domain face/edge/corner with its outward normal),
+// applied on the device in insertion order.
+struct DevBC {
+  int type;
+  int nrm[3];
+  int box[6];  // x0 x1 y0 y1 z0 z1 (inclusive)
+  int profile;
+  int time_mode;
+  int inc;     // 1: incompressible variant of the condition
+  int pad;
+  double value;
+  double dir[3];
+  double center[2];
+  double radius[2];
+  double t_period, t_amp, t_offset, t_phase;
+};
+
+// One functional term: weight * sum over box of g_kind(state); kinds:
+// 0 rho, 1 u_x, 2 total-pressure flux (p + |u|^2/2) u_x, 3 u_x T.
+struct DevTerm {
+  int box[6];
+  int kind;
+  int pad;
+  double weight;
+};
+
+// Model constants passed by value to every kernel.
+struct ModelParams {
+  double omega;
+  double alpha_max;
+  double interp_q;
+  double body[3];
+  double tdiff, ramp_p, ramp_ck;
+  int design;
+  int pad;
+};
+
+struct Geometry {
+  int nx, ny, nz;
+  int pad;
+  long long N;  // nodes
+};
+
+struct ObjArgs {
+  int nterm;
+  int bbox[6];        // union of term boxes (skip test)
+  int pad;
+  double time_const;
+  DevTerm terms[kMaxTerms];
+  double* acc;        // per-workgroup running sums (FP64), null = off
+};
+
+// Forward sweep: dst = C(B_t(S(src))) ; objective of B_t(S(src)) accumulated.
+struct FwdArgs {
+  const void* src;
+  void* dst;
+  const void* gamma;
+  const DevBC* bcs;
+  int nbc;
+  int t;              // time of the new state (boundary values)
+  int mode;           // 0 full sweep, 1 stream+BC only (no collision), 2 collide only (src is AoS FP64 state)
+  int pad;
+  void* aos_out;      // mode 1: pre-collision state written AoS FP64
+  Geometry g;
+  ModelParams m;
+  ObjArgs obj;
+};
+
+// Adjoint sweep at reverse step t: reads p_{t-1} (fwd_src), nu_t (lam_in), writes nu_{t-1}.
+struct AdjArgs {
+  const void* fwd_src;
+  const void* lam_in;
+  void* lam_out;
+  const void* gamma;
+  double* sens;
+  const DevBC* bcs;
+  int nbc;
+  int t;
+  Geometry g;
+  ModelParams m;
+  ObjArgs obj;
+};
+
+}  // namespace lbmx

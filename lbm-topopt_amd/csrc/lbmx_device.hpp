@@ -1,0 +1,1052 @@
+// Device-side lattice Boltzmann operators for CDNA4 (gfx950): lattices, storage codecs,
+// collision operators with their exact reverse (transpose) operators, boundary maps with
+// their transposes, and functional terms. Everything is header-only and compile-time
+// specialised: moment matrices are constexpr tables, so after unrolling every zero entry
+// disappears and the work equals a hand-sparse implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lbmx_common.h"
+#include "lbmx.h"
+
+namespace lbmx {
+
+// ---------------------------------------------------------------------------------------
+// Lattices (velocity order of src/latticeBoltzmann/lattices2d.hh:16-84 and
+// lattices3d.hh:34-53: direction q and q+HALF are opposite).
+// ---------------------------------------------------------------------------------------
+struct LatD2Q9 {
+  static constexpr int D = 2, Q = 9, QF = 9, QT = 0, HALF = 4;
+  static constexpr int ex[Q] = {0, 1, 0, -1, -1, -1, 0, 1, 1};
+  static constexpr int ey[Q] = {0, -1, -1, -1, 0, 1, 1, 1, 0};
+  static constexpr int ez[Q] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int opp[Q] = {0, 5, 6, 7, 8, 1, 2, 3, 4};
+  static constexpr double w[Q] = {4. / 9., 1. / 36., 1. / 9., 1. / 36., 1. / 9.,
+                                  1. / 36., 1. / 9., 1. / 36., 1. / 9.};
+};
+
+struct LatD3Q19 {
+  static constexpr int D = 3, Q = 19, QF = 19, QT = 0, HALF = 9;
+  static constexpr int ex[Q] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+  static constexpr int ey[Q] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+  static constexpr int ez[Q] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+  static constexpr int opp[Q] = {0, 10, 11, 12, 13, 14, 15, 16, 17, 18, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+  static constexpr double w[Q] = {1. / 3., 1. / 18., 1. / 18., 1. / 18., 1. / 36., 1. / 36., 1. / 36.,
+                                  1. / 36., 1. / 36., 1. / 36., 1. / 18., 1. / 18., 1. / 18.,
+                                  1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+};
+
+// D2Q9 flow populations followed by the D2Q5 temperature populations of one node, the
+// interleave of ThermalDualCollision (src/dynamics/thermalDualCollision.hh:57-64).
+struct LatD2Q9D2Q5 {
+  static constexpr int D = 2, Q = 14, QF = 9, QT = 5, HALF = 4;
+  static constexpr int ex[Q] = {0, 1, 0, -1, -1, -1, 0, 1, 1, 0, 0, -1, 0, 1};
+  static constexpr int ey[Q] = {0, -1, -1, -1, 0, 1, 1, 1, 0, 0, -1, 0, 1, 0};
+  static constexpr int ez[Q] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int opp[Q] = {0, 5, 6, 7, 8, 1, 2, 3, 4, 9, 12, 13, 10, 11};
+  static constexpr double w[Q] = {4. / 9., 1. / 36., 1. / 9., 1. / 36., 1. / 9., 1. / 36., 1. / 9.,
+                                  1. / 36., 1. / 9., 1. / 3., 1. / 6., 1. / 6., 1. / 6., 1. / 6.};
+};
+
+// ---------------------------------------------------------------------------------------
+// Moment bases. Row k: polynomial of the velocity, relaxation kind (0 conserved, 1 the
+// viscous rate omega, 2 a constant s), and equilibrium m_eq = a rho + b.v + v^T C v / rho_s.
+// D2Q9: Lallemand & Luo (2000) with the rates of src/dynamics/incompressibleMRT2d.hh:88-90.
+// D3Q19: d'Humieres et al. (2002) with MRT3d_Compile_constants (src/dynamics/MRT3d.hh:9-22).
+// D2Q5: ThermalMRT2d<D2Q5_TMRT> (src/dynamics/thermalMRT2d.hh:50-101).
+// ---------------------------------------------------------------------------------------
+template <int Q>
+struct Basis {
+  double M[Q][Q];
+  double invn2[Q];
+  int rate[Q];
+  double s[Q];
+  double a[Q];
+  double b[Q][3];
+  double C[Q][3][3];
+  constexpr Basis() : M{}, invn2{}, rate{}, s{}, a{}, b{}, C{} {}
+  constexpr void finish() {
+    for (int k = 0; k < Q; ++k) {
+      double n = 0;
+      for (int q = 0; q < Q; ++q) n += M[k][q] * M[k][q];
+      invn2[k] = 1.0 / n;
+    }
+  }
+};
+
+constexpr Basis<9> make_basis_d2q9() {
+  Basis<9> B;
+  for (int q = 0; q < 9; ++q) {
+    double cx = LatD2Q9::ex[q], cy = LatD2Q9::ey[q], c2 = cx * cx + cy * cy;
+    B.M[0][q] = 1;
+    B.M[1][q] = cx;
+    B.M[2][q] = cy;
+    B.M[3][q] = -4 + 3 * c2;
+    B.M[4][q] = 4 - 10.5 * c2 + 4.5 * c2 * c2;
+    B.M[5][q] = (-5 + 3 * c2) * cx;
+    B.M[6][q] = (-5 + 3 * c2) * cy;
+    B.M[7][q] = cx * cx - cy * cy;
+    B.M[8][q] = cx * cy;
+  }
+  B.a[0] = 1;
+  B.b[1][0] = 1;
+  B.b[2][1] = 1;
+  B.a[3] = -2; B.C[3][0][0] = 3; B.C[3][1][1] = 3; B.rate[3] = 2; B.s[3] = 1.4;
+  B.a[4] = 1; B.C[4][0][0] = -3; B.C[4][1][1] = -3; B.rate[4] = 2; B.s[4] = 1.4;
+  B.b[5][0] = -1; B.rate[5] = 2; B.s[5] = 1.2;
+  B.b[6][1] = -1; B.rate[6] = 2; B.s[6] = 1.2;
+  B.C[7][0][0] = 1; B.C[7][1][1] = -1; B.rate[7] = 1;
+  B.C[8][0][1] = 0.5; B.C[8][1][0] = 0.5; B.rate[8] = 1;
+  B.finish();
+  return B;
+}
+
+constexpr Basis<19> make_basis_d3q19() {
+  Basis<19> B;
+  for (int q = 0; q < 19; ++q) {
+    double cx = LatD3Q19::ex[q], cy = LatD3Q19::ey[q], cz = LatD3Q19::ez[q];
+    double c2 = cx * cx + cy * cy + cz * cz;
+    B.M[0][q] = 1;
+    B.M[1][q] = cx;
+    B.M[2][q] = cy;
+    B.M[3][q] = cz;
+    B.M[4][q] = 19 * c2 - 30;
+    B.M[5][q] = (21 * c2 * c2 - 53 * c2 + 24) / 2;
+    B.M[6][q] = (5 * c2 - 9) * cx;
+    B.M[7][q] = (5 * c2 - 9) * cy;
+    B.M[8][q] = (5 * c2 - 9) * cz;
+    B.M[9][q] = 3 * cx * cx - c2;
+    B.M[10][q] = (3 * c2 - 5) * (3 * cx * cx - c2);
+    B.M[11][q] = cy * cy - cz * cz;
+    B.M[12][q] = (3 * c2 - 5) * (cy * cy - cz * cz);
+    B.M[13][q] = cx * cy;
+    B.M[14][q] = cy * cz;
+    B.M[15][q] = cx * cz;
+    B.M[16][q] = (cy * cy - cz * cz) * cx;
+    B.M[17][q] = (cz * cz - cx * cx) * cy;
+    B.M[18][q] = (cx * cx - cy * cy) * cz;
+  }
+  const double wxx = -0.5;
+  B.a[0] = 1;
+  for (int d = 0; d < 3; ++d) B.b[1 + d][d] = 1;
+  B.a[4] = -11; for (int d = 0; d < 3; ++d) B.C[4][d][d] = 19; B.rate[4] = 2; B.s[4] = 1.19;
+  B.a[5] = 3.; for (int d = 0; d < 3; ++d) B.C[5][d][d] = -5.5; B.rate[5] = 2; B.s[5] = 1.4;
+  for (int d = 0; d < 3; ++d) { B.b[6 + d][d] = -2. / 3.; B.rate[6 + d] = 2; B.s[6 + d] = 1.2; }
+  B.C[9][0][0] = 2; B.C[9][1][1] = -1; B.C[9][2][2] = -1; B.rate[9] = 1;
+  B.C[10][0][0] = 2 * wxx; B.C[10][1][1] = -wxx; B.C[10][2][2] = -wxx; B.rate[10] = 2; B.s[10] = 1.4;
+  B.C[11][1][1] = 1; B.C[11][2][2] = -1; B.rate[11] = 1;
+  B.C[12][1][1] = wxx; B.C[12][2][2] = -wxx; B.rate[12] = 2; B.s[12] = 1.4;
+  B.C[13][0][1] = 0.5; B.C[13][1][0] = 0.5; B.rate[13] = 1;
+  B.C[14][1][2] = 0.5; B.C[14][2][1] = 0.5; B.rate[14] = 1;
+  B.C[15][0][2] = 0.5; B.C[15][2][0] = 0.5; B.rate[15] = 1;
+  for (int k = 16; k < 19; ++k) { B.rate[k] = 2; B.s[k] = 1.4; }
+  B.finish();
+  return B;
+}
+
+// thermal rows: T, q_x, q_y, e, p_xx ; m_eq = T (a + b.u)
+constexpr Basis<5> make_basis_d2q5() {
+  Basis<5> B;
+  constexpr int ex[5] = {0, 0, -1, 0, 1};
+  constexpr int ey[5] = {0, -1, 0, 1, 0};
+  for (int q = 0; q < 5; ++q) {
+    double cx = ex[q], cy = ey[q], c2 = cx * cx + cy * cy;
+    B.M[0][q] = 1;
+    B.M[1][q] = cx;
+    B.M[2][q] = cy;
+    B.M[3][q] = 5 * c2 - 4;
+    B.M[4][q] = cx * cx - cy * cy;
+  }
+  B.a[0] = 1;
+  B.b[1][0] = 1; B.rate[1] = 1;
+  B.b[2][1] = 1; B.rate[2] = 1;
+  B.a[3] = -2; B.rate[3] = 2; B.s[3] = 1.5;
+  B.rate[4] = 2; B.s[4] = 1.5;
+  B.finish();
+  return B;
+}
+
+static constexpr Basis<9> kB9 = make_basis_d2q9();
+static constexpr Basis<19> kB19 = make_basis_d3q19();
+static constexpr Basis<5> kB5 = make_basis_d2q5();
+
+template <int Q> struct BasisOf;
+template <> struct BasisOf<9> { static constexpr const Basis<9>& B = kB9; };
+template <> struct BasisOf<19> { static constexpr const Basis<19>& B = kB19; };
+
+// ---------------------------------------------------------------------------------------
+// Storage codecs. Plain forms store f; shifted forms store g = f/w_q - 1 in 16 bits so
+// that near-equilibrium values use the full mantissa, decoded as f = w_q + w_q g.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) {
+  return __uint_as_float(static_cast<unsigned int>(h) << 16);
+}
+__device__ __forceinline__ unsigned short f32_to_bf16(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return static_cast<unsigned short>(u >> 16);
+}
+
+struct PrecF64 {
+  using S = double;   // state storage
+  using T = double;   // arithmetic
+  using G = double;   // design field storage
+  using LS = double;  // adjoint storage
+  static constexpr bool shifted = false;
+  template <class L, int q> __device__ static T dec(S v) { return v; }
+  template <class L, int q> __device__ static S enc(T v) { return v; }
+};
+struct PrecF32 {
+  using S = float; using T = float; using G = float; using LS = float;
+  static constexpr bool shifted = false;
+  template <class L, int q> __device__ static T dec(S v) { return v; }
+  template <class L, int q> __device__ static S enc(T v) { return v; }
+};
+struct PrecF16 {
+  using S = _Float16; using T = float; using G = float; using LS = float;
+  static constexpr bool shifted = true;
+  template <class L, int q> __device__ static T dec(S v) {
+    constexpr float w = static_cast<float>(L::w[q]);
+    return __builtin_fmaf(w, static_cast<float>(v), w);
+  }
+  template <class L, int q> __device__ static S enc(T v) {
+    constexpr float iw = static_cast<float>(1.0 / L::w[q]);
+    return static_cast<_Float16>(__builtin_fmaf(v, iw, -1.0f));
+  }
+};
+struct PrecBF16 {
+  using S = unsigned short; using T = float; using G = float; using LS = float;
+  static constexpr bool shifted = true;
+  template <class L, int q> __device__ static T dec(S v) {
+    constexpr float w = static_cast<float>(L::w[q]);
+    return __builtin_fmaf(w, bf16_to_f32(v), w);
+  }
+  template <class L, int q> __device__ static S enc(T v) {
+    constexpr float iw = static_cast<float>(1.0 / L::w[q]);
+    return f32_to_bf16(__builtin_fmaf(v, iw, -1.0f));
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// Compile-time loop helper: body(std::integral_constant-like) for q = 0..N-1.
+// ---------------------------------------------------------------------------------------
+template <int I> struct IC { static constexpr int value = I; };
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(IC<B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Design interpolations (src/core/interpolationFunctions.hh:7-54)
+// ---------------------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T bp_interp(T q, T s) { return T(1) - s * (T(1) + q) / (q + s); }
+template <class T> __device__ __forceinline__ T bp_diff(T q, T s) {
+  T d = q + s;
+  return -(T(1) + q) / d + s * (T(1) + q) / (d * d);
+}
+template <class T> __device__ __forceinline__ T ramp_interp(T p, T ck, T s) {
+  return (s * (ck * (T(1) + p) - T(1)) + T(1)) / (ck * (T(1) + p * s));
+}
+template <class T> __device__ __forceinline__ T ramp_diff(T p, T ck, T s) {
+  T den = ck * (T(1) + p * s);
+  return (ck * (T(1) + p) - T(1)) / den - ((s * (ck * (T(1) + p) - T(1)) + T(1)) * p) / (den * (T(1) + p * s));
+}
+
+// ---------------------------------------------------------------------------------------
+// Flow collision (incompressible and standard BGK / MRT) with PBB or Brinkman coupling.
+// ---------------------------------------------------------------------------------------
+enum : int { C_BGK = LBMX_BGK, C_MRT = LBMX_MRT, C_CASC = LBMX_CASCADED,
+             C_MRT_STD = LBMX_MRT_STANDARD, C_BGK_STD = LBMX_BGK_STANDARD };
+
+template <int COLL> struct CollTraits {
+  static constexpr bool inc = !(COLL == C_MRT_STD || COLL == C_BGK_STD);
+  static constexpr bool mrt = (COLL == C_MRT || COLL == C_MRT_STD);
+};
+
+template <class L, class T>
+__device__ __forceinline__ void flow_macros(const T (&f)[L::Q], T& rho, T (&j)[3]) {
+  T r = 0, jx = 0, jy = 0, jz = 0;
+  static_for<0, L::QF>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    r += f[q];
+    if constexpr (L::ex[q] == 1) jx += f[q];
+    if constexpr (L::ex[q] == -1) jx -= f[q];
+    if constexpr (L::ey[q] == 1) jy += f[q];
+    if constexpr (L::ey[q] == -1) jy -= f[q];
+    if constexpr (L::ez[q] == 1) jz += f[q];
+    if constexpr (L::ez[q] == -1) jz -= f[q];
+  });
+  rho = r; j[0] = jx; j[1] = jy; j[2] = jz;
+}
+
+// c_q . v with compile-time velocity
+template <class L, int q, class T>
+__device__ __forceinline__ T cdot(const T (&v)[3]) {
+  T s = 0;
+  if constexpr (L::ex[q] != 0) s += T(L::ex[q]) * v[0];
+  if constexpr (L::ey[q] != 0) s += T(L::ey[q]) * v[1];
+  if constexpr (L::ez[q] != 0) s += T(L::ez[q]) * v[2];
+  return s;
+}
+
+// equilibrium moment of row k and its derivatives
+template <int Q, int k, class T>
+__device__ __forceinline__ T row_meq(T rho, const T (&v)[3], T inv_rho_s) {
+  T m = 0;
+  if constexpr (BasisOf<Q>::B.a[k] != 0) m += T(BasisOf<Q>::B.a[k]) * rho;
+  static_for<0, 3>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    if constexpr (BasisOf<Q>::B.b[k][a] != 0) m += T(BasisOf<Q>::B.b[k][a]) * v[a];
+  });
+  T quad = 0;
+  bool any = false;
+  static_for<0, 3>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    static_for<0, 3>([&](auto bc) {
+      constexpr int b = decltype(bc)::value;
+      if constexpr (BasisOf<Q>::B.C[k][a][b] != 0) { quad += T(BasisOf<Q>::B.C[k][a][b]) * v[a] * v[b]; any = true; }
+    });
+  });
+  if (any) m += quad * inv_rho_s;
+  return m;
+}
+
+template <int Q, int k>
+constexpr bool row_has_quad() {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      if (BasisOf<Q>::B.C[k][a][b] != 0) return true;
+  return false;
+}
+
+template <class T> __device__ __forceinline__ T rate_of(int kind, double s, T omega) {
+  return kind == 0 ? T(0) : (kind == 1 ? omega : T(s));
+}
+
+// Guo source with F = -alpha v:  w_q [3 (c-v).F + 9 (c.v)(c.F)]
+template <class L, int q, class T>
+__device__ __forceinline__ T guo_src(const T (&v)[3], T u2, T alpha) {
+  T cu = cdot<L, q>(v);
+  return -alpha * T(L::w[q]) * (T(3) * cu + T(9) * cu * cu - T(3) * u2);
+}
+
+// Base relaxation: f <- f - relax(f) [+ (I - S/2) source]; v is the equilibrium velocity.
+template <class L, int COLL, class T>
+__device__ __forceinline__ void base_relax(T (&f)[L::Q], T rho, const T (&v)[3], T omega, T alpha,
+                                           bool forced) {
+  constexpr bool inc = CollTraits<COLL>::inc;
+  const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  if constexpr (!CollTraits<COLL>::mrt) {
+    // BGK: f' = f (1 - omega) + omega f_eq  (src/dynamics/incompressibleBGK2d.hh:38-47)
+    const T kappa = T(1) - T(0.5) * omega;
+    static_for<0, L::QF>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T cu = cdot<L, q>(v);
+      T poly = T(3) * cu + T(4.5) * cu * cu - T(1.5) * u2;
+      T feq = inc ? T(L::w[q]) * (rho + poly) : T(L::w[q]) * rho * (T(1) + poly);
+      T out = f[q] * (T(1) - omega) + omega * feq;
+      if (forced) out += kappa * guo_src<L, q>(v, u2, alpha);
+      f[q] = out;
+    });
+  } else {
+    constexpr int Q = L::QF;
+    const T inv_rho_s = inc ? T(1) : T(1) / rho;
+    T dm[Q];
+    static_for<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      dm[k] = 0;
+      if constexpr (BasisOf<Q>::B.rate[k] != 0) {
+        T mk = 0;
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if constexpr (BasisOf<Q>::B.M[k][q] != 0) mk += T(BasisOf<Q>::B.M[k][q]) * f[q];
+        });
+        T sk = BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]);
+        dm[k] = sk * (mk - row_meq<Q, k>(rho, v, inv_rho_s)) * T(BasisOf<Q>::B.invn2[k]);
+      }
+    });
+    if (forced) {
+      T src[Q];
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        src[q] = guo_src<L, q>(v, u2, alpha);
+      });
+      static_for<0, Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        T sk = 0;
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if constexpr (BasisOf<Q>::B.M[k][q] != 0) sk += T(BasisOf<Q>::B.M[k][q]) * src[q];
+        });
+        T rk = BasisOf<Q>::B.rate[k] == 0 ? T(0) : (BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]));
+        dm[k] -= (T(1) - T(0.5) * rk) * sk * T(BasisOf<Q>::B.invn2[k]);
+      });
+    }
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T acc = f[q];
+      static_for<0, Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (BasisOf<Q>::B.M[k][q] != 0) {
+          if (forced || BasisOf<Q>::B.rate[k] != 0) acc -= T(BasisOf<Q>::B.M[k][q]) * dm[k];
+        }
+      });
+      f[q] = acc;
+    });
+  }
+}
+
+// thermal: ThermalObstacleCollision<ThermalMRT2d<D2Q5_TMRT>,Ramp> on populations QF..QF+4
+template <class T>
+__device__ __forceinline__ T thermal_omega(const ModelParams& mp, T gamma) {
+  T diff = T(mp.tdiff) * ramp_interp<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
+  return T(1) / (T(5) * diff + T(0.5));
+}
+
+template <class L, class T>
+__device__ __forceinline__ void thermal_relax(T (&f)[L::Q], const T (&j)[3], T gamma,
+                                              const ModelParams& mp, T& Tout) {
+  constexpr int o = L::QF;
+  T temp = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
+  T omT = thermal_omega<T>(mp, gamma);
+  T dm[5];
+  static_for<1, 5>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    T mk = 0;
+    static_for<0, 5>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (kB5.M[k][q] != 0) mk += T(kB5.M[k][q]) * f[o + q];
+    });
+    T meq = temp * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
+    T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
+    dm[k] = sk * (mk - meq) * T(kB5.invn2[k]);
+  });
+  static_for<0, 5>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    T acc = f[o + q];
+    static_for<1, 5>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if constexpr (kB5.M[k][q] != 0) acc -= T(kB5.M[k][q]) * dm[k];
+    });
+    f[o + q] = acc;
+  });
+  Tout = temp;
+}
+
+// Full node collision in place. rho, j (pre-collision macros) are returned.
+template <class L, int COLL, class T>
+__device__ __forceinline__ void collide(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho,
+                                        T (&j)[3]) {
+  constexpr bool inc = CollTraits<COLL>::inc;
+  flow_macros<L>(f, rho, j);
+  const T omega = T(mp.omega);
+  T v[3];
+  T alpha = 0;
+  const bool forced = mp.design == LBMX_DESIGN_BRINKMAN;
+  if (forced) {
+    alpha = T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
+    T beta = inc ? T(1) / (T(1) + T(0.5) * alpha) : T(1) / (rho * (T(1) + T(0.5) * alpha));
+    v[0] = beta * j[0]; v[1] = beta * j[1]; v[2] = beta * j[2];
+  } else if (inc) {
+    v[0] = j[0]; v[1] = j[1]; v[2] = j[2];
+  } else {
+    T ir = T(1) / rho;
+    v[0] = j[0] * ir; v[1] = j[1] * ir; v[2] = j[2] * ir;
+  }
+  T fin_t[L::QT > 0 ? 5 : 1];
+  if constexpr (L::QT > 0) {
+    static_for<0, 5>([&](auto qc) { fin_t[decltype(qc)::value] = f[L::QF + decltype(qc)::value]; });
+  }
+  base_relax<L, COLL>(f, rho, v, omega, alpha, forced);
+  if (mp.design == LBMX_DESIGN_PBB) {
+    // PartialBouncebackCollision (src/dynamics/partialBouncebackCollision.hh:53-65)
+    T imult = T(0.5) * T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
+    static_for<1, L::HALF + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      T a = f[d], b = f[d + L::HALF];
+      f[d] = a + imult * (b - a);
+      f[d + L::HALF] = b + imult * (a - b);
+    });
+  }
+  if constexpr (L::QT > 0) {
+    T Tn;
+    (void)fin_t;
+    thermal_relax<L>(f, j, gamma, mp, Tn);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Reverse collision (incompressible operators): lam <- (dC/df)^T lam,  dgam += (dC/dgamma)^T lam.
+// The forward is recomputed from f, as the reference's reverse operators do
+// (src/adjointDynamics/reversePartialBounceback.hh:18-41).
+// ---------------------------------------------------------------------------------------
+template <class L, int COLL, class T>
+__device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, const ModelParams& mp,
+                                                T (&lam)[L::Q], T& dgam) {
+  constexpr int Q = L::QF;
+  T rho, j[3];
+  T ff[L::Q];
+  static_for<0, L::Q>([&](auto qc) { ff[decltype(qc)::value] = f[decltype(qc)::value]; });
+  flow_macros<L>(ff, rho, j);
+  const T omega = T(mp.omega);
+  const bool forced = mp.design == LBMX_DESIGN_BRINKMAN;
+  T alpha = 0, beta = 1;
+  if (forced) {
+    alpha = T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
+    beta = T(1) / (T(1) + T(0.5) * alpha);
+  }
+  T v[3] = {beta * j[0], beta * j[1], beta * j[2]};
+  const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  T gbar = 0;
+  T lamT[L::QT > 0 ? 5 : 1];
+  if constexpr (L::QT > 0) {
+    static_for<0, 5>([&](auto qc) { lamT[decltype(qc)::value] = lam[Q + decltype(qc)::value]; });
+  }
+
+  if (mp.design == LBMX_DESIGN_PBB) {
+    T g[L::Q];
+    static_for<0, L::Q>([&](auto qc) { g[decltype(qc)::value] = f[decltype(qc)::value]; });
+    base_relax<L, COLL>(g, rho, v, omega, T(0), false);
+    T imult = T(0.5) * T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
+    T mbar = 0;
+    static_for<1, L::HALF + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      T a = lam[d], b = lam[d + L::HALF];
+      mbar += (a - b) * (g[d + L::HALF] - g[d]);
+      lam[d] = a + imult * (b - a);
+      lam[d + L::HALF] = b + imult * (a - b);
+    });
+    gbar += mbar * T(0.5) * T(mp.alpha_max) * bp_diff<T>(T(mp.interp_q), gamma);
+  }
+
+  T rho_bar = 0, vb[3] = {0, 0, 0}, alpha_bar = 0;
+  if constexpr (!CollTraits<COLL>::mrt) {
+    const T kappa = T(1) - T(0.5) * omega;
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T cu = cdot<L, q>(v);
+      T lq = lam[q];
+      T t = omega * T(L::w[q]) * lq;
+      rho_bar += t;
+      T common = T(9) * cu;
+      if (forced) {
+        T ks = kappa * lq * T(L::w[q]);
+        alpha_bar -= ks * (T(3) * cu + T(9) * cu * cu - T(3) * u2);
+        // d/dv of omega feq and of kappa * src
+        T c1 = t * (T(3) + common) - alpha * ks * (T(3) + T(18) * cu);
+        T c2 = -t * T(3) + alpha * ks * T(6);
+        if constexpr (L::ex[q] != 0) vb[0] += T(L::ex[q]) * c1;
+        if constexpr (L::ey[q] != 0) vb[1] += T(L::ey[q]) * c1;
+        if constexpr (L::ez[q] != 0) vb[2] += T(L::ez[q]) * c1;
+        vb[0] += c2 * v[0]; vb[1] += c2 * v[1]; vb[2] += c2 * v[2];
+      } else {
+        T c1 = t * (T(3) + common);
+        if constexpr (L::ex[q] != 0) vb[0] += T(L::ex[q]) * c1;
+        if constexpr (L::ey[q] != 0) vb[1] += T(L::ey[q]) * c1;
+        if constexpr (L::ez[q] != 0) vb[2] += T(L::ez[q]) * c1;
+        vb[0] -= T(3) * t * v[0]; vb[1] -= T(3) * t * v[1]; vb[2] -= T(3) * t * v[2];
+      }
+      lam[q] = (T(1) - omega) * lq;
+    });
+  } else {
+    T ak[Q];
+    static_for<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      ak[k] = 0;
+      if (BasisOf<Q>::B.rate[k] != 0 || forced) {
+        T s = 0;
+        static_for<0, Q>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if constexpr (BasisOf<Q>::B.M[k][q] != 0) s += T(BasisOf<Q>::B.M[k][q]) * lam[q];
+        });
+        ak[k] = s * T(BasisOf<Q>::B.invn2[k]);
+      }
+    });
+    // meq path: rho_bar, v_bar from b_k = s_k a_k
+    static_for<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if constexpr (BasisOf<Q>::B.rate[k] != 0) {
+        T bk = (BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k])) * ak[k];
+        if constexpr (BasisOf<Q>::B.a[k] != 0) rho_bar += bk * T(BasisOf<Q>::B.a[k]);
+        static_for<0, 3>([&](auto ac) {
+          constexpr int a = decltype(ac)::value;
+          T d = 0;
+          if constexpr (BasisOf<Q>::B.b[k][a] != 0) d += T(BasisOf<Q>::B.b[k][a]);
+          static_for<0, 3>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            if constexpr (BasisOf<Q>::B.C[k][a][b] + BasisOf<Q>::B.C[k][b][a] != 0) d += T(BasisOf<Q>::B.C[k][a][b] + BasisOf<Q>::B.C[k][b][a]) * v[b];
+          });
+          vb[a] += bk * d;
+        });
+      }
+    });
+    if (forced) {
+      // sigma_q = sum_k M_kq (1 - s_k/2) a_k ; source derivative
+      static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        T sg = 0;
+        static_for<0, Q>([&](auto kc) {
+          constexpr int k = decltype(kc)::value;
+          if constexpr (BasisOf<Q>::B.M[k][q] != 0) {
+            T rk = BasisOf<Q>::B.rate[k] == 0 ? T(0) : (BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]));
+            sg += T(BasisOf<Q>::B.M[k][q]) * (T(1) - T(0.5) * rk) * ak[k];
+          }
+        });
+        T cu = cdot<L, q>(v);
+        T ks = sg * T(L::w[q]);
+        alpha_bar -= ks * (T(3) * cu + T(9) * cu * cu - T(3) * u2);
+        T c1 = -alpha * ks * (T(3) + T(18) * cu);
+        if constexpr (L::ex[q] != 0) vb[0] += T(L::ex[q]) * c1;
+        if constexpr (L::ey[q] != 0) vb[1] += T(L::ey[q]) * c1;
+        if constexpr (L::ez[q] != 0) vb[2] += T(L::ez[q]) * c1;
+        T c2 = alpha * ks * T(6);
+        vb[0] += c2 * v[0]; vb[1] += c2 * v[1]; vb[2] += c2 * v[2];
+      });
+    }
+    // direct path: lam_q - sum_k M_kq s_k a_k
+    static_for<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T acc = lam[q];
+      static_for<0, Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (BasisOf<Q>::B.M[k][q] != 0 && BasisOf<Q>::B.rate[k] != 0) {
+          T sk = BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]);
+          acc -= T(BasisOf<Q>::B.M[k][q]) * sk * ak[k];
+        }
+      });
+      lam[q] = acc;
+    });
+  }
+  T jb[3] = {beta * vb[0], beta * vb[1], beta * vb[2]};
+  if (forced) {
+    alpha_bar -= T(0.5) * beta * beta * (vb[0] * j[0] + vb[1] * j[1] + vb[2] * j[2]);
+    gbar += alpha_bar * T(mp.alpha_max) * bp_diff<T>(T(mp.interp_q), gamma);
+  }
+
+  if constexpr (L::QT > 0) {
+    constexpr int o = Q;
+    T temp = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
+    T omT = thermal_omega<T>(mp, gamma);
+    T ak[5];
+    static_for<1, 5>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      T s = 0;
+      static_for<0, 5>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (kB5.M[k][q] != 0) s += T(kB5.M[k][q]) * lamT[q];
+      });
+      ak[k] = s * T(kB5.invn2[k]);
+    });
+    T Tbar = 0, ombar = 0;
+    static_for<1, 5>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
+      T bk = sk * ak[k];
+      Tbar += bk * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
+      if constexpr (kB5.b[k][0] != 0) jb[0] += bk * temp * T(kB5.b[k][0]);
+      if constexpr (kB5.b[k][1] != 0) jb[1] += bk * temp * T(kB5.b[k][1]);
+      if constexpr (kB5.rate[k] == 1) {
+        T mk = 0;
+        static_for<0, 5>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if constexpr (kB5.M[k][q] != 0) mk += T(kB5.M[k][q]) * f[o + q];
+        });
+        T meq = temp * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
+        ombar -= ak[k] * (mk - meq);
+      }
+    });
+    static_for<0, 5>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T acc = lamT[q];
+      static_for<1, 5>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (kB5.M[k][q] != 0) {
+          T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
+          acc -= T(kB5.M[k][q]) * sk * ak[k];
+        }
+      });
+      lam[o + q] = acc + Tbar;
+    });
+    T dR = ramp_diff<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
+    gbar += ombar * (-omT * omT * T(5) * T(mp.tdiff) * dR);
+  }
+
+  static_for<0, Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    T add = rho_bar;
+    if constexpr (L::ex[q] != 0) add += T(L::ex[q]) * jb[0];
+    if constexpr (L::ey[q] != 0) add += T(L::ey[q]) * jb[1];
+    if constexpr (L::ez[q] != 0) add += T(L::ez[q]) * jb[2];
+    lam[q] += add;
+  });
+  dgam += gbar;
+}
+
+// ---------------------------------------------------------------------------------------
+// Boundary maps. Each piece is node-local; pieces run in insertion order after streaming
+// (src/latticeBoltzmann/LBSolver.cc:288-293). Adjoints run the transposes in reverse order.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double bc_value(const DevBC& b, int D, int t, int x, int y, int z) {
+  double s, tt;
+  if (D == 2) { s = b.nrm[0] != 0 ? y : x; tt = 0; }
+  else if (b.nrm[0] != 0) { s = y; tt = z; }
+  else if (b.nrm[1] != 0) { s = x; tt = z; }
+  else { s = x; tt = y; }
+  double shape = 1.0;
+  if (b.profile == LBMX_PROFILE_PARABOLIC) {
+    double R = b.radius[0], d = s - b.center[0];
+    shape = (R * R - d * d) / (R * R);
+  } else if (b.profile == LBMX_PROFILE_PARABOLIC_DISK) {
+    double R = b.radius[0], d0 = s - b.center[0], d1 = tt - b.center[1];
+    double r2 = d0 * d0 + d1 * d1;
+    shape = r2 <= R * R ? 1.0 - r2 / (R * R) : 0.0;
+  } else if (b.profile == LBMX_PROFILE_PARABOLIC_DUCT) {
+    double d0 = (s - b.center[0]) / b.radius[0], d1 = (tt - b.center[1]) / b.radius[1];
+    shape = (1.0 - d0 * d0) * (1.0 - d1 * d1);
+  }
+  double tf = 1.0;
+  if (b.time_mode == LBMX_TIME_SINE)
+    tf = b.t_offset + b.t_amp * sin(2.0 * 3.14159265358979323846 * t / b.t_period + b.t_phase);
+  return b.value * shape * tf;
+}
+
+__device__ __forceinline__ bool in_box(const int* bx, int x, int y, int z) {
+  return x >= bx[0] && x <= bx[1] && y >= bx[2] && y <= bx[3] && z >= bx[4] && z <= bx[5];
+}
+
+// on-grid bounce-back: q takes opp(q) when c_q.n <= 0 (incoming or tangential), per
+// OnGridBounceback2d/3d (src/boundaryConditions/bounceback2d.cc:7-193, bounceback3d.cc:8-511);
+// the (x0,y0,z0) corner of bounceback3d.cc:434-451 keeps f13 and f9 and sets f18 = f9.
+template <class L, int q>
+__device__ __forceinline__ bool bb_from_opp(const int* n) {
+  if constexpr (q == 0 || q >= L::QF) return false;
+  else {
+    int cn = L::ex[q] * n[0] + L::ey[q] * n[1] + L::ez[q] * n[2];
+    bool r = cn <= 0;
+    if constexpr (L::D == 3 && (q == 13 || q == 9)) {
+      if (n[0] == -1 && n[1] == -1 && n[2] == -1) r = false;
+    }
+    return r;
+  }
+}
+
+template <class L, class T>
+__device__ __forceinline__ void bc_bounceback(T (&f)[L::Q], const int* n) {
+  T o[L::QF];
+  static_for<0, L::QF>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    o[q] = bb_from_opp<L, q>(n) ? f[L::opp[q]] : f[q];
+  });
+  static_for<0, L::QF>([&](auto qc) { f[decltype(qc)::value] = o[decltype(qc)::value]; });
+}
+
+template <class L, class T>
+__device__ __forceinline__ void bc_bounceback_adj(T (&l)[L::Q], const int* n) {
+  T o[L::QF];
+  static_for<0, L::QF>([&](auto qc) {
+    constexpr int r = decltype(qc)::value;
+    T acc = bb_from_opp<L, r>(n) ? T(0) : l[r];
+    if constexpr (r != 0) {
+      if (bb_from_opp<L, L::opp[r]>(n)) acc += l[L::opp[r]];
+    }
+    o[r] = acc;
+  });
+  static_for<0, L::QF>([&](auto qc) { l[decltype(qc)::value] = o[decltype(qc)::value]; });
+}
+
+// D2Q9 Zou-He (src/boundaryConditions/ZouHe2d.hh): incompressible velocity W/N/E,
+// standard velocity W, pressure E/W (both variants).
+template <class T>
+__device__ __forceinline__ void zh_velocity(T* f, const int* n, bool inc, T ux, T uy) {
+  if (n[0] == -1) {
+    if (inc) {
+      f[1] = T(0.5) * (f[6] - f[2]) + f[5] + ux / T(6) - T(0.5) * uy;
+      f[7] = T(0.5) * (f[2] - f[6]) + f[3] + ux / T(6) + T(0.5) * uy;
+      f[8] = f[4] + T(2) / T(3) * ux;
+    } else {
+      T rho = (f[0] + f[2] + f[6] + T(2) * (f[3] + f[4] + f[5])) / (T(1) - ux);
+      f[1] = f[5] + T(0.5) * (f[6] - f[2]) + rho * ux / T(6) - T(0.5) * rho * uy;
+      f[7] = f[3] + T(0.5) * (f[2] - f[6]) + rho * ux / T(6) + T(0.5) * rho * uy;
+      f[8] = f[4] + T(2) / T(3) * rho * ux;
+    }
+  } else if (n[1] == 1) {
+    f[1] = T(0.5) * (f[4] - f[8]) + f[5] - uy / T(6) + T(0.5) * ux;
+    f[2] = f[6] - T(2) / T(3) * uy;
+    f[3] = T(0.5) * (f[8] - f[4]) + f[7] - uy / T(6) - T(0.5) * ux;
+  } else {
+    f[3] = f[7] + T(0.5) * (f[6] - f[2]) - ux / T(6) - T(0.5) * uy;
+    f[4] = f[8] - T(2) / T(3) * ux;
+    f[5] = f[1] + T(0.5) * (f[2] - f[6]) - ux / T(6) + T(0.5) * uy;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void zh_velocity_adj(T* l, const int* n) {
+  if (n[0] == -1) {
+    l[2] += T(0.5) * (l[7] - l[1]);
+    l[6] += T(0.5) * (l[1] - l[7]);
+    l[5] += l[1]; l[3] += l[7]; l[4] += l[8];
+    l[1] = 0; l[7] = 0; l[8] = 0;
+  } else if (n[1] == 1) {
+    l[4] += T(0.5) * (l[1] - l[3]);
+    l[8] += T(0.5) * (l[3] - l[1]);
+    l[5] += l[1]; l[6] += l[2]; l[7] += l[3];
+    l[1] = 0; l[2] = 0; l[3] = 0;
+  } else {
+    l[7] += l[3];
+    l[6] += T(0.5) * (l[3] - l[5]);
+    l[2] += T(0.5) * (l[5] - l[3]);
+    l[8] += l[4]; l[1] += l[5];
+    l[3] = 0; l[4] = 0; l[5] = 0;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void zh_pressure(T* f, const int* n, bool inc, T r0) {
+  if (n[0] == 1) {
+    T s = f[0] + f[2] + f[6] + T(2) * (f[1] + f[7] + f[8]);
+    T u0 = inc ? s - r0 : T(-1) + s / r0;
+    T sc = inc ? T(1) : r0;
+    f[4] = f[8] - T(2) / T(3) * sc * u0;
+    f[3] = f[7] - sc * u0 / T(6) + T(0.5) * (f[6] - f[2]);
+    f[5] = f[1] - sc * u0 / T(6) + T(0.5) * (f[2] - f[6]);
+  } else {
+    T s = f[0] + f[2] + f[6] + T(2) * (f[3] + f[4] + f[5]);
+    T u0 = inc ? r0 - s : T(1) - s / r0;
+    T sc = inc ? T(1) : r0;
+    f[1] = f[5] + sc * u0 / T(6) + T(0.5) * (f[6] - f[2]);
+    f[7] = f[3] + sc * u0 / T(6) + T(0.5) * (f[2] - f[6]);
+    f[8] = f[4] + T(2) / T(3) * sc * u0;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void zh_pressure_adj(T* l, const int* n) {
+  if (n[0] == 1) {
+    T ub = -(T(2) / T(3)) * l[4] - l[3] / T(6) - l[5] / T(6);
+    l[8] += l[4]; l[7] += l[3]; l[1] += l[5];
+    T d = T(0.5) * (l[3] - l[5]);
+    l[6] += d; l[2] -= d;
+    l[0] += ub; l[2] += ub; l[6] += ub;
+    l[1] += T(2) * ub; l[7] += T(2) * ub; l[8] += T(2) * ub;
+    l[3] = 0; l[4] = 0; l[5] = 0;
+  } else {
+    T sb = -(l[1] / T(6) + l[7] / T(6) + (T(2) / T(3)) * l[8]);
+    l[5] += l[1]; l[3] += l[7]; l[4] += l[8];
+    T d = T(0.5) * (l[1] - l[7]);
+    l[6] += d; l[2] -= d;
+    l[0] += sb; l[2] += sb; l[6] += sb;
+    l[3] += T(2) * sb; l[4] += T(2) * sb; l[5] += T(2) * sb;
+    l[1] = 0; l[7] = 0; l[8] = 0;
+  }
+}
+
+// D3Q19 Hecht & Harting (src/boundaryConditions/Hecht3d.hh): velocity west, pressure east.
+template <class T>
+__device__ __forceinline__ void hecht_velocity(T* f, bool inc, T ux, T uy, T uz) {
+  T rho = T(1);
+  if (!inc)
+    rho = (f[0] + f[1] + f[2] + f[4] + f[5] + f[10] + f[11] + f[13] + f[14] +
+           T(2) * (f[3] + f[6] + f[8] + f[16] + f[18])) / (T(1) - ux);
+  T Nyx = T(0.5) * (f[5] + f[11] + f[13] - (f[2] + f[4] + f[14])) - rho * uy / T(3);
+  T Nzx = T(0.5) * (f[10] + f[13] + f[14] - (f[1] + f[4] + f[5])) - rho * uz / T(3);
+  f[7] = f[16] + rho * (ux - uz) / T(6) + Nzx;
+  f[9] = f[18] + rho * (ux - uy) / T(6) + Nyx;
+  f[12] = f[3] + rho * ux / T(3);
+  f[15] = f[6] + rho * (ux + uz) / T(6) - Nzx;
+  f[17] = f[8] + rho * (ux + uy) / T(6) - Nyx;
+}
+
+template <class T>
+__device__ __forceinline__ void hecht_nbar(T* l, T nyx, T nzx) {
+  T hy = T(0.5) * nyx, hz = T(0.5) * nzx;
+  l[5] += hy; l[11] += hy; l[13] += hy; l[2] -= hy; l[4] -= hy; l[14] -= hy;
+  l[10] += hz; l[13] += hz; l[14] += hz; l[1] -= hz; l[4] -= hz; l[5] -= hz;
+}
+
+template <class T>
+__device__ __forceinline__ void hecht_velocity_adj(T* l) {
+  T nzx = l[7] - l[15], nyx = l[9] - l[17];
+  l[16] += l[7]; l[18] += l[9]; l[3] += l[12]; l[6] += l[15]; l[8] += l[17];
+  hecht_nbar(l, nyx, nzx);
+  l[7] = 0; l[9] = 0; l[12] = 0; l[15] = 0; l[17] = 0;
+}
+
+template <class T>
+__device__ __forceinline__ void hecht_pressure(T* f, bool inc, T r0) {
+  T s = f[0] + f[1] + f[2] + f[4] + f[5] + f[10] + f[11] + f[13] + f[14] +
+        T(2) * (f[7] + f[9] + f[12] + f[15] + f[17]);
+  T ux = inc ? s - r0 : T(-1) + s / r0;
+  T sc = inc ? T(1) : r0;
+  T Nyx = T(0.5) * (f[5] + f[11] + f[13] - (f[2] + f[4] + f[14]));
+  T Nzx = T(0.5) * (f[10] + f[13] + f[14] - (f[1] + f[4] + f[5]));
+  f[3] = f[12] - sc * ux / T(3);
+  f[6] = f[15] - sc * ux / T(6) + Nzx;
+  f[8] = f[17] - sc * ux / T(6) + Nyx;
+  f[16] = f[7] - sc * ux / T(6) - Nzx;
+  f[18] = f[9] - sc * ux / T(6) - Nyx;
+}
+
+template <class T>
+__device__ __forceinline__ void hecht_pressure_adj(T* l) {
+  T ub = -l[3] / T(3) - (l[6] + l[8] + l[16] + l[18]) / T(6);
+  T nzx = l[6] - l[16], nyx = l[8] - l[18];
+  l[12] += l[3]; l[15] += l[6]; l[17] += l[8]; l[7] += l[16]; l[9] += l[18];
+  l[0] += ub; l[1] += ub; l[2] += ub; l[4] += ub; l[5] += ub;
+  l[10] += ub; l[11] += ub; l[13] += ub; l[14] += ub;
+  T u2 = T(2) * ub;
+  l[7] += u2; l[9] += u2; l[12] += u2; l[15] += u2; l[17] += u2;
+  hecht_nbar(l, nyx, nzx);
+  l[3] = 0; l[6] = 0; l[8] = 0; l[16] = 0; l[18] = 0;
+}
+
+// D2Q5 part (offset QF): adiabatic bounce-back (ThermalBounceback2d, thermalBounceback2d.hh:273-490)
+// and Dirichlet temperature (ThermalZouHeWest2d, thermalZouHe2d.hh:62-93).
+__device__ __forceinline__ int d2q5_incoming_mask(const int* n) {
+  // bit q set when c_q . n < 0 (q = 1..4 of D2Q5: (0,-1), (-1,0), (0,1), (1,0))
+  int m = 0;
+  if (-n[1] < 0) m |= 1 << 1;
+  if (-n[0] < 0) m |= 1 << 2;
+  if (n[1] < 0) m |= 1 << 3;
+  if (n[0] < 0) m |= 1 << 4;
+  return m;
+}
+
+template <class L, class T>
+__device__ __forceinline__ void bc_node(T (&f)[L::Q], const DevBC& b, int t, int x, int y, int z) {
+  switch (b.type) {
+    case LBMX_BC_BOUNCEBACK: bc_bounceback<L>(f, b.nrm); break;
+    case LBMX_BC_ZOUHE_VELOCITY:
+      if constexpr (L::D == 2) {
+        T v = T(bc_value(b, 2, t, x, y, z));
+        zh_velocity<T>(f, b.nrm, b.inc, v * T(b.dir[0]), v * T(b.dir[1]));
+      }
+      break;
+    case LBMX_BC_ZOUHE_PRESSURE:
+      if constexpr (L::D == 2) zh_pressure<T>(f, b.nrm, b.inc, T(bc_value(b, 2, t, x, y, z)));
+      break;
+    case LBMX_BC_HECHT_VELOCITY:
+      if constexpr (L::D == 3) {
+        T v = T(bc_value(b, 3, t, x, y, z));
+        hecht_velocity<T>(f, b.inc, v * T(b.dir[0]), v * T(b.dir[1]), v * T(b.dir[2]));
+      }
+      break;
+    case LBMX_BC_HECHT_PRESSURE:
+      if constexpr (L::D == 3) hecht_pressure<T>(f, b.inc, T(bc_value(b, 3, t, x, y, z)));
+      break;
+    case LBMX_BC_THERMAL_BOUNCEBACK:
+      if constexpr (L::QT == 5) {
+        int m = d2q5_incoming_mask(b.nrm);
+        constexpr int o = L::QF;
+        T g1 = f[o + 1], g2 = f[o + 2], g3 = f[o + 3], g4 = f[o + 4];
+        if (m & 2) f[o + 1] = g3;
+        if (m & 4) f[o + 2] = g4;
+        if (m & 8) f[o + 3] = g1;
+        if (m & 16) f[o + 4] = g2;
+      }
+      break;
+    case LBMX_BC_THERMAL_DIRICHLET:
+      if constexpr (L::QT == 5) {
+        constexpr int o = L::QF;
+        T Tv = T(bc_value(b, 2, t, x, y, z));
+        T s = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
+        int m = d2q5_incoming_mask(b.nrm);
+        if (m & 2) f[o + 1] = Tv - (s - f[o + 1]);
+        else if (m & 4) f[o + 2] = Tv - (s - f[o + 2]);
+        else if (m & 8) f[o + 3] = Tv - (s - f[o + 3]);
+        else if (m & 16) f[o + 4] = Tv - (s - f[o + 4]);
+      }
+      break;
+    default: break;
+  }
+}
+
+template <class L, class T>
+__device__ __forceinline__ void bc_node_adj(T (&l)[L::Q], const DevBC& b) {
+  switch (b.type) {
+    case LBMX_BC_BOUNCEBACK: bc_bounceback_adj<L>(l, b.nrm); break;
+    case LBMX_BC_ZOUHE_VELOCITY:
+      if constexpr (L::D == 2) zh_velocity_adj<T>(l, b.nrm);
+      break;
+    case LBMX_BC_ZOUHE_PRESSURE:
+      if constexpr (L::D == 2) zh_pressure_adj<T>(l, b.nrm);
+      break;
+    case LBMX_BC_HECHT_VELOCITY:
+      if constexpr (L::D == 3) hecht_velocity_adj<T>(l);
+      break;
+    case LBMX_BC_HECHT_PRESSURE:
+      if constexpr (L::D == 3) hecht_pressure_adj<T>(l);
+      break;
+    case LBMX_BC_THERMAL_BOUNCEBACK:
+      if constexpr (L::QT == 5) {
+        int m = d2q5_incoming_mask(b.nrm);
+        constexpr int o = L::QF;
+        if (m & 2) { l[o + 3] += l[o + 1]; l[o + 1] = 0; }
+        if (m & 4) { l[o + 4] += l[o + 2]; l[o + 2] = 0; }
+        if (m & 8) { l[o + 1] += l[o + 3]; l[o + 3] = 0; }
+        if (m & 16) { l[o + 2] += l[o + 4]; l[o + 4] = 0; }
+      }
+      break;
+    case LBMX_BC_THERMAL_DIRICHLET:
+      if constexpr (L::QT == 5) {
+        constexpr int o = L::QF;
+        int m = d2q5_incoming_mask(b.nrm);
+        int u = (m & 2) ? 1 : (m & 4) ? 2 : (m & 8) ? 3 : 4;
+        T lu = u == 1 ? l[o + 1] : u == 2 ? l[o + 2] : u == 3 ? l[o + 3] : l[o + 4];
+        l[o] -= lu;
+        if (u != 1) l[o + 1] -= lu; else l[o + 1] = 0;
+        if (u != 2) l[o + 2] -= lu; else l[o + 2] = 0;
+        if (u != 3) l[o + 3] -= lu; else l[o + 3] = 0;
+        if (u != 4) l[o + 4] -= lu; else l[o + 4] = 0;
+      }
+      break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Functional terms: value and gradient w.r.t. the node's pre-collision populations.
+// ---------------------------------------------------------------------------------------
+template <class L, int COLL, class T>
+__device__ __forceinline__ T term_value(int kind, const T (&f)[L::Q], T rho, const T (&j)[3]) {
+  constexpr bool inc = CollTraits<COLL>::inc;
+  switch (kind) {
+    case 0: return rho;
+    case 1: return inc ? j[0] : j[0] / rho;
+    case 2: return (rho / T(3) + T(0.5) * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2])) * j[0];
+    case 3: {
+      T temp = 0;
+      if constexpr (L::QT > 0) temp = f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
+      return j[0] * temp;
+    }
+  }
+  return T(0);
+}
+
+template <class L, class T>
+__device__ __forceinline__ void term_grad(int kind, const T (&f)[L::Q], T rho, const T (&j)[3], T wgt,
+                                          T (&g)[L::Q]) {
+  T temp = 0;
+  if constexpr (L::QT > 0) temp = f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
+  const T pt = rho / T(3) + T(0.5) * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+  static_for<0, L::QF>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr T cx = T(L::ex[q]);
+    T add = 0;
+    if (kind == 0) add = wgt;
+    else if (kind == 1) add = wgt * cx;
+    else if (kind == 2) add = wgt * (j[0] / T(3) + cdot<L, q>(j) * j[0] + pt * cx);
+    else if (kind == 3) add = wgt * cx * temp;
+    g[q] += add;
+  });
+  if constexpr (L::QT > 0) {
+    if (kind == 3) {
+      static_for<L::QF, L::Q>([&](auto qc) { g[decltype(qc)::value] += wgt * j[0]; });
+    }
+  }
+}
+
+}  // namespace lbmx

@@ -1,0 +1,299 @@
+"""Python host binding of the lbmx C ABI (``include/lbmx.h``) via ctypes.
+
+The engine itself is C++/HIP (``csrc/``) built into ``liblbmx.so`` next to this file.
+This module only marshals arguments; it fails loudly when the library is missing.
+Array conventions follow the reference's single-rank PETSc DMDA order:
+design/sensitivity ``(k*ny+j)*nx+i``; distributions array-of-structures ``[node][q]``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbmx.so")
+
+# ---- enums (include/lbmx.h) ----
+D2Q9, D3Q19, D2Q9_D2Q5 = 0, 1, 2
+BGK, MRT, CASCADED, MRT_STANDARD, BGK_STANDARD = 0, 1, 2, 3, 4
+DESIGN_NONE, DESIGN_PBB, DESIGN_BRINKMAN, DESIGN_DARCY = 0, 1, 2, 3
+FP64, FP32, FP16_SHIFTED, BF16_SHIFTED = 0, 1, 2, 3
+BC_BOUNCEBACK, BC_ZOUHE_VELOCITY, BC_ZOUHE_PRESSURE, BC_ZOUHE_NEUMANN = 0, 1, 2, 3
+BC_HECHT_VELOCITY, BC_HECHT_PRESSURE, BC_THERMAL_BOUNCEBACK, BC_THERMAL_DIRICHLET = 4, 5, 6, 7
+PROFILE_UNIFORM, PROFILE_PARABOLIC, PROFILE_PARABOLIC_DISK, PROFILE_PARABOLIC_DUCT = 0, 1, 2, 3
+TIME_CONSTANT, TIME_SINE = 0, 1
+OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX = 0, 1, 2, 3, 4
+
+LBMX_OK, ERR_ARG, ERR_SUP, ERR_MEM, ERR_STATE, ERR_DEVICE = 0, 62, 56, 55, 73, 76
+
+Q_OF = {D2Q9: 9, D3Q19: 19, D2Q9_D2Q5: 14}
+
+
+class Desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lattice", C.c_int), ("collision", C.c_int), ("design", C.c_int),
+                ("precision", C.c_int), ("reserved0", C.c_int),
+                ("omega", C.c_double), ("interp_q", C.c_double), ("alpha_max", C.c_double),
+                ("body_force", C.c_double * 3),
+                ("thermal_diffusivity", C.c_double), ("ramp_p", C.c_double), ("ramp_ck", C.c_double)]
+
+
+class Box(C.Structure):
+    _fields_ = [("x0", C.c_int), ("x1", C.c_int), ("y0", C.c_int), ("y1", C.c_int),
+                ("z0", C.c_int), ("z1", C.c_int)]
+
+
+class BCDesc(C.Structure):
+    _fields_ = [("type", C.c_int), ("profile", C.c_int), ("time_mode", C.c_int), ("reserved0", C.c_int),
+                ("value", C.c_double), ("dir", C.c_double * 3), ("center", C.c_double * 2),
+                ("radius", C.c_double * 2), ("t_period", C.c_double), ("t_amp", C.c_double),
+                ("t_offset", C.c_double), ("t_phase", C.c_double)]
+
+
+class Objective(C.Structure):
+    _fields_ = [("kind", C.c_int), ("reserved0", C.c_int), ("time_const", C.c_double),
+                ("box_a", Box), ("box_b", Box)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("fwd_kernel_ms", C.c_double), ("adj_kernel_ms", C.c_double),
+                ("forward_phase_ms", C.c_double), ("reverse_phase_ms", C.c_double),
+                ("fwd_sweeps", C.c_longlong), ("adj_sweeps", C.c_longlong),
+                ("checkpoint_copies", C.c_longlong), ("recompute_sweeps", C.c_longlong)]
+
+
+def make_desc(nx, ny, nz=1, lattice=D2Q9, collision=BGK, design=DESIGN_PBB, precision=FP64,
+              omega=1.25, interp_q=0.1, alpha_max=1.0, body_force=(0.0, 0.0, 0.0),
+              thermal_diffusivity=0.01, ramp_p=1.0, ramp_ck=0.1):
+    d = Desc()
+    d.nx, d.ny, d.nz = nx, ny, nz
+    d.lattice, d.collision, d.design, d.precision = lattice, collision, design, precision
+    d.omega, d.interp_q, d.alpha_max = omega, interp_q, alpha_max
+    for a in range(3):
+        d.body_force[a] = body_force[a]
+    d.thermal_diffusivity, d.ramp_p, d.ramp_ck = thermal_diffusivity, ramp_p, ramp_ck
+    return d
+
+
+def make_box(x0, x1, y0, y1, z0=0, z1=0):
+    return Box(x0, x1, y0, y1, z0, z1)
+
+
+def make_bc(type, value=0.0, dir=(1.0, 0.0, 0.0), profile=PROFILE_UNIFORM, center=(0.0, 0.0),
+            radius=(1.0, 1.0), time_mode=TIME_CONSTANT, t_period=1.0, t_amp=0.0, t_offset=1.0,
+            t_phase=0.0):
+    b = BCDesc()
+    b.type, b.profile, b.time_mode = type, profile, time_mode
+    b.value = value
+    for a in range(3):
+        b.dir[a] = dir[a]
+    b.center[0], b.center[1] = center
+    b.radius[0], b.radius[1] = radius
+    b.t_period, b.t_amp, b.t_offset, b.t_phase = t_period, t_amp, t_offset, t_phase
+    return b
+
+
+def make_objective(kind, box_a, box_b=None, time_const=1.0):
+    o = Objective()
+    o.kind, o.time_const = kind, time_const
+    o.box_a = box_a
+    o.box_b = box_b if box_b is not None else Box(0, -1, 0, -1, 0, -1)
+    return o
+
+
+_lib = None
+
+
+def library():
+    """Load liblbmx.so (the HIP engine). Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"lbmx engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    dp = P(C.c_double)
+    sig = {
+        "lbmx_create": [P(Desc), P(C.c_void_p)],
+        "lbmx_destroy": [C.c_void_p],
+        "lbmx_set_stream": [C.c_void_p, C.c_void_p],
+        "lbmx_add_boundary": [C.c_void_p, P(Box), P(BCDesc)],
+        "lbmx_set_design": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_get_design": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_set_objective": [C.c_void_p, P(Objective)],
+        "lbmx_init_equilibrium": [C.c_void_p, C.c_double, dp, C.c_double],
+        "lbmx_init_equilibrium_field": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_forward": [C.c_void_p, C.c_int, dp],
+        "lbmx_get_state": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_set_state": [C.c_void_p, dp, C.c_size_t, C.c_int],
+        "lbmx_get_macros": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_get_timestep": [C.c_void_p, P(C.c_int)],
+        "lbmx_checkpoint_config": [C.c_void_p, C.c_int],
+        "lbmx_gradient": [C.c_void_p, C.c_int, C.c_double, dp, C.c_double, dp, dp, C.c_size_t],
+        "lbmx_get_sensitivity": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_get_adjoint": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_get_stats": [C.c_void_p, P(Stats)],
+        "lbmx_device_bytes": [C.c_void_p, P(C.c_size_t)],
+        "lbmx_describe_boundaries": [C.c_void_p, P(C.c_int), C.c_int, P(C.c_int)],
+        "lbmx_checkpoint_simulate": [C.c_int, C.c_int, P(C.c_longlong), P(C.c_int)],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.lbmx_last_error.argtypes = []
+    lib.lbmx_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error", "lbmx_add_boundary",
+            "lbmx_set_design", "lbmx_get_design", "lbmx_set_objective", "lbmx_init_equilibrium",
+            "lbmx_init_equilibrium_field", "lbmx_forward", "lbmx_get_state", "lbmx_set_state",
+            "lbmx_get_macros", "lbmx_get_timestep", "lbmx_checkpoint_config", "lbmx_gradient",
+            "lbmx_get_sensitivity", "lbmx_get_adjoint", "lbmx_get_stats", "lbmx_device_bytes",
+            "lbmx_describe_boundaries", "lbmx_checkpoint_simulate"]
+
+
+def checkpoint_simulate(nslots, nsteps):
+    """Host replay of the gradient's checkpoint schedule: (recompute advances, max slots used)."""
+    adv = C.c_longlong(0)
+    mx = C.c_int(0)
+    _check(library().lbmx_checkpoint_simulate(nslots, nsteps, C.byref(adv), C.byref(mx)))
+    return adv.value, mx.value
+
+
+class LbmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lbmx error {code}: {msg}")
+        self.code = code
+
+
+def _check(code):
+    if code != 0:
+        msg = library().lbmx_last_error()
+        raise LbmxError(code, msg.decode() if msg else "")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One lattice (forward + adjoint) on the current GPU; mirrors IsothermalLBSolver2d/3d +
+    AdjointLBSolver of the reference (src/latticeBoltzmann/isothermalLBSolver2d.hh:20-110,
+    src/adjointLatticeBoltzmann/adjointLBSolver.hh:11-46)."""
+
+    def __init__(self, desc):
+        self.lib = library()
+        self.desc = desc
+        self.nodes = desc.nx * desc.ny * desc.nz
+        self.Q = Q_OF[desc.lattice]
+        h = C.c_void_p()
+        _check(self.lib.lbmx_create(C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.lbmx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr):
+        _check(self.lib.lbmx_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+    def add_boundary(self, box, bc):
+        _check(self.lib.lbmx_add_boundary(self.h, C.byref(box), C.byref(bc)))
+
+    def set_design(self, gamma):
+        g = np.ascontiguousarray(gamma, dtype=np.float64).ravel()
+        _check(self.lib.lbmx_set_design(self.h, _dptr(g), g.size))
+
+    def get_design(self):
+        g = np.empty(self.nodes)
+        _check(self.lib.lbmx_get_design(self.h, _dptr(g), g.size))
+        return g
+
+    def set_objective(self, obj):
+        _check(self.lib.lbmx_set_objective(self.h, C.byref(obj)))
+
+    def init_equilibrium(self, rho0=1.0, u0=(0.0, 0.0, 0.0), T0=0.0):
+        u = np.asarray(u0, dtype=np.float64)
+        _check(self.lib.lbmx_init_equilibrium(self.h, rho0, _dptr(u), T0))
+
+    def init_equilibrium_field(self, macros):
+        m = np.ascontiguousarray(macros, dtype=np.float64).ravel()
+        _check(self.lib.lbmx_init_equilibrium_field(self.h, _dptr(m), m.size))
+
+    def forward(self, nsteps):
+        J = C.c_double(0.0)
+        _check(self.lib.lbmx_forward(self.h, nsteps, C.byref(J)))
+        return J.value
+
+    def get_state(self):
+        f = np.empty(self.nodes * self.Q)
+        _check(self.lib.lbmx_get_state(self.h, _dptr(f), f.size))
+        return f
+
+    def set_state(self, f, timestep):
+        a = np.ascontiguousarray(f, dtype=np.float64).ravel()
+        _check(self.lib.lbmx_set_state(self.h, _dptr(a), a.size, timestep))
+
+    def get_macros(self):
+        m = np.empty(self.nodes * 5)
+        _check(self.lib.lbmx_get_macros(self.h, _dptr(m), m.size))
+        return m.reshape(self.nodes, 5)
+
+    def timestep(self):
+        t = C.c_int(0)
+        _check(self.lib.lbmx_get_timestep(self.h, C.byref(t)))
+        return t.value
+
+    def checkpoint_config(self, nslots):
+        _check(self.lib.lbmx_checkpoint_config(self.h, nslots))
+
+    def gradient(self, nsteps, rho0=1.0, u0=(0.0, 0.0, 0.0), T0=0.0, want_sens=True):
+        u = np.asarray(u0, dtype=np.float64)
+        J = np.zeros(1)
+        sens = np.empty(self.nodes) if want_sens else None
+        _check(self.lib.lbmx_gradient(self.h, nsteps, rho0, _dptr(u), T0, _dptr(J),
+                                      _dptr(sens) if want_sens else None,
+                                      self.nodes if want_sens else 0))
+        return J[0], sens
+
+    def get_sensitivity(self):
+        s = np.empty(self.nodes)
+        _check(self.lib.lbmx_get_sensitivity(self.h, _dptr(s), s.size))
+        return s
+
+    def get_adjoint(self):
+        a = np.empty(self.nodes * self.Q)
+        _check(self.lib.lbmx_get_adjoint(self.h, _dptr(a), a.size))
+        return a
+
+    def stats(self):
+        st = Stats()
+        _check(self.lib.lbmx_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def boundaries(self):
+        cnt = C.c_int(0)
+        _check(self.lib.lbmx_describe_boundaries(self.h, None, 0, C.byref(cnt)))
+        buf = (C.c_int * (10 * max(1, cnt.value)))()
+        _check(self.lib.lbmx_describe_boundaries(self.h, buf, cnt.value, C.byref(cnt)))
+        out = []
+        for k in range(cnt.value):
+            v = buf[10 * k:10 * k + 10]
+            out.append((v[0], tuple(v[1:4]), tuple(v[4:10])))
+        return out
+
+    def device_bytes(self):
+        b = C.c_size_t(0)
+        _check(self.lib.lbmx_device_bytes(self.h, C.byref(b)))
+        return b.value

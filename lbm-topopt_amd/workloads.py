@@ -1,0 +1,108 @@
+"""Synthetic workloads: design fields and the channel / duct problem setups.
+
+Every setup function takes any object with ``add_boundary``, ``set_design`` and
+``set_objective`` (the engine's ``lbmx.Solver``), so the same problem can be built on
+other implementations of that interface too.
+
+Design field: gamma ~ U[0,1) per node from SplitMix64 (seeded), smoothed with a 5x5 (5x5x5)
+box mean truncated at the domain edges (mean over the in-domain neighbours).
+Porosity map: alpha = alpha_max * q (1 - gamma) / (q + gamma), alpha_max = 0.1, q = 0.1
+(the reference's BPInterpolation, src/core/interpolationFunctions.hh:18-21).
+"""
+import numpy as np
+
+import lbmx
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64_uniform(n, seed):
+    """n doubles in [0,1) from the SplitMix64 stream with the given seed."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _box_sum_axis(a, r, axis):
+    c = np.cumsum(a, axis=axis)
+    pad = [(0, 0)] * a.ndim
+    pad[axis] = (1, 0)
+    c = np.pad(c, pad)
+    n = a.shape[axis]
+    idx = np.arange(n)
+    hi = np.minimum(idx + r, n - 1) + 1
+    lo = np.maximum(idx - r, 0)
+    return np.take(c, hi, axis=axis) - np.take(c, lo, axis=axis)
+
+
+def box_filter(a, r=2):
+    s = a.astype(np.float64)
+    cnt = np.ones_like(s)
+    for ax in range(a.ndim):
+        if a.shape[ax] > 1:
+            s = _box_sum_axis(s, r, ax)
+            cnt = _box_sum_axis(cnt, r, ax)
+    return s / cnt
+
+
+def design_field(nx, ny, nz=1, seed=0, smooth=True):
+    """gamma in natural order (k*ny + j)*nx + i, shape (nz, ny, nx) flattened."""
+    g = splitmix64_uniform(nx * ny * nz, seed).reshape(nz, ny, nx)
+    if smooth:
+        g = box_filter(g, 2)
+    return np.ascontiguousarray(g).ravel()
+
+
+def channel2d(obj, nx, ny, u_max=0.05, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_DROP, time_const=1.0,
+              modulation=None):
+    """Config-0/1 channel: Zou-He parabolic velocity inlet (x=0, 1<=y<=ny-2), Zou-He pressure
+    outlet (x=nx-1), on-grid bounce-back walls at y=0 and y=ny-1 (corners included)."""
+    B = lbmx.make_box
+    mid = (ny - 1) / 2.0
+    kw = {}
+    if modulation is not None:
+        kw = dict(time_mode=lbmx.TIME_SINE, t_period=modulation[0], t_amp=modulation[1], t_offset=1.0)
+    obj.add_boundary(B(0, nx - 1, 0, 0), lbmx.make_bc(lbmx.BC_BOUNCEBACK))
+    obj.add_boundary(B(0, nx - 1, ny - 1, ny - 1), lbmx.make_bc(lbmx.BC_BOUNCEBACK))
+    obj.add_boundary(B(0, 0, 1, ny - 2), lbmx.make_bc(lbmx.BC_ZOUHE_VELOCITY, u_max, (1.0, 0.0, 0.0),
+                                                      lbmx.PROFILE_PARABOLIC, (mid, 0.0), (mid, 1.0), **kw))
+    obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_ZOUHE_PRESSURE, rho_out))
+    if objective != lbmx.OBJ_NONE:
+        inlet = B(0, 0, 1, ny - 2)
+        outlet = B(nx - 1, nx - 1, 1, ny - 2)
+        obj.set_objective(lbmx.make_objective(objective, inlet, outlet, time_const))
+
+
+def duct3d(obj, nx, ny, nz, u_in=0.04, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_DROP, time_const=1.0):
+    """Config-3 duct: Hecht velocity inlet on the west face, Hecht pressure outlet on the east
+    face, on-grid bounce-back on the four side walls (edges and corners included), as
+    applications/Diffuser3d/diffuser3d.cc:60-99 arranges its walls."""
+    B = lbmx.make_box
+    bb = lbmx.make_bc(lbmx.BC_BOUNCEBACK)
+    obj.add_boundary(B(0, nx - 1, 0, 0, 0, nz - 1), bb)
+    obj.add_boundary(B(0, nx - 1, ny - 1, ny - 1, 0, nz - 1), bb)
+    obj.add_boundary(B(0, nx - 1, 1, ny - 2, 0, 0), bb)
+    obj.add_boundary(B(0, nx - 1, 1, ny - 2, nz - 1, nz - 1), bb)
+    obj.add_boundary(B(0, 0, 1, ny - 2, 1, nz - 2), lbmx.make_bc(lbmx.BC_HECHT_VELOCITY, u_in, (1.0, 0.0, 0.0)))
+    obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2, 1, nz - 2), lbmx.make_bc(lbmx.BC_HECHT_PRESSURE, rho_out))
+    if objective != lbmx.OBJ_NONE:
+        inlet = B(0, 0, 1, ny - 2, 1, nz - 2)
+        outlet = B(nx - 1, nx - 1, 1, ny - 2, 1, nz - 2)
+        obj.set_objective(lbmx.make_objective(objective, inlet, outlet, time_const))
+
+
+def thermal_channel2d(obj, nx, ny, u_max=0.05, T_in=1.0, T_wall=0.0, time_const=1.0):
+    """Config-2 thermal channel: flow as channel2d; hot inlet T=1 (Dirichlet, west), cold
+    walls T=0 (Dirichlet, north/south), adiabatic outlet (bounce-back of the D2Q5 part);
+    objective: outlet heat flux (ThermalOutflowEast2d)."""
+    B = lbmx.make_box
+    channel2d(obj, nx, ny, u_max, objective=lbmx.OBJ_NONE)
+    obj.add_boundary(B(1, nx - 2, 0, 0), lbmx.make_bc(lbmx.BC_THERMAL_DIRICHLET, T_wall))
+    obj.add_boundary(B(1, nx - 2, ny - 1, ny - 1), lbmx.make_bc(lbmx.BC_THERMAL_DIRICHLET, T_wall))
+    obj.add_boundary(B(0, 0, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_DIRICHLET, T_in))
+    obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_BOUNCEBACK))
+    obj.set_objective(lbmx.make_objective(lbmx.OBJ_HEAT_FLUX, B(nx - 1, nx - 1, 1, ny - 2), None, time_const))

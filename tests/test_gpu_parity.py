@@ -1,0 +1,104 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle: FP64 plain form.
+
+Bars (written here, from the engine's contract): populations 1e-12 relative max-norm,
+objective 1e-10 relative, gradient 1e-10 relative max-norm.
+"""
+import numpy as np
+import pytest
+
+import lbmx
+from helpers import build_pair, relmax
+
+pytestmark = pytest.mark.gpu
+
+POP_TOL = 1e-12
+OBJ_TOL = 1e-10
+GRAD_TOL = 1e-10
+
+CASES_2D = [(lbmx.BGK, lbmx.DESIGN_PBB), (lbmx.MRT, lbmx.DESIGN_PBB),
+            (lbmx.BGK, lbmx.DESIGN_BRINKMAN), (lbmx.MRT, lbmx.DESIGN_BRINKMAN),
+            (lbmx.BGK, lbmx.DESIGN_NONE)]
+CASES_3D = [(lbmx.MRT, lbmx.DESIGN_PBB), (lbmx.MRT, lbmx.DESIGN_BRINKMAN), (lbmx.BGK, lbmx.DESIGN_BRINKMAN)]
+
+
+@pytest.mark.parametrize("coll,design", CASES_2D)
+def test_forward_2d(lbmx_mod, coll, design):
+    s, o, _ = build_pair(lbmx.D2Q9, coll, design, 70, 34)
+    s.init_equilibrium(); o.init_equilibrium()
+    Js = s.forward(150); Jo = o.forward(150)
+    assert s.timestep() == 150
+    assert relmax(s.get_state(), o.f) < POP_TOL
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+
+
+@pytest.mark.parametrize("coll,design", CASES_3D)
+def test_forward_3d(lbmx_mod, coll, design):
+    s, o, _ = build_pair(lbmx.D3Q19, coll, design, 40, 18, 14, setup="duct", omega=1.6)
+    s.init_equilibrium(); o.init_equilibrium()
+    Js = s.forward(60); Jo = o.forward(60)
+    assert relmax(s.get_state(), o.f) < POP_TOL
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+
+
+def test_forward_standard_mrt3d(lbmx_mod):
+    s, o, _ = build_pair(lbmx.D3Q19, lbmx.MRT_STANDARD, lbmx.DESIGN_NONE, 30, 14, 12, setup="duct", omega=1.6)
+    s.init_equilibrium(); o.init_equilibrium()
+    s.forward(40); o.forward(40)
+    assert relmax(s.get_state(), o.f) < POP_TOL
+
+
+@pytest.mark.parametrize("coll,design", CASES_2D[:4])
+def test_gradient_2d(lbmx_mod, coll, design):
+    s, o, _ = build_pair(lbmx.D2Q9, coll, design, 50, 26, time_const=1.0 / 80)
+    s.checkpoint_config(0)
+    Js, gs = s.gradient(80)
+    Jo, go, ao = o.gradient(80)
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+    assert relmax(gs, go) < GRAD_TOL
+    assert relmax(s.get_adjoint(), ao) < GRAD_TOL
+
+
+@pytest.mark.parametrize("coll,design", CASES_3D)
+def test_gradient_3d(lbmx_mod, coll, design):
+    s, o, _ = build_pair(lbmx.D3Q19, coll, design, 30, 14, 12, setup="duct", omega=1.6, time_const=1.0 / 30)
+    s.checkpoint_config(0)
+    Js, gs = s.gradient(30)
+    Jo, go, _ = o.gradient(30)
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+    assert relmax(gs, go) < GRAD_TOL
+
+
+def test_checkpointed_gradient_equals_stored(lbmx_mod):
+    s, _, _ = build_pair(lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_PBB, 40, 20, time_const=1.0 / 100)
+    s.checkpoint_config(0)
+    J0, g0 = s.gradient(100)
+    for slots in (2, 5, 17):
+        s.checkpoint_config(slots)
+        J1, g1 = s.gradient(100)
+        st = s.stats()
+        assert J1 == J0
+        assert np.array_equal(g1, g0), slots
+        assert st["recompute_sweeps"] > 0
+
+
+def test_thermal_forward_and_gradient(lbmx_mod):
+    s, o, _ = build_pair(lbmx.D2Q9_D2Q5, lbmx.MRT, lbmx.DESIGN_PBB, 40, 20, setup="thermal", time_const=1.0 / 40)
+    s.init_equilibrium(T0=0.5); o.init_equilibrium(T0=0.5)
+    s.forward(40); o.forward(40)
+    assert relmax(s.get_state(), o.f) < POP_TOL
+    s.checkpoint_config(0)
+    Js, gs = s.gradient(40, T0=0.5)
+    Jo, go, _ = o.gradient(40, T0=0.5)
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+    assert relmax(gs, go) < GRAD_TOL
+
+
+def test_set_state_roundtrip(lbmx_mod):
+    s, o, _ = build_pair(lbmx.D2Q9, lbmx.BGK, lbmx.DESIGN_PBB, 33, 17)
+    o.init_equilibrium()
+    o.forward(25)
+    s.set_state(o.f, 25)
+    assert np.array_equal(s.get_state(), o.f)
+    s.forward(10); o.forward(10)
+    assert s.timestep() == 35
+    assert relmax(s.get_state(), o.f) < POP_TOL
