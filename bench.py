@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: forward + adjoint MLUPS of the D3Q19 MRT design-gradient pass on one MI355X.
+
+Workload (the headline configuration): D3Q19 incompressible MRT with Brinkman design
+coupling, 384 x 192 x 192 duct (Hecht velocity inlet u=0.04, Hecht pressure outlet rho=1,
+bounce-back side walls), objective = pressure drop, T = 3000 time steps, dynamic (Wang)
+checkpointing with 48 HBM slots. Synthetic design field: SplitMix64 uniform, seed 0,
+5x5x5 box filter.
+
+One bench step = one design-gradient evaluation: equilibrium init, T recorded forward
+sweeps, T reverse sweeps with checkpoint recomputation. MLUPS counts every grid node once
+per time step:  value = nodes * T / (time of one step) / 1e6, FP64 plain form.
+The FP16-shifted-storage form is measured the same way and reported under "forms".
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lbm-topopt_amd"))
+
+METRIC = "forward+adjoint MLUPS and % peak HBM BW, D3Q19 MRT, FP64 plain vs FP16-storage"
+PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bytes_per_lu(form, Q=19):
+    """Algorithmic bytes per lattice update (forward, adjoint)."""
+    if form == "fp64":
+        sf, sl, sg = 8, 8, 8
+    else:
+        sf, sl, sg = 2, 4, 4
+    return 2 * Q * sf + sg, Q * sf + 2 * Q * sl + sg + 2 * 8
+
+
+def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, sweeps):
+    prec = {"fp64": lbmx.FP64, "fp16": lbmx.FP16_SHIFTED, "bf16": lbmx.BF16_SHIFTED, "fp32": lbmx.FP32}[form]
+    d = lbmx.make_desc(nx, ny, nz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
+                       precision=prec, omega=1.6, interp_q=0.1, alpha_max=0.1)
+    s = lbmx.Solver(d)
+    workloads.duct3d(s, nx, ny, nz, u_in=0.04, time_const=1.0 / horizon)
+    gamma = workloads.design_field(nx, ny, nz, seed=0)
+    s.set_design(gamma)
+    s.checkpoint_config(slots)
+    N = nx * ny * nz
+    # kernel-only sweep timing (HIP events on the engine's stream)
+    s.init_equilibrium()
+    s.time_sweeps(0, 5)
+    fwd_ms = s.time_sweeps(0, sweeps) / sweeps
+    s.time_sweeps(1, 3)
+    adj_ms = s.time_sweeps(1, sweeps) / sweeps
+    log(f"{form}: forward sweep {fwd_ms:.4f} ms, adjoint sweep {adj_ms:.4f} ms")
+    for w in range(warmup):
+        t0 = time.perf_counter()
+        s.gradient(horizon)
+        log(f"{form}: warmup step {w} {time.perf_counter() - t0:.2f} s")
+    times, fphase, rphase = [], [], []
+    J = None
+    for k in range(steps):
+        t0 = time.perf_counter()
+        J, sens = s.gradient(horizon, want_sens=(k == steps - 1))
+        dt = time.perf_counter() - t0
+        st = s.stats()
+        times.append(dt)
+        fphase.append(st["forward_phase_ms"] / 1e3)
+        rphase.append(st["reverse_phase_ms"] / 1e3)
+        log(f"{form}: step {k} {dt:.2f} s (forward {fphase[-1]:.2f} s, reverse {rphase[-1]:.2f} s, "
+            f"recompute sweeps {st['recompute_sweeps']})")
+    st = s.stats()
+    dev_bytes = s.device_bytes()
+    s.close()
+    t_step = max(times)
+    bf, ba = bytes_per_lu(form)
+    return {
+        "ms_per_step": 1e3 * t_step,
+        "mlups_fwd_adj": N * horizon / t_step / 1e6,
+        "mlups_forward_phase": N * horizon / max(fphase) / 1e6,
+        "mlups_reverse_phase": N * horizon / max(rphase) / 1e6,
+        "fwd_sweep_ms": fwd_ms,
+        "adj_sweep_ms": adj_ms,
+        "mlups_fwd_kernel": N / (fwd_ms * 1e3),
+        "mlups_adj_kernel": N / (adj_ms * 1e3),
+        "fwd_gbs": bf * N / (fwd_ms * 1e-3) / 1e9,
+        "adj_gbs": ba * N / (adj_ms * 1e-3) / 1e9,
+        "fwd_bytes_per_lu": bf,
+        "adj_bytes_per_lu": ba,
+        "objective": J,
+        "device_gb": dev_bytes / 1e9,
+    }
+
+
+def cpu_baseline(workloads, lbmx, nx=96, ny=48, nz=48, horizon=30):
+    """The CPU oracle (port of the reference's algorithm) on a bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # noqa: E402  (baseline leg only)
+    threads = min(16, os.cpu_count() or 1)
+    d = lbmx.make_desc(nx, ny, nz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
+                       omega=1.6, interp_q=0.1, alpha_max=0.1)
+    o = oracle.Oracle(d, threads=threads)
+    workloads.duct3d(o, nx, ny, nz, u_in=0.04, time_const=1.0 / horizon)
+    o.set_design(workloads.design_field(nx, ny, nz, seed=0))
+    t0 = time.perf_counter()
+    o.gradient(horizon)
+    dt = time.perf_counter() - t0
+    v = nx * ny * nz * horizon / dt / 1e6
+    log(f"cpu baseline: {v:.3f} MLUPS forward+adjoint on {threads} threads ({dt:.1f} s)")
+    return {"value": v, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"D3Q19 MRT Brinkman duct {nx}x{ny}x{nz}, {horizon} steps forward+adjoint "
+                      f"(every state stored), FP64, OpenMP over nodes"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--horizon", type=int, default=3000)
+    ap.add_argument("--slots", type=int, default=48)
+    ap.add_argument("--grid", type=str, default="384x192x192")
+    ap.add_argument("--sweeps", type=int, default=50)
+    ap.add_argument("--forms", type=str, default="fp64,fp16")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but "
+                                   "not built (slab decomposition, see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import lbmx
+    import workloads
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    forms = [f for f in args.forms.split(",") if f]
+    res = {}
+    for f in forms:
+        res[f] = run_form(lbmx, workloads, f, nx, ny, nz, args.horizon, args.slots, args.steps, args.warmup,
+                          args.sweeps)
+    head = res.get("fp64", res[forms[0]])
+    head_form = "fp64" if "fp64" in res else forms[0]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_d3q19_mrt_fwd.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get(head_form)
+    roof = {"bound": "hbm", "achieved": round(head["fwd_gbs"], 1), "peak": PEAK_GBS, "unit": "GB/s",
+            "frac": round(head["fwd_gbs"] / PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_fwd<D3Q19,MRT,%s> (forward sweep)" % head_form,
+            "bytes_per_lu": head["fwd_bytes_per_lu"],
+            "adjoint_kernel": {"achieved": round(head["adj_gbs"], 1), "frac": round(head["adj_gbs"] / PEAK_GBS, 4),
+                               "bytes_per_lu": head["adj_bytes_per_lu"]}}
+    out = {
+        "metric": METRIC,
+        "value": round(head["mlups_fwd_adj"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head_form == "fp64" else head_form,
+        "data": "synthetic (SplitMix64 design field seed 0, 5x5x5 box filter)",
+        "config": {"workload": f"D3Q19 MRT Brinkman duct {nx}x{ny}x{nz}, {args.horizon} time steps, "
+                               f"forward+adjoint with {args.slots} HBM checkpoints",
+                   "nodes": nx * ny * nz, "time_steps": args.horizon, "checkpoints": args.slots},
+        "roofline": roof,
+        "forms": {f: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for f, r in res.items()},
+    }
+    if "fp16" in res and "fp64" in res:
+        out["fp16_over_fp64"] = round(res["fp16"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(workloads, lbmx)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -227,6 +227,11 @@ typedef struct {
   long long recompute_sweeps;
 } lbmx_stats;
 lbmx_status lbmx_get_stats(lbmx_solver* s, lbmx_stats* st);
+/* Times nsweeps back-to-back sweeps with HIP events on the handle's stream. kind 0: forward
+   sweeps from the current state (the state advances, objective accumulation on); kind 1:
+   adjoint sweeps reading the current state as p_{t-1} with ping-ponged adjoint buffers.
+   *ms = total device time of the sweeps. */
+lbmx_status lbmx_time_sweeps(lbmx_solver* s, int kind, int nsweeps, double* ms);
 /* Bytes of device memory held by the handle (state, adjoint, checkpoints). */
 lbmx_status lbmx_device_bytes(lbmx_solver* s, size_t* bytes);
 

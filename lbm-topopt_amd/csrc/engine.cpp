@@ -629,6 +629,68 @@ struct Engine {
       }
   }
 
+  double time_sweeps(int kind, int nsweeps) {
+    require_state();
+    if (nsweeps < 1) throw Error{LBMX_ERR_ARG, "nsweeps must be >= 1"};
+    if (kind == 1 && !inc) throw Error{LBMX_ERR_SUP, "no adjoint for the standard (compressible) operators"};
+    prepare();
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    float ms = 0;
+    if (kind == 0) {
+      const bool with_obj = obj_set && obj.nterm > 0;
+      if (with_obj) zero_acc();
+      HIP_OK(hipEventRecord(e0, stream));
+      for (int k = 0; k < nsweeps; ++k) {
+        FwdArgs a = fwd_args(cur, prev, t + 1, with_obj);
+        HIP_OK(run_fwd(a));
+        std::swap(cur, prev);
+        ++t;
+      }
+      HIP_OK(hipEventRecord(e1, stream));
+      state_kind = 0;
+      stats.fwd_sweeps += nsweeps;
+    } else {
+      lam_a.ensure(lam_bytes());
+      lam_b.ensure(lam_bytes());
+      sens_dev.ensure((size_t)N * 8);
+      zero_async(lam_a.p, lam_bytes(), stream);
+      zero_async(lam_b.p, lam_bytes(), stream);
+      zero_async(sens_dev.p, (size_t)N * 8, stream);
+      ObjArgs oa = obj;
+      oa.acc = nullptr;
+      if (!obj_set) oa.nterm = 0;
+      void* lin = lam_a.p;
+      void* lout = lam_b.p;
+      HIP_OK(hipEventRecord(e0, stream));
+      for (int k = 0; k < nsweeps; ++k) {
+        AdjArgs a{};
+        a.fwd_src = cur;
+        a.lam_in = lin;
+        a.lam_out = lout;
+        a.gamma = gamma_dev.p;
+        a.sens = static_cast<double*>(sens_dev.p);
+        a.bcs = static_cast<const DevBC*>(bc_dev.p);
+        a.nbc = (int)bcs.size();
+        a.t = std::max(1, t);
+        a.g = geom();
+        a.m = params();
+        a.obj = oa;
+        HIP_OK(run_adj(a));
+        std::swap(lin, lout);
+      }
+      HIP_OK(hipEventRecord(e1, stream));
+      stats.adj_sweeps += nsweeps;
+    }
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kind == 0) stats.fwd_kernel_ms = ms; else stats.adj_kernel_ms = ms;
+    return ms;
+  }
+
   size_t device_bytes() const {
     size_t b = gamma_dev.bytes + bc_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
                lam_b.bytes + raw_aos.bytes;
@@ -903,6 +965,12 @@ lbmx_status lbmx_get_stats(lbmx_solver* s, lbmx_stats* st) {
   if (!st) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   *st = s->e->stats;
   return LBMX_OK;
+}
+
+lbmx_status lbmx_time_sweeps(lbmx_solver* s, int kind, int nsweeps, double* ms) {
+  CHECK_HANDLE(s);
+  if (!ms) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { *ms = s->e->time_sweeps(kind, nsweeps); });
 }
 
 lbmx_status lbmx_device_bytes(lbmx_solver* s, size_t* bytes) {

@@ -135,6 +135,7 @@ def library():
         "lbmx_get_adjoint": [C.c_void_p, dp, C.c_size_t],
         "lbmx_get_stats": [C.c_void_p, P(Stats)],
         "lbmx_device_bytes": [C.c_void_p, P(C.c_size_t)],
+        "lbmx_time_sweeps": [C.c_void_p, C.c_int, C.c_int, dp],
         "lbmx_describe_boundaries": [C.c_void_p, P(C.c_int), C.c_int, P(C.c_int)],
         "lbmx_checkpoint_simulate": [C.c_int, C.c_int, P(C.c_longlong), P(C.c_int)],
     }
@@ -153,7 +154,7 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_init_equilibrium_field", "lbmx_forward", "lbmx_get_state", "lbmx_set_state",
             "lbmx_get_macros", "lbmx_get_timestep", "lbmx_checkpoint_config", "lbmx_gradient",
             "lbmx_get_sensitivity", "lbmx_get_adjoint", "lbmx_get_stats", "lbmx_device_bytes",
-            "lbmx_describe_boundaries", "lbmx_checkpoint_simulate"]
+            "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps"]
 
 
 def checkpoint_simulate(nslots, nsteps):
@@ -281,6 +282,12 @@ class Solver:
         st = Stats()
         _check(self.lib.lbmx_get_stats(self.h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def time_sweeps(self, kind, nsweeps):
+        """Device time (ms, HIP events) of nsweeps forward (kind 0) or adjoint (kind 1) sweeps."""
+        ms = C.c_double(0.0)
+        _check(self.lib.lbmx_time_sweeps(self.h, kind, nsweeps, C.byref(ms)))
+        return ms.value
 
     def boundaries(self):
         cnt = C.c_int(0)

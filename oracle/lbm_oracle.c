@@ -499,21 +499,28 @@ static void base_collide(const orc_problem* p, const double* f, double* out, dou
   /* MRT: f' = f - M^-1 S (M f - m_eq)   (src/dynamics/incompressibleMRT2d.hh:44-86,
      src/dynamics/MRT3d.hh:44-176, :222-359) */
   const orc_mrt* m = &p->mrt;
-  double dm[QMAX], srcm[QMAX];
+  double dm[QMAX], srcm[QMAX], src[QMAX];
+  if (forced)
+    for (int q = 0; q < Q; ++q) src[q] = guo_src(p, q, v, alpha);
   for (int k = 0; k < Q; ++k) {
     double mk = 0;
-    for (int q = 0; q < Q; ++q) mk += m->M[k][q] * f[q];
-    dm[k] = row_s(&m->row[k], om) * (mk - row_meq(&m->row[k], rho, v, rho_s)) * m->inv_norm2[k];
+    dm[k] = 0;
     srcm[k] = 0;
+    if (m->row[k].rate != 0) {
+      for (int q = 0; q < Q; ++q)
+        if (m->M[k][q] != 0.) mk += m->M[k][q] * f[q];
+      dm[k] = row_s(&m->row[k], om) * (mk - row_meq(&m->row[k], rho, v, rho_s)) * m->inv_norm2[k];
+    }
     if (forced) {
       double sk = 0;
-      for (int q = 0; q < Q; ++q) sk += m->M[k][q] * guo_src(p, q, v, alpha);
+      for (int q = 0; q < Q; ++q) sk += m->M[k][q] * src[q];
       srcm[k] = (1. - 0.5 * row_s(&m->row[k], om)) * sk * m->inv_norm2[k];
     }
   }
   for (int q = 0; q < Q; ++q) {
     double acc = f[q];
-    for (int k = 0; k < Q; ++k) acc += m->M[k][q] * (srcm[k] - dm[k]);
+    for (int k = 0; k < Q; ++k)
+      if (m->M[k][q] != 0. && (forced || m->row[k].rate != 0)) acc += m->M[k][q] * (srcm[k] - dm[k]);
     out[q] = acc;
   }
 }
@@ -1135,14 +1142,16 @@ static void collide_all(const orc_problem* p, double* f) {
 void orc_forward(const orc_problem* p, double* f, int nsteps, int* t, double* J) {
   size_t len = p->nodes * p->Q;
   double* tmp = (double*)malloc(len * sizeof(double));
+  double *a = f, *b = tmp;
   for (int s = 0; s < nsteps; ++s) {
-    collide_all(p, f);
-    orc_stream(p, f, tmp);
-    memcpy(f, tmp, len * sizeof(double));
+    collide_all(p, a);
+    orc_stream(p, a, b);
+    double* sw = a; a = b; b = sw;
     ++*t;
-    for (int b = 0; b < p->nbc; ++b) orc_apply_bc(p, b, *t, f);
-    if (J) *J += orc_objective(p, f, *t);
+    for (int k = 0; k < p->nbc; ++k) orc_apply_bc(p, k, *t, a);
+    if (J) *J += orc_objective(p, a, *t);
   }
+  if (a != f) memcpy(f, a, len * sizeof(double));
   free(tmp);
 }
 
