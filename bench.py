@@ -174,6 +174,8 @@ def main():
     }
     if "fp16" in res and "fp64" in res:
         out["fp16_over_fp64"] = round(res["fp16"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
+    out["stream_copy_gbs"] = round(lbmx.stream_copy_gbs(), 1)
+    log(f"stream copy ceiling {out['stream_copy_gbs']} GB/s")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(workloads, lbmx)
     print(json.dumps(out), flush=True)

@@ -232,6 +232,9 @@ lbmx_status lbmx_get_stats(lbmx_solver* s, lbmx_stats* st);
    adjoint sweeps reading the current state as p_{t-1} with ping-ponged adjoint buffers.
    *ms = total device time of the sweeps. */
 lbmx_status lbmx_time_sweeps(lbmx_solver* s, int kind, int nsweeps, double* ms);
+/* Measured HBM ceiling: iters launches of a streaming copy of `bytes` (read + write);
+   *gbs = (2 * bytes * iters) / device time. Also the calibration kernel for byte counters. */
+lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs);
 /* Bytes of device memory held by the handle (state, adjoint, checkpoints). */
 lbmx_status lbmx_device_bytes(lbmx_solver* s, size_t* bytes);
 

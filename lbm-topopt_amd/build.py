@@ -23,7 +23,8 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-UNITS = ["kernels_d3q19.hip", "kernels_d2q9.hip", "kernels_thermal.hip", "engine.cpp"]
+UNITS = ([f"kernels_{lat}_{p}.hip" for lat in ("d3q19", "d2q9", "thermal") for p in ("f64", "f32", "f16", "bf16")]
+         + ["engine.cpp", "probe.hip"])
 
 
 def _compile(unit):

@@ -79,13 +79,16 @@ struct Engine {
   hipStream_t stream = nullptr;
 
   std::vector<DevBC> bcs;           // pieces in insertion order
+  std::vector<DevTerm> terms;       // functional terms
+  bool terms_dirty = true;
+  Layout ls{}, ll{};                // padded layouts of state and adjoint fields
   std::vector<double> gamma_host;   // design field
   bool gamma_dirty = true;
   ObjArgs obj{};                    // acc pointer filled at launch
   bool obj_set = false;
 
   // device
-  DeviceBuffer gamma_dev, bc_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
+  DeviceBuffer gamma_dev, bc_dev, term_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
   std::vector<std::unique_ptr<DeviceBuffer>> pool;  // state buffers: [0],[1] working; rest slots
   bool bc_dirty = true;
   void* cur = nullptr;   // p_t
@@ -119,7 +122,13 @@ struct Engine {
     if (d.design == LBMX_DESIGN_DARCY) throw Error{LBMX_ERR_SUP, "DarcyMRT2d coupling is not available"};
     inc = !(d.collision == LBMX_MRT_STANDARD || d.collision == LBMX_BGK_STANDARD);
     N = (long long)d.nx * d.ny * d.nz;
+    if (N >= (1LL << 31)) throw Error{LBMX_ERR_SUP, "grids of 2^31 nodes or more are not supported"};
     gamma_host.assign(N, 1.0);
+    ls = make_layout(d.nx, d.ny, d.nz, D, (int)prec_state_bytes(d.precision));
+    ll = make_layout(d.nx, d.ny, d.nz, D, (int)prec_lam_bytes(d.precision));
+    if ((unsigned long long)ls.plane * prec_state_bytes(d.precision) >= (1ULL << 32) ||
+        (unsigned long long)ll.plane * prec_lam_bytes(d.precision) >= (1ULL << 32))
+      throw Error{LBMX_ERR_SUP, "a population plane must stay below 4 GiB (32-bit lane offsets)"};
     obj.nterm = 0;
   }
 
@@ -128,7 +137,15 @@ struct Engine {
   }
   bool touched = false;
 
-  Geometry geom() const { return Geometry{d.nx, d.ny, d.nz, 0, N}; }
+  Geometry geom() const {
+    Geometry g{};
+    g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
+    g.nblk_x = (d.nx + kTileX - 1) / kTileX;
+    g.nblk_y = (d.ny + kTileY - 1) / kTileY;
+    g.nblocks = g.nblk_x * g.nblk_y * d.nz;
+    g.N = N;
+    return g;
+  }
 
   ModelParams params() const {
     ModelParams m{};
@@ -143,11 +160,9 @@ struct Engine {
     return m;
   }
 
-  size_t state_bytes() const { return (size_t)N * Q * prec_state_bytes(d.precision); }
-  size_t lam_bytes() const { return (size_t)N * Q * prec_lam_bytes(d.precision); }
-  int grid_blocks() const {
-    return ((d.nx + kTileX - 1) / kTileX) * ((d.ny + kTileY - 1) / kTileY) * d.nz;
-  }
+  size_t state_bytes() const { return (size_t)ls.plane * Q * prec_state_bytes(d.precision); }
+  size_t lam_bytes() const { return (size_t)ll.plane * Q * prec_lam_bytes(d.precision); }
+  int grid_blocks() const { return geom().nblocks; }
 
   // ---- boundary pieces: NewLBSolver2d::addBoundaryCondition (src/latticeBoltzmann/NewLBSolver2d.cc:106-155),
   //      getBoxBoundaries (src/core/geometry2d.cc:31-53, src/core/geometry3d.cc:36-85) ----
@@ -246,6 +261,13 @@ struct Engine {
         (bc.radius[0] == 0.0))
       throw Error{LBMX_ERR_ARG, "profile radius must be non-zero"};
     if (bc.time_mode == LBMX_TIME_SINE && bc.t_period == 0.0) throw Error{LBMX_ERR_ARG, "sine period must be non-zero"};
+    int npieces = 0;
+    for (auto& pc : ps) {
+      int o[6];
+      if (intersect(b, pc.box, o)) ++npieces;
+    }
+    if ((int)bcs.size() + npieces > kMaxBC)
+      throw Error{LBMX_ERR_SUP, "more than " + std::to_string(kMaxBC) + " boundary pieces"};
     for (auto& pc : ps) {
       int o[6];
       if (!intersect(b, pc.box, o)) continue;
@@ -269,6 +291,7 @@ struct Engine {
   // ---- functional terms (src/functionals/*.hh) ----
   void set_objective(const lbmx_objective& o) {
     ObjArgs a{};
+    std::vector<DevTerm> tv;
     a.time_const = o.time_const;
     auto size = [](const lbmx_box& b) {
       return (double)(b.x1 - b.x0 + 1) * (b.y1 - b.y0 + 1) * (b.z1 - b.z0 + 1);
@@ -277,7 +300,9 @@ struct Engine {
       if (b.x1 < b.x0 || b.y1 < b.y0 || b.z1 < b.z0) throw Error{LBMX_ERR_ARG, "empty objective box"};
       if (b.x0 < 0 || b.y0 < 0 || b.z0 < 0 || b.x1 >= d.nx || b.y1 >= d.ny || b.z1 >= d.nz)
         throw Error{LBMX_ERR_ARG, "objective box outside the grid"};
-      DevTerm& t = a.terms[a.nterm++];
+      tv.emplace_back();
+      a.nterm = (int)tv.size();
+      DevTerm& t = tv.back();
       t.box[0] = b.x0; t.box[1] = b.x1; t.box[2] = b.y0; t.box[3] = b.y1; t.box[4] = b.z0; t.box[5] = b.z1;
       t.weight = w;
       t.kind = kind;
@@ -302,11 +327,13 @@ struct Engine {
     int bb[6] = {INT32_MAX, -1, INT32_MAX, -1, INT32_MAX, -1};
     for (int m = 0; m < a.nterm; ++m)
       for (int k = 0; k < 3; ++k) {
-        bb[2 * k] = std::min(bb[2 * k], a.terms[m].box[2 * k]);
-        bb[2 * k + 1] = std::max(bb[2 * k + 1], a.terms[m].box[2 * k + 1]);
+        bb[2 * k] = std::min(bb[2 * k], tv[m].box[2 * k]);
+        bb[2 * k + 1] = std::max(bb[2 * k + 1], tv[m].box[2 * k + 1]);
       }
     for (int k = 0; k < 6; ++k) a.bbox[k] = bb[k];
     obj = a;
+    terms = tv;
+    terms_dirty = true;
     obj_set = true;
   }
 
@@ -329,6 +356,13 @@ struct Engine {
       }
       gamma_dirty = false;
     }
+    if (terms_dirty) {
+      term_dev.ensure(std::max<size_t>(1, terms.size()) * sizeof(DevTerm));
+      if (!terms.empty())
+        HIP_OK(hipMemcpyAsync(term_dev.p, terms.data(), terms.size() * sizeof(DevTerm), hipMemcpyHostToDevice, stream));
+      HIP_OK(hipStreamSynchronize(stream));
+      terms_dirty = false;
+    }
     if (bc_dirty) {
       bc_dev.ensure(std::max<size_t>(1, bcs.size()) * sizeof(DevBC));
       if (!bcs.empty())
@@ -345,8 +379,20 @@ struct Engine {
     }
   }
 
+  // time factor of each piece at step t (the functor's t-dependence, e.g. the sinusoidal
+  // inlet modulation of applications/RobustPump2d/robustPump2d.cc:14-19)
+  void time_factors(int tstep, double* tf) const {
+    for (size_t b = 0; b < bcs.size(); ++b) {
+      const DevBC& e = bcs[b];
+      tf[b] = e.time_mode == LBMX_TIME_SINE
+                  ? e.t_offset + e.t_amp * std::sin(2.0 * M_PI * tstep / e.t_period + e.t_phase)
+                  : 1.0;
+    }
+  }
+
   FwdArgs fwd_args(const void* src, void* dst, int tnew, bool with_obj) {
     FwdArgs a{};
+    time_factors(tnew, a.tfac);
     a.src = src;
     a.dst = dst;
     a.gamma = gamma_dev.p;
@@ -355,32 +401,69 @@ struct Engine {
     a.t = tnew;
     a.mode = 0;
     a.g = geom();
+    a.ls = ls;
     a.m = params();
     a.obj = obj;
+    a.obj.terms = static_cast<const DevTerm*>(term_dev.p);
     a.obj.acc = (with_obj && obj_set && obj.nterm > 0) ? static_cast<double*>(acc_dev.p) : nullptr;
     return a;
   }
 
+#define LBMX_PREC_SWITCH(LAT, CALL)                        \
+  switch (d.precision) {                                   \
+    case LBMX_FP64: return CALL(LAT##_f64);                \
+    case LBMX_FP32: return CALL(LAT##_f32);                \
+    case LBMX_FP16_SHIFTED: return CALL(LAT##_f16);        \
+    default: return CALL(LAT##_bf16);                      \
+  }
   hipError_t run_fwd(const FwdArgs& a) {
+#define CALL(S) launch_fwd_##S(d.collision, d.design, a, stream)
     switch (d.lattice) {
-      case LBMX_D2Q9: return launch_fwd_d2q9(d.collision, d.precision, a, stream);
-      case LBMX_D3Q19: return launch_fwd_d3q19(d.collision, d.precision, a, stream);
-      default: return launch_fwd_thermal(d.collision, d.precision, a, stream);
+      case LBMX_D2Q9: LBMX_PREC_SWITCH(d2q9, CALL)
+      case LBMX_D3Q19: LBMX_PREC_SWITCH(d3q19, CALL)
+      default: LBMX_PREC_SWITCH(thermal, CALL)
     }
+#undef CALL
   }
   hipError_t run_init(const FwdArgs& a, const double* macros) {
+#define CALL(S) launch_init_##S(d.collision, d.design, a, macros, init_u, init_rho, init_T, stream)
     switch (d.lattice) {
-      case LBMX_D2Q9: return launch_init_d2q9(d.collision, d.precision, a, macros, init_u, init_rho, init_T, stream);
-      case LBMX_D3Q19: return launch_init_d3q19(d.collision, d.precision, a, macros, init_u, init_rho, init_T, stream);
-      default: return launch_init_thermal(d.collision, d.precision, a, macros, init_u, init_rho, init_T, stream);
+      case LBMX_D2Q9: LBMX_PREC_SWITCH(d2q9, CALL)
+      case LBMX_D3Q19: LBMX_PREC_SWITCH(d3q19, CALL)
+      default: LBMX_PREC_SWITCH(thermal, CALL)
     }
+#undef CALL
   }
   hipError_t run_adj(const AdjArgs& a) {
+#define CALL(S) launch_adj_##S(d.collision, d.design, a, stream)
     switch (d.lattice) {
-      case LBMX_D2Q9: return launch_adj_d2q9(d.collision, d.precision, a, stream);
-      case LBMX_D3Q19: return launch_adj_d3q19(d.collision, d.precision, a, stream);
-      default: return launch_adj_thermal(d.collision, d.precision, a, stream);
+      case LBMX_D2Q9: LBMX_PREC_SWITCH(d2q9, CALL)
+      case LBMX_D3Q19: LBMX_PREC_SWITCH(d3q19, CALL)
+      default: LBMX_PREC_SWITCH(thermal, CALL)
     }
+#undef CALL
+  }
+#undef LBMX_PREC_SWITCH
+
+  AdjArgs adj_args(const void* fsrc, const void* lin, void* lout, int tstep, const ObjArgs& oa) {
+    AdjArgs a{};
+    a.fwd_src = fsrc;
+    a.lam_in = lin;
+    a.lam_out = lout;
+    a.gamma = gamma_dev.p;
+    a.sens = static_cast<double*>(sens_dev.p);
+    a.bcs = static_cast<const DevBC*>(bc_dev.p);
+    a.nbc = (int)bcs.size();
+    a.t = tstep;
+    time_factors(tstep, a.tfac);
+    a.g = geom();
+    a.ls = ls;
+    a.ll = ll;
+    a.m = params();
+    a.obj = oa;
+    a.obj.terms = static_cast<const DevTerm*>(term_dev.p);
+    a.obj.acc = nullptr;
+    return a;
   }
 
   void prepare() {
@@ -579,18 +662,7 @@ struct Engine {
           fsrc = s2;
         }
       }
-      AdjArgs a{};
-      a.fwd_src = fsrc;
-      a.lam_in = lin;
-      a.lam_out = lout;
-      a.gamma = gamma_dev.p;
-      a.sens = static_cast<double*>(sens_dev.p);
-      a.bcs = static_cast<const DevBC*>(bc_dev.p);
-      a.nbc = (int)bcs.size();
-      a.t = step;
-      a.g = geom();
-      a.m = params();
-      a.obj = oa;
+      AdjArgs a = adj_args(fsrc, lin, lout, step, oa);
       HIP_OK(run_adj(a));
       std::swap(lin, lout);
       if (!naive) sched->release();
@@ -619,14 +691,54 @@ struct Engine {
   void get_adjoint(double* out, size_t n) {
     if (!grad_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
     if ((long long)n != N * Q) throw Error{LBMX_ERR_ARG, "adjoint length != nodes * Q"};
-    size_t eb = prec_lam_bytes(d.precision);
-    std::vector<char> h((size_t)N * Q * eb);
+    const size_t eb = prec_lam_bytes(d.precision);
+    std::vector<char> h(lam_bytes());
     HIP_OK(hipMemcpy(h.data(), lam_a.p, h.size(), hipMemcpyDeviceToHost));
-    for (long long node = 0; node < N; ++node)
-      for (int q = 0; q < Q; ++q) {
-        size_t i = (size_t)q * N + node;
-        out[node * Q + q] = eb == 8 ? reinterpret_cast<double*>(h.data())[i] : reinterpret_cast<float*>(h.data())[i];
-      }
+    auto val = [&](long long e) -> double {
+      return eb == 8 ? reinterpret_cast<const double*>(h.data())[e] : reinterpret_cast<const float*>(h.data())[e];
+    };
+    const int* ex = lat_ex();
+    const int* ey = lat_ey();
+    const int* ez = lat_ez();
+    const int* op = lat_opp();
+    for (int z = 0; z < d.nz; ++z)
+      for (int y = 0; y < d.ny; ++y)
+        for (int x = 0; x < d.nx; ++x) {
+          const long long node = ((long long)z * d.ny + y) * d.nx + x;
+          const long long off = ((long long)(z + ll.oz) * ll.py + (y + 1)) * ll.px + ll.ax + x;
+          for (int q = 0; q < Q; ++q) {
+            const int xo = x + ex[q], yo = y + ey[q], zo = z + ez[q];
+            const bool out_dom = (ex[q] || ey[q] || ez[q]) &&
+                                 (xo < 0 || xo >= d.nx || yo < 0 || yo >= d.ny || zo < 0 || zo >= d.nz);
+            // pushed into the halo cell x + c_q, plane opp(q), when x + c_q is outside
+            const long long e = out_dom ? (long long)op[q] * ll.plane + off +
+                                              (ex[q] + (long long)ey[q] * ll.px + (long long)ez[q] * ll.px * ll.py)
+                                        : (long long)q * ll.plane + off;
+            out[node * Q + q] = val(e);
+          }
+        }
+  }
+
+  // lattice velocity tables on the host (same order as the device lattices)
+  const int* lat_ex() const {
+    static const int d2[14] = {0, 1, 0, -1, -1, -1, 0, 1, 1, 0, 0, -1, 0, 1};
+    static const int d3[19] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+    return D == 3 ? d3 : d2;
+  }
+  const int* lat_ey() const {
+    static const int d2[14] = {0, -1, -1, -1, 0, 1, 1, 1, 0, 0, -1, 0, 1, 0};
+    static const int d3[19] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+    return D == 3 ? d3 : d2;
+  }
+  const int* lat_ez() const {
+    static const int d2[14] = {0};
+    static const int d3[19] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+    return D == 3 ? d3 : d2;
+  }
+  const int* lat_opp() const {
+    static const int d2[14] = {0, 5, 6, 7, 8, 1, 2, 3, 4, 9, 12, 13, 10, 11};
+    static const int d3[19] = {0, 10, 11, 12, 13, 14, 15, 16, 17, 18, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+    return D == 3 ? d3 : d2;
   }
 
   double time_sweeps(int kind, int nsweeps) {
@@ -665,18 +777,7 @@ struct Engine {
       void* lout = lam_b.p;
       HIP_OK(hipEventRecord(e0, stream));
       for (int k = 0; k < nsweeps; ++k) {
-        AdjArgs a{};
-        a.fwd_src = cur;
-        a.lam_in = lin;
-        a.lam_out = lout;
-        a.gamma = gamma_dev.p;
-        a.sens = static_cast<double*>(sens_dev.p);
-        a.bcs = static_cast<const DevBC*>(bc_dev.p);
-        a.nbc = (int)bcs.size();
-        a.t = std::max(1, t);
-        a.g = geom();
-        a.m = params();
-        a.obj = oa;
+        AdjArgs a = adj_args(cur, lin, lout, std::max(1, t), oa);
         HIP_OK(run_adj(a));
         std::swap(lin, lout);
       }
@@ -692,7 +793,7 @@ struct Engine {
   }
 
   size_t device_bytes() const {
-    size_t b = gamma_dev.bytes + bc_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
+    size_t b = gamma_dev.bytes + bc_dev.bytes + term_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
                lam_b.bytes + raw_aos.bytes;
     for (auto& p : pool) b += p->bytes;
     return b;

@@ -1,0 +1,9 @@
+// Sweeps for D3Q19: incompressible BGK / MRT (forward + adjoint), standard MRT / BGK (forward); storage bf16.
+#include "lbmx_kernels.hpp"
+#include "lbmx_launch.h"
+#define LAT LatD3Q19
+#define PREC PrecBF16
+#define SUFFIX d3q19_bf16
+#define FWD_OPS X(C_BGK) X(C_MRT) X(C_MRT_STD) X(C_BGK_STD)
+#define ADJ_OPS X(C_BGK) X(C_MRT)
+#include "lbmx_dispatch.inc"

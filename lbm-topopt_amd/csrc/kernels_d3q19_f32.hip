@@ -1,8 +1,9 @@
-// D3Q19 sweeps: incompressible BGK / MRT (forward + adjoint), standard MRT / BGK (forward).
+// Sweeps for D3Q19: incompressible BGK / MRT (forward + adjoint), standard MRT / BGK (forward); storage f32.
 #include "lbmx_kernels.hpp"
 #include "lbmx_launch.h"
 #define LAT LatD3Q19
-#define SUFFIX d3q19
+#define PREC PrecF32
+#define SUFFIX d3q19_f32
 #define FWD_OPS X(C_BGK) X(C_MRT) X(C_MRT_STD) X(C_BGK_STD)
 #define ADJ_OPS X(C_BGK) X(C_MRT)
 #include "lbmx_dispatch.inc"

@@ -1,0 +1,9 @@
+// Sweeps for D2Q9 + D2Q5 double distribution (flow BGK / MRT with thermal MRT), forward + adjoint; storage f32.
+#include "lbmx_kernels.hpp"
+#include "lbmx_launch.h"
+#define LAT LatD2Q9D2Q5
+#define PREC PrecF32
+#define SUFFIX thermal_f32
+#define FWD_OPS X(C_BGK) X(C_MRT)
+#define ADJ_OPS X(C_BGK) X(C_MRT)
+#include "lbmx_dispatch.inc"

@@ -1,0 +1,523 @@
+// Collision operators and their exact reverse operators, specialised at compile time on the
+// lattice, the operator (BGK / MRT, incompressible or standard), the design coupling
+// (none / partial bounce-back / Brinkman) and the arithmetic type.
+//
+// The flow populations are handled in "pair space": f_0, s_d = f_d + f_d' and a_d = f_d - f_d'
+// for the opposite pairs (d, d' = d + HALF). Every moment-basis row is even or odd in c, so
+// moments read only s (even rows) or a (odd rows) and the back transform updates s by -2E and
+// a by -2O: half the multiply-adds of a direct M f / M^T dm. Partial bounce-back only scales
+// a by (1 - 2 imult). The macros are rho = f_0 + sum s_d, j = sum c_d a_d. The moments of the
+// Guo source w_q[3(c-v).F + 9(c.v)(c.F)] are contractions with compile-time tables.
+#pragma once
+#include "lbmx_device.hpp"
+
+namespace lbmx {
+
+enum : int { D_NONE = LBMX_DESIGN_NONE, D_PBB = LBMX_DESIGN_PBB, D_BRINK = LBMX_DESIGN_BRINKMAN };
+
+#define LBMX_BAS BasisOf<L::QF>::B
+
+template <class T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// BPInterpolation (src/core/interpolationFunctions.hh:18-25)
+template <class T> __device__ __forceinline__ T bp_val(T q, T s) { return T(1) - s * (T(1) + q) * recip<T>(q + s); }
+template <class T> __device__ __forceinline__ T bp_der(T q, T s) {
+  T r = recip<T>(q + s);
+  return -(T(1) + q) * r + s * (T(1) + q) * (r * r);
+}
+
+template <class L, class T>
+struct Pairs {
+  T f0;
+  T s[L::HALF + 1];
+  T a[L::HALF + 1];
+};
+
+template <class L, class T>
+__device__ __forceinline__ void to_pairs(const T (&f)[L::Q], Pairs<L, T>& p) {
+  p.f0 = f[0];
+  p.s[0] = p.a[0] = T(0);
+  static_for<1, L::HALF + 1>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    p.s[d] = f[d] + f[d + L::HALF];
+    p.a[d] = f[d] - f[d + L::HALF];
+  });
+}
+
+template <class L, class T>
+__device__ __forceinline__ void from_pairs(const Pairs<L, T>& p, T (&f)[L::Q]) {
+  f[0] = p.f0;
+  static_for<1, L::HALF + 1>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    f[d] = T(0.5) * (p.s[d] + p.a[d]);
+    f[d + L::HALF] = T(0.5) * (p.s[d] - p.a[d]);
+  });
+}
+
+template <class L, class T>
+__device__ __forceinline__ void pair_macros(const Pairs<L, T>& p, T& rho, T (&j)[3]) {
+  T r = p.f0, jx = 0, jy = 0, jz = 0;
+  static_for<1, L::HALF + 1>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    r += p.s[d];
+    if constexpr (L::ex[d] == 1) jx += p.a[d];
+    if constexpr (L::ex[d] == -1) jx -= p.a[d];
+    if constexpr (L::ey[d] == 1) jy += p.a[d];
+    if constexpr (L::ey[d] == -1) jy -= p.a[d];
+    if constexpr (L::ez[d] == 1) jz += p.a[d];
+    if constexpr (L::ez[d] == -1) jz -= p.a[d];
+  });
+  rho = r; j[0] = jx; j[1] = jy; j[2] = jz;
+}
+
+template <class L, int k>
+constexpr bool row_has_src() {
+  if (LBMX_BAS.G0[k] != 0) return true;
+  for (int a = 0; a < 3; ++a) {
+    if (LBMX_BAS.G1[k][a] != 0) return true;
+    for (int b = 0; b < 3; ++b)
+      if (LBMX_BAS.G2[k][a][b] != 0) return true;
+  }
+  return false;
+}
+
+template <class L, int DES, int k>
+constexpr bool row_active() {
+  return LBMX_BAS.rate[k] != 0 || (DES == D_BRINK && row_has_src<L, k>());
+}
+
+// row k of M applied in pair space
+template <class L, int k, class T>
+__device__ __forceinline__ T pair_moment(const Pairs<L, T>& p) {
+  static_assert(LBMX_BAS.par[k] != 0, "basis row without parity");
+  T m = 0;
+  if constexpr (LBMX_BAS.par[k] == 1) {
+    if constexpr (LBMX_BAS.M[k][0] != 0) m += T(LBMX_BAS.M[k][0]) * p.f0;
+    static_for<1, L::HALF + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      if constexpr (LBMX_BAS.M[k][d] != 0) m += T(LBMX_BAS.M[k][d]) * p.s[d];
+    });
+  } else {
+    static_for<1, L::HALF + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      if constexpr (LBMX_BAS.M[k][d] != 0) m += T(LBMX_BAS.M[k][d]) * p.a[d];
+    });
+  }
+  return m;
+}
+
+// x -= M^T c over the rows selected by Rows (and by parity when ParMask != 0), in pair space
+template <class L, class Rows, int ParMask, class T>
+__device__ __forceinline__ void pair_back(Pairs<L, T>& x, const T (&c)[L::QF]) {
+  constexpr int Q = L::QF;
+  if constexpr (ParMask != -1) {
+    T e0 = 0;
+    static_for<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if constexpr (Rows::template on<k>() && LBMX_BAS.par[k] == 1 && LBMX_BAS.M[k][0] != 0)
+        e0 += T(LBMX_BAS.M[k][0]) * c[k];
+    });
+    x.f0 -= e0;
+  }
+  static_for<1, L::HALF + 1>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    T e = 0, o = 0;
+    static_for<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if constexpr (Rows::template on<k>() && LBMX_BAS.M[k][d] != 0) {
+        if constexpr (LBMX_BAS.par[k] == 1 && ParMask != -1) e += T(LBMX_BAS.M[k][d]) * c[k];
+        if constexpr (LBMX_BAS.par[k] == -1 && ParMask != 1) o += T(LBMX_BAS.M[k][d]) * c[k];
+      }
+    });
+    if constexpr (ParMask != -1) x.s[d] -= T(2) * e;
+    if constexpr (ParMask != 1) x.a[d] -= T(2) * o;
+  });
+}
+
+template <class L, int DES> struct ActiveRows {
+  template <int k> static constexpr bool on() { return row_active<L, DES, k>(); }
+};
+template <class L> struct RelaxedRows {
+  template <int k> static constexpr bool on() { return LBMX_BAS.rate[k] != 0; }
+};
+
+// m_eq of row k (src/dynamics/incompressibleMRT2d.hh:62-67, src/dynamics/MRT3d.hh:92-109)
+template <class L, int k, class T>
+__device__ __forceinline__ T meq_row(T rho, const T (&v)[3], T inv_rho_s) {
+  T m = 0;
+  if constexpr (LBMX_BAS.a[k] != 0) m += T(LBMX_BAS.a[k]) * rho;
+  static_for<0, 3>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    if constexpr (LBMX_BAS.b[k][a] != 0) m += T(LBMX_BAS.b[k][a]) * v[a];
+  });
+  T quad = 0;
+  static_for<0, 3>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    static_for<0, 3>([&](auto bc) {
+      constexpr int b = decltype(bc)::value;
+      if constexpr (LBMX_BAS.C[k][a][b] != 0) quad += T(LBMX_BAS.C[k][a][b]) * v[a] * v[b];
+    });
+  });
+  return m + quad * inv_rho_s;
+}
+
+// P_k(v) = 3 v.G1_k + 9 v^T G2_k v - 3 |v|^2 G0_k ; source moment = -alpha P_k
+template <class L, int k, class T>
+__device__ __forceinline__ T src_row(const T (&v)[3], T u2) {
+  T p = 0;
+  if constexpr (LBMX_BAS.G0[k] != 0) p -= T(3.0 * LBMX_BAS.G0[k]) * u2;
+  static_for<0, 3>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    if constexpr (LBMX_BAS.G1[k][a] != 0) p += T(3.0 * LBMX_BAS.G1[k][a]) * v[a];
+    static_for<0, 3>([&](auto bc) {
+      constexpr int b = decltype(bc)::value;
+      if constexpr (LBMX_BAS.G2[k][a][b] != 0) p += T(9.0 * LBMX_BAS.G2[k][a][b]) * v[a] * v[b];
+    });
+  });
+  return p;
+}
+
+// ---- forward (pair space) ---------------------------------------------------------------
+template <class L, int COLL, int DES, class T>
+__device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3], T omega, T alpha) {
+  constexpr bool inc = CollTraits<COLL>::inc;
+  const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  if constexpr (!CollTraits<COLL>::mrt) {
+    // BGK: f' = f (1 - omega) + omega f_eq  (src/dynamics/incompressibleBGK2d.hh:38-47);
+    // even/odd parts of f_eq: 2 w (rho + 4.5 cu^2 - 1.5 u^2), 2 w 3 cu (incompressible)
+    const T one_m = T(1) - omega;
+    const T kap_a = (T(1) - T(0.5) * omega) * alpha;
+    const T rs = inc ? T(1) : rho;  // standard: f_eq = w rho (1 + ...)
+    {
+      T feq0 = inc ? T(L::w[0]) * (rho - T(1.5) * u2) : T(L::w[0]) * rho * (T(1) - T(1.5) * u2);
+      T out = p.f0 * one_m + omega * feq0;
+      if constexpr (DES == D_BRINK) out += kap_a * T(L::w[0]) * T(3) * u2;
+      p.f0 = out;
+    }
+    static_for<1, L::HALF + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      const T cu = cdot<L, d>(v);
+      const T w2 = T(2.0 * L::w[d]);
+      T ev = inc ? w2 * (rho + T(4.5) * cu * cu - T(1.5) * u2) : w2 * rho * (T(1) + T(4.5) * cu * cu - T(1.5) * u2);
+      T od = w2 * rs * T(3) * cu;
+      T s = p.s[d] * one_m + omega * ev;
+      T a = p.a[d] * one_m + omega * od;
+      if constexpr (DES == D_BRINK) {
+        s -= kap_a * w2 * (T(9) * cu * cu - T(3) * u2);
+        a -= kap_a * w2 * T(3) * cu;
+      }
+      p.s[d] = s;
+      p.a[d] = a;
+    });
+  } else {
+    // MRT: f' = f - M^-1 [S (M f - m_eq) - (I - S/2) M src]
+    constexpr int Q = L::QF;
+    const T inv_rho_s = inc ? T(1) : recip<T>(rho);
+    T dm[Q];
+    static_for<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      dm[k] = 0;
+      if constexpr (LBMX_BAS.rate[k] != 0) {
+        T mk = pair_moment<L, k>(p);
+        T sk = LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]);
+        T val = sk * (mk - meq_row<L, k>(rho, v, inv_rho_s));
+        if constexpr (DES == D_BRINK && row_has_src<L, k>())
+          val += (T(1) - T(0.5) * sk) * alpha * src_row<L, k>(v, u2);
+        dm[k] = val * T(LBMX_BAS.invn2[k]);
+      } else if constexpr (DES == D_BRINK && row_has_src<L, k>()) {
+        dm[k] = alpha * src_row<L, k>(v, u2) * T(LBMX_BAS.invn2[k]);
+      }
+    });
+    pair_back<L, ActiveRows<L, DES>, 0>(p, dm);
+  }
+}
+
+// thermal part: ThermalObstacleCollision<ThermalMRT2d<D2Q5_TMRT>,Ramp>
+// (src/dynamics/thermalObstacleCollision.hh:47-53, src/dynamics/thermalMRT2d.hh:76-101)
+template <class T>
+__device__ __forceinline__ T thermal_omega(const ModelParams& mp, T gamma) {
+  T diff = T(mp.tdiff) * ramp_interp<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
+  return T(1) / (T(5) * diff + T(0.5));
+}
+
+template <class L, class T>
+__device__ __forceinline__ void thermal_relax(T (&f)[L::Q], const T (&j)[3], T gamma, const ModelParams& mp) {
+  constexpr int o = L::QF;
+  T temp = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
+  T omT = thermal_omega<T>(mp, gamma);
+  T dm[5];
+  static_for<1, 5>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    T mk = 0;
+    static_for<0, 5>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (kB5.M[k][q] != 0) mk += T(kB5.M[k][q]) * f[o + q];
+    });
+    T meq = temp * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
+    T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
+    dm[k] = sk * (mk - meq) * T(kB5.invn2[k]);
+  });
+  static_for<0, 5>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    T acc = f[o + q];
+    static_for<1, 5>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if constexpr (kB5.M[k][q] != 0) acc -= T(kB5.M[k][q]) * dm[k];
+    });
+    f[o + q] = acc;
+  });
+}
+
+template <class L, int COLL, int DES, class T>
+__device__ __forceinline__ void eq_velocity(T rho, const T (&j)[3], T alpha, T (&v)[3], T& beta) {
+  constexpr bool inc = CollTraits<COLL>::inc;
+  if constexpr (DES == D_BRINK) {
+    beta = inc ? recip<T>(T(1) + T(0.5) * alpha) : recip<T>(rho * (T(1) + T(0.5) * alpha));
+    v[0] = beta * j[0]; v[1] = beta * j[1]; v[2] = beta * j[2];
+  } else if constexpr (inc) {
+    beta = T(1);
+    v[0] = j[0]; v[1] = j[1]; v[2] = j[2];
+  } else {
+    beta = recip<T>(rho);
+    v[0] = j[0] * beta; v[1] = j[1] * beta; v[2] = j[2] * beta;
+  }
+}
+
+// Full node collision in place; rho, j are the pre-collision macros.
+template <class L, int COLL, int DES, class T>
+__device__ __forceinline__ void collide(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
+  Pairs<L, T> p;
+  to_pairs<L>(f, p);
+  pair_macros<L>(p, rho, j);
+  const T omega = T(mp.omega);
+  T alpha = 0, beta, v[3];
+  if constexpr (DES == D_BRINK) alpha = T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+  eq_velocity<L, COLL, DES>(rho, j, alpha, v, beta);
+  base_relax<L, COLL, DES>(p, rho, v, omega, alpha);
+  if constexpr (DES == D_PBB) {
+    // PartialBouncebackCollision (src/dynamics/partialBouncebackCollision.hh:53-65): the
+    // pair blend keeps f_d + f_d' and scales f_d - f_d' by 1 - 2 imult
+    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T sc = T(1) - T(2) * imult;
+    static_for<1, L::HALF + 1>([&](auto dc) { p.a[decltype(dc)::value] *= sc; });
+  }
+  from_pairs<L>(p, f);
+  if constexpr (L::QT > 0) thermal_relax<L>(f, j, gamma, mp);
+}
+
+// ---- reverse (pair space) ---------------------------------------------------------------
+// lam <- (dC/df)^T lam + (A + c.B) [+ AT on the thermal populations],
+// dgam += (dC/dgamma)^T lam. (A, B, AT) carry a functional source that is affine in c, so it
+// is added with the macro-space adjoints at no extra cost. The forward quantities needed are
+// recomputed from f, as the reference's reverse operators do
+// (src/adjointDynamics/reversePartialBounceback.hh:18-41, reverseIncompressibleBGK2d.hh:64-121,
+// reverseIncompressibleMRT2d.hh:29-83, reverseMRT3d.hh:29-365).
+template <class L, int COLL, int DES, class T>
+__device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, const ModelParams& mp, T (&lam)[L::Q],
+                                                T& dgam, T srcA, const T (&srcB)[3], T srcAT) {
+  static_assert(CollTraits<COLL>::inc, "reverse operators exist for the incompressible operators");
+  constexpr int Q = L::QF, H = L::HALF;
+  Pairs<L, T> pf;
+  to_pairs<L>(f, pf);
+  T rho, j[3];
+  pair_macros<L>(pf, rho, j);
+  const T omega = T(mp.omega);
+  T alpha = 0, beta, v[3];
+  if constexpr (DES == D_BRINK) alpha = T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+  eq_velocity<L, COLL, DES>(rho, j, alpha, v, beta);
+  const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  T gbar = 0;
+  T lamT[L::QT > 0 ? 5 : 1];
+  if constexpr (L::QT > 0) {
+    static_for<0, 5>([&](auto qc) { lamT[decltype(qc)::value] = lam[Q + decltype(qc)::value]; });
+  }
+  Pairs<L, T> pl;
+  to_pairs<L>(lam, pl);
+
+  if constexpr (DES == D_PBB) {
+    // odd part g_d - g_d' of the base collision output
+    T gd[H + 1];
+    if constexpr (!CollTraits<COLL>::mrt) {
+      static_for<1, H + 1>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        gd[d] = pf.a[d] * (T(1) - omega) + omega * T(6.0 * L::w[d]) * cdot<L, d>(v);
+      });
+    } else {
+      T dm[Q];
+      static_for<0, Q>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        dm[k] = 0;
+        if constexpr (LBMX_BAS.rate[k] != 0 && LBMX_BAS.par[k] == -1) {
+          T sk = LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]);
+          dm[k] = sk * (pair_moment<L, k>(pf) - meq_row<L, k>(rho, v, T(1))) * T(LBMX_BAS.invn2[k]);
+        }
+      });
+      Pairs<L, T> g = pf;
+      pair_back<L, RelaxedRows<L>, -1>(g, dm);
+      static_for<1, H + 1>([&](auto dc) { gd[decltype(dc)::value] = g.a[decltype(dc)::value]; });
+    }
+    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T sc = T(1) - T(2) * imult;
+    T mbar = 0;
+    static_for<1, H + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      mbar -= pl.a[d] * gd[d];
+      pl.a[d] *= sc;
+    });
+    gbar += mbar * T(0.5) * T(mp.alpha_max) * bp_der<T>(T(mp.interp_q), gamma);
+  }
+
+  T rho_bar = 0, vb[3] = {0, 0, 0}, alpha_bar = 0;
+  if constexpr (!CollTraits<COLL>::mrt) {
+    const T kap = T(1) - T(0.5) * omega;
+    T S0 = T(L::w[0]) * pl.f0;
+    T ce[3] = {0, 0, 0};  // sum_d w_d (3 c_d la_d + 9 cu_d c_d ls_d)
+    T cs[3] = {0, 0, 0};  // sum_d w_d (3 c_d la_d + 18 cu_d c_d ls_d) (source)
+    T asum = 0;           // sum_d w_d (3 cu_d la_d + (9 cu_d^2 - 3 u2) ls_d) + w0 l0 (-3 u2)
+    if constexpr (DES == D_BRINK) asum = T(L::w[0]) * pl.f0 * (-T(3) * u2);
+    static_for<1, H + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      const T cu = cdot<L, d>(v);
+      const T wd = T(L::w[d]);
+      S0 += wd * pl.s[d];
+      const T t1 = wd * (T(3) * pl.a[d] + T(9) * cu * pl.s[d]);
+      if constexpr (L::ex[d] != 0) ce[0] += T(L::ex[d]) * t1;
+      if constexpr (L::ey[d] != 0) ce[1] += T(L::ey[d]) * t1;
+      if constexpr (L::ez[d] != 0) ce[2] += T(L::ez[d]) * t1;
+      if constexpr (DES == D_BRINK) {
+        const T t2 = wd * (T(3) * pl.a[d] + T(18) * cu * pl.s[d]);
+        if constexpr (L::ex[d] != 0) cs[0] += T(L::ex[d]) * t2;
+        if constexpr (L::ey[d] != 0) cs[1] += T(L::ey[d]) * t2;
+        if constexpr (L::ez[d] != 0) cs[2] += T(L::ez[d]) * t2;
+        asum += wd * (T(3) * cu * pl.a[d] + (T(9) * cu * cu - T(3) * u2) * pl.s[d]);
+      }
+    });
+    rho_bar = omega * S0;
+    static_for<0, 3>([&](auto ac) {
+      constexpr int a = decltype(ac)::value;
+      vb[a] = omega * (ce[a] - T(3) * v[a] * S0);
+      if constexpr (DES == D_BRINK) vb[a] -= alpha * kap * (cs[a] - T(6) * v[a] * S0);
+    });
+    if constexpr (DES == D_BRINK) alpha_bar = -kap * asum;
+    const T one_m = T(1) - omega;
+    pl.f0 *= one_m;
+    static_for<1, H + 1>([&](auto dc) {
+      pl.s[decltype(dc)::value] *= one_m;
+      pl.a[decltype(dc)::value] *= one_m;
+    });
+  } else {
+    T bk[Q];
+    static_for<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      bk[k] = 0;
+      if constexpr (row_active<L, DES, k>()) {
+        const T ak = pair_moment<L, k>(pl) * T(LBMX_BAS.invn2[k]);
+        const T sk = LBMX_BAS.rate[k] == 0 ? T(0) : (LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]));
+        if constexpr (LBMX_BAS.rate[k] != 0) {
+          const T b = sk * ak;
+          bk[k] = b;
+          if constexpr (LBMX_BAS.a[k] != 0) rho_bar += b * T(LBMX_BAS.a[k]);
+          static_for<0, 3>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            if constexpr (LBMX_BAS.b[k][a] != 0) vb[a] += b * T(LBMX_BAS.b[k][a]);
+            static_for<0, 3>([&](auto cc) {
+              constexpr int c = decltype(cc)::value;
+              constexpr double csum = LBMX_BAS.C[k][a][c] + LBMX_BAS.C[k][c][a];
+              if constexpr (csum != 0) vb[a] += b * T(csum) * v[c];
+            });
+          });
+        }
+        if constexpr (DES == D_BRINK && row_has_src<L, k>()) {
+          // the source moment alpha P_k enters dm_k with (1 - s_k/2); out -= M^T N dm
+          const T pbar = -(T(1) - T(0.5) * sk) * ak;  // adjoint of alpha P_k
+          alpha_bar += pbar * src_row<L, k>(v, u2);
+          const T pa = pbar * alpha;
+          if constexpr (LBMX_BAS.G0[k] != 0) {
+            vb[0] -= pa * T(6.0 * LBMX_BAS.G0[k]) * v[0];
+            vb[1] -= pa * T(6.0 * LBMX_BAS.G0[k]) * v[1];
+            vb[2] -= pa * T(6.0 * LBMX_BAS.G0[k]) * v[2];
+          }
+          static_for<0, 3>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            if constexpr (LBMX_BAS.G1[k][a] != 0) vb[a] += pa * T(3.0 * LBMX_BAS.G1[k][a]);
+            static_for<0, 3>([&](auto cc) {
+              constexpr int c = decltype(cc)::value;
+              constexpr double gs = LBMX_BAS.G2[k][a][c] + LBMX_BAS.G2[k][c][a];
+              if constexpr (gs != 0) vb[a] += pa * T(9.0 * gs) * v[c];
+            });
+          });
+        }
+      }
+    });
+    pair_back<L, RelaxedRows<L>, 0>(pl, bk);
+  }
+  T jb[3] = {beta * vb[0] + srcB[0], beta * vb[1] + srcB[1], beta * vb[2] + srcB[2]};
+  rho_bar += srcA;
+  if constexpr (DES == D_BRINK) {
+    alpha_bar -= T(0.5) * beta * beta * (vb[0] * j[0] + vb[1] * j[1] + vb[2] * j[2]);
+    gbar += alpha_bar * T(mp.alpha_max) * bp_der<T>(T(mp.interp_q), gamma);
+  }
+
+  if constexpr (L::QT > 0) {
+    constexpr int o = Q;
+    T temp = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
+    T omT = thermal_omega<T>(mp, gamma);
+    T ak[5];
+    static_for<1, 5>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      T s = 0;
+      static_for<0, 5>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (kB5.M[k][q] != 0) s += T(kB5.M[k][q]) * lamT[q];
+      });
+      ak[k] = s * T(kB5.invn2[k]);
+    });
+    T Tbar = srcAT, ombar = 0;
+    static_for<1, 5>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
+      T b = sk * ak[k];
+      Tbar += b * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
+      if constexpr (kB5.b[k][0] != 0) jb[0] += b * temp * T(kB5.b[k][0]);
+      if constexpr (kB5.b[k][1] != 0) jb[1] += b * temp * T(kB5.b[k][1]);
+      if constexpr (kB5.rate[k] == 1) {
+        T mk = 0;
+        static_for<0, 5>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if constexpr (kB5.M[k][q] != 0) mk += T(kB5.M[k][q]) * f[o + q];
+        });
+        T meq = temp * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
+        ombar -= ak[k] * (mk - meq);
+      }
+    });
+    static_for<0, 5>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T acc = lamT[q];
+      static_for<1, 5>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (kB5.M[k][q] != 0) {
+          T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
+          acc -= T(kB5.M[k][q]) * sk * ak[k];
+        }
+      });
+      lam[o + q] = acc + Tbar;
+    });
+    T dR = ramp_diff<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
+    gbar += ombar * (-omT * omT * T(5) * T(mp.tdiff) * dR);
+  }
+
+  // lam_pre = lam_f + rho_bar + c.j_bar (pair space: f0 += rb, s += 2 rb, a += 2 c.jb)
+  pl.f0 += rho_bar;
+  static_for<1, H + 1>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    pl.s[d] += T(2) * rho_bar;
+    pl.a[d] += T(2) * cdot<L, d>(jb);
+  });
+  from_pairs<L>(pl, lam);
+  dgam += gbar;
+}
+
+#undef LBMX_BAS
+
+}  // namespace lbmx

@@ -6,6 +6,7 @@
 namespace lbmx {
 
 constexpr int kMaxTerms = 4;
+constexpr int kMaxBC = 64;  // boundary pieces per lattice
 constexpr int kTileX = 64;   // threads along x per workgroup (one wave)
 constexpr int kTileY = 4;    // waves per workgroup, stacked along y
 
@@ -46,10 +47,21 @@ struct ModelParams {
   int pad;
 };
 
+// Padded structure-of-arrays layout of one population field in HBM. Interior node (x,y,z)
+// lives at element  q*plane + ((z+oz)*py + y+1)*px + ax + x ; rows are 128-byte aligned
+// (ax = 128 bytes / element size) and a one-cell halo surrounds the domain. The halo holds
+// the mid-way bounce-back reflections written by the sweep that produced the field.
+struct Layout {
+  int px, py;        // padded row length (elements), padded rows per slice
+  int ax, oz;        // x offset of interior x = 0; z offset (1 in 3D, 0 in 2D)
+  long long plane;   // elements per population plane
+};
+
 struct Geometry {
   int nx, ny, nz;
-  int pad;
-  long long N;  // nodes
+  int nblk_x, nblk_y;  // workgroup tiles along x, y
+  int nblocks;         // total workgroups
+  long long N;         // interior nodes
 };
 
 struct ObjArgs {
@@ -57,8 +69,8 @@ struct ObjArgs {
   int bbox[6];        // union of term boxes (skip test)
   int pad;
   double time_const;
-  DevTerm terms[kMaxTerms];
-  double* acc;        // per-workgroup running sums (FP64), null = off
+  const DevTerm* terms;  // device array of nterm terms
+  double* acc;           // per-workgroup running sums (FP64), null = off
 };
 
 // Forward sweep: dst = C(B_t(S(src))) ; objective of B_t(S(src)) accumulated.
@@ -69,10 +81,12 @@ struct FwdArgs {
   const DevBC* bcs;
   int nbc;
   int t;              // time of the new state (boundary values)
-  int mode;           // 0 full sweep, 1 stream+BC only (no collision), 2 collide only (src is AoS FP64 state)
+  double tfac[kMaxBC];  // time factor of each boundary piece at t (computed on the host)
+  int mode;           // 0 full sweep, 1 stream+BC only (AoS FP64 out), 2 collide AoS FP64 in
   int pad;
   void* aos_out;      // mode 1: pre-collision state written AoS FP64
   Geometry g;
+  Layout ls;          // layout of src/dst
   ModelParams m;
   ObjArgs obj;
 };
@@ -87,9 +101,24 @@ struct AdjArgs {
   const DevBC* bcs;
   int nbc;
   int t;
+  double tfac[kMaxBC];
   Geometry g;
+  Layout ls;          // layout of the forward state
+  Layout ll;          // layout of the adjoint fields
   ModelParams m;
   ObjArgs obj;
 };
+
+inline Layout make_layout(int nx, int ny, int nz, int dim, int elem_bytes) {
+  Layout l{};
+  l.ax = 128 / elem_bytes;
+  int rowlen = ((nx + 1 + l.ax - 1) / l.ax) * l.ax;
+  l.px = l.ax + rowlen;
+  l.py = ny + 2;
+  l.oz = dim == 3 ? 1 : 0;
+  long long pz = dim == 3 ? nz + 2 : 1;
+  l.plane = (long long)l.px * l.py * pz;
+  return l;
+}
 
 }  // namespace lbmx

@@ -64,12 +64,30 @@ struct Basis {
   double a[Q];
   double b[Q][3];
   double C[Q][3][3];
-  constexpr Basis() : M{}, invn2{}, rate{}, s{}, a{}, b{}, C{} {}
-  constexpr void finish() {
+  int par[Q];         // +1: row even in c (depends on f_d + f_opp), -1: odd
+  double G0[Q];       // sum_q M_kq w_q
+  double G1[Q][3];    // sum_q M_kq w_q c_qa
+  double G2[Q][3][3]; // sum_q M_kq w_q c_qa c_qb
+  constexpr Basis() : M{}, invn2{}, rate{}, s{}, a{}, b{}, C{}, par{}, G0{}, G1{}, G2{} {}
+  constexpr void finish(const int* ex, const int* ey, const int* ez, const double* w, const int* opp) {
     for (int k = 0; k < Q; ++k) {
       double n = 0;
       for (int q = 0; q < Q; ++q) n += M[k][q] * M[k][q];
       invn2[k] = 1.0 / n;
+      bool even = true, odd = true;
+      for (int q = 0; q < Q; ++q) {
+        if (M[k][q] != M[k][opp[q]]) even = false;
+        if (M[k][q] != -M[k][opp[q]]) odd = false;
+      }
+      par[k] = even ? 1 : (odd ? -1 : 0);
+      for (int q = 0; q < Q; ++q) {
+        const double c[3] = {double(ex[q]), double(ey[q]), double(ez[q])};
+        G0[k] += M[k][q] * w[q];
+        for (int x = 0; x < 3; ++x) {
+          G1[k][x] += M[k][q] * w[q] * c[x];
+          for (int y = 0; y < 3; ++y) G2[k][x][y] += M[k][q] * w[q] * c[x] * c[y];
+        }
+      }
     }
   }
 };
@@ -97,7 +115,7 @@ constexpr Basis<9> make_basis_d2q9() {
   B.b[6][1] = -1; B.rate[6] = 2; B.s[6] = 1.2;
   B.C[7][0][0] = 1; B.C[7][1][1] = -1; B.rate[7] = 1;
   B.C[8][0][1] = 0.5; B.C[8][1][0] = 0.5; B.rate[8] = 1;
-  B.finish();
+  B.finish(LatD2Q9::ex, LatD2Q9::ey, LatD2Q9::ez, LatD2Q9::w, LatD2Q9::opp);
   return B;
 }
 
@@ -140,7 +158,7 @@ constexpr Basis<19> make_basis_d3q19() {
   B.C[14][1][2] = 0.5; B.C[14][2][1] = 0.5; B.rate[14] = 1;
   B.C[15][0][2] = 0.5; B.C[15][2][0] = 0.5; B.rate[15] = 1;
   for (int k = 16; k < 19; ++k) { B.rate[k] = 2; B.s[k] = 1.4; }
-  B.finish();
+  B.finish(LatD3Q19::ex, LatD3Q19::ey, LatD3Q19::ez, LatD3Q19::w, LatD3Q19::opp);
   return B;
 }
 
@@ -162,7 +180,10 @@ constexpr Basis<5> make_basis_d2q5() {
   B.b[2][1] = 1; B.rate[2] = 1;
   B.a[3] = -2; B.rate[3] = 2; B.s[3] = 1.5;
   B.rate[4] = 2; B.s[4] = 1.5;
-  B.finish();
+  constexpr int ez[5] = {0, 0, 0, 0, 0};
+  constexpr double w[5] = {1. / 3., 1. / 6., 1. / 6., 1. / 6., 1. / 6.};
+  constexpr int opp[5] = {0, 3, 4, 1, 2};
+  B.finish(ex, ey, ez, w, opp);
   return B;
 }
 
@@ -256,461 +277,31 @@ template <class T> __device__ __forceinline__ T ramp_diff(T p, T ck, T s) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Flow collision (incompressible and standard BGK / MRT) with PBB or Brinkman coupling.
-// ---------------------------------------------------------------------------------------
-enum : int { C_BGK = LBMX_BGK, C_MRT = LBMX_MRT, C_CASC = LBMX_CASCADED,
-             C_MRT_STD = LBMX_MRT_STANDARD, C_BGK_STD = LBMX_BGK_STANDARD };
-
-template <int COLL> struct CollTraits {
-  static constexpr bool inc = !(COLL == C_MRT_STD || COLL == C_BGK_STD);
-  static constexpr bool mrt = (COLL == C_MRT || COLL == C_MRT_STD);
-};
-
-template <class L, class T>
-__device__ __forceinline__ void flow_macros(const T (&f)[L::Q], T& rho, T (&j)[3]) {
-  T r = 0, jx = 0, jy = 0, jz = 0;
-  static_for<0, L::QF>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    r += f[q];
-    if constexpr (L::ex[q] == 1) jx += f[q];
-    if constexpr (L::ex[q] == -1) jx -= f[q];
-    if constexpr (L::ey[q] == 1) jy += f[q];
-    if constexpr (L::ey[q] == -1) jy -= f[q];
-    if constexpr (L::ez[q] == 1) jz += f[q];
-    if constexpr (L::ez[q] == -1) jz -= f[q];
-  });
-  rho = r; j[0] = jx; j[1] = jy; j[2] = jz;
-}
-
-// c_q . v with compile-time velocity
-template <class L, int q, class T>
-__device__ __forceinline__ T cdot(const T (&v)[3]) {
-  T s = 0;
-  if constexpr (L::ex[q] != 0) s += T(L::ex[q]) * v[0];
-  if constexpr (L::ey[q] != 0) s += T(L::ey[q]) * v[1];
-  if constexpr (L::ez[q] != 0) s += T(L::ez[q]) * v[2];
-  return s;
-}
-
-// equilibrium moment of row k and its derivatives
-template <int Q, int k, class T>
-__device__ __forceinline__ T row_meq(T rho, const T (&v)[3], T inv_rho_s) {
-  T m = 0;
-  if constexpr (BasisOf<Q>::B.a[k] != 0) m += T(BasisOf<Q>::B.a[k]) * rho;
-  static_for<0, 3>([&](auto ac) {
-    constexpr int a = decltype(ac)::value;
-    if constexpr (BasisOf<Q>::B.b[k][a] != 0) m += T(BasisOf<Q>::B.b[k][a]) * v[a];
-  });
-  T quad = 0;
-  bool any = false;
-  static_for<0, 3>([&](auto ac) {
-    constexpr int a = decltype(ac)::value;
-    static_for<0, 3>([&](auto bc) {
-      constexpr int b = decltype(bc)::value;
-      if constexpr (BasisOf<Q>::B.C[k][a][b] != 0) { quad += T(BasisOf<Q>::B.C[k][a][b]) * v[a] * v[b]; any = true; }
-    });
-  });
-  if (any) m += quad * inv_rho_s;
-  return m;
-}
-
-template <int Q, int k>
-constexpr bool row_has_quad() {
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      if (BasisOf<Q>::B.C[k][a][b] != 0) return true;
-  return false;
-}
-
-template <class T> __device__ __forceinline__ T rate_of(int kind, double s, T omega) {
-  return kind == 0 ? T(0) : (kind == 1 ? omega : T(s));
-}
-
-// Guo source with F = -alpha v:  w_q [3 (c-v).F + 9 (c.v)(c.F)]
-template <class L, int q, class T>
-__device__ __forceinline__ T guo_src(const T (&v)[3], T u2, T alpha) {
-  T cu = cdot<L, q>(v);
-  return -alpha * T(L::w[q]) * (T(3) * cu + T(9) * cu * cu - T(3) * u2);
-}
-
-// Base relaxation: f <- f - relax(f) [+ (I - S/2) source]; v is the equilibrium velocity.
-template <class L, int COLL, class T>
-__device__ __forceinline__ void base_relax(T (&f)[L::Q], T rho, const T (&v)[3], T omega, T alpha,
-                                           bool forced) {
-  constexpr bool inc = CollTraits<COLL>::inc;
-  const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-  if constexpr (!CollTraits<COLL>::mrt) {
-    // BGK: f' = f (1 - omega) + omega f_eq  (src/dynamics/incompressibleBGK2d.hh:38-47)
-    const T kappa = T(1) - T(0.5) * omega;
-    static_for<0, L::QF>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      T cu = cdot<L, q>(v);
-      T poly = T(3) * cu + T(4.5) * cu * cu - T(1.5) * u2;
-      T feq = inc ? T(L::w[q]) * (rho + poly) : T(L::w[q]) * rho * (T(1) + poly);
-      T out = f[q] * (T(1) - omega) + omega * feq;
-      if (forced) out += kappa * guo_src<L, q>(v, u2, alpha);
-      f[q] = out;
-    });
-  } else {
-    constexpr int Q = L::QF;
-    const T inv_rho_s = inc ? T(1) : T(1) / rho;
-    T dm[Q];
-    static_for<0, Q>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      dm[k] = 0;
-      if constexpr (BasisOf<Q>::B.rate[k] != 0) {
-        T mk = 0;
-        static_for<0, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          if constexpr (BasisOf<Q>::B.M[k][q] != 0) mk += T(BasisOf<Q>::B.M[k][q]) * f[q];
-        });
-        T sk = BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]);
-        dm[k] = sk * (mk - row_meq<Q, k>(rho, v, inv_rho_s)) * T(BasisOf<Q>::B.invn2[k]);
-      }
-    });
-    if (forced) {
-      T src[Q];
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        src[q] = guo_src<L, q>(v, u2, alpha);
-      });
-      static_for<0, Q>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        T sk = 0;
-        static_for<0, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          if constexpr (BasisOf<Q>::B.M[k][q] != 0) sk += T(BasisOf<Q>::B.M[k][q]) * src[q];
-        });
-        T rk = BasisOf<Q>::B.rate[k] == 0 ? T(0) : (BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]));
-        dm[k] -= (T(1) - T(0.5) * rk) * sk * T(BasisOf<Q>::B.invn2[k]);
-      });
-    }
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      T acc = f[q];
-      static_for<0, Q>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        if constexpr (BasisOf<Q>::B.M[k][q] != 0) {
-          if (forced || BasisOf<Q>::B.rate[k] != 0) acc -= T(BasisOf<Q>::B.M[k][q]) * dm[k];
-        }
-      });
-      f[q] = acc;
-    });
-  }
-}
-
-// thermal: ThermalObstacleCollision<ThermalMRT2d<D2Q5_TMRT>,Ramp> on populations QF..QF+4
-template <class T>
-__device__ __forceinline__ T thermal_omega(const ModelParams& mp, T gamma) {
-  T diff = T(mp.tdiff) * ramp_interp<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
-  return T(1) / (T(5) * diff + T(0.5));
-}
-
-template <class L, class T>
-__device__ __forceinline__ void thermal_relax(T (&f)[L::Q], const T (&j)[3], T gamma,
-                                              const ModelParams& mp, T& Tout) {
-  constexpr int o = L::QF;
-  T temp = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
-  T omT = thermal_omega<T>(mp, gamma);
-  T dm[5];
-  static_for<1, 5>([&](auto kc) {
-    constexpr int k = decltype(kc)::value;
-    T mk = 0;
-    static_for<0, 5>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      if constexpr (kB5.M[k][q] != 0) mk += T(kB5.M[k][q]) * f[o + q];
-    });
-    T meq = temp * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
-    T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
-    dm[k] = sk * (mk - meq) * T(kB5.invn2[k]);
-  });
-  static_for<0, 5>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    T acc = f[o + q];
-    static_for<1, 5>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      if constexpr (kB5.M[k][q] != 0) acc -= T(kB5.M[k][q]) * dm[k];
-    });
-    f[o + q] = acc;
-  });
-  Tout = temp;
-}
-
-// Full node collision in place. rho, j (pre-collision macros) are returned.
-template <class L, int COLL, class T>
-__device__ __forceinline__ void collide(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho,
-                                        T (&j)[3]) {
-  constexpr bool inc = CollTraits<COLL>::inc;
-  flow_macros<L>(f, rho, j);
-  const T omega = T(mp.omega);
-  T v[3];
-  T alpha = 0;
-  const bool forced = mp.design == LBMX_DESIGN_BRINKMAN;
-  if (forced) {
-    alpha = T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
-    T beta = inc ? T(1) / (T(1) + T(0.5) * alpha) : T(1) / (rho * (T(1) + T(0.5) * alpha));
-    v[0] = beta * j[0]; v[1] = beta * j[1]; v[2] = beta * j[2];
-  } else if (inc) {
-    v[0] = j[0]; v[1] = j[1]; v[2] = j[2];
-  } else {
-    T ir = T(1) / rho;
-    v[0] = j[0] * ir; v[1] = j[1] * ir; v[2] = j[2] * ir;
-  }
-  T fin_t[L::QT > 0 ? 5 : 1];
-  if constexpr (L::QT > 0) {
-    static_for<0, 5>([&](auto qc) { fin_t[decltype(qc)::value] = f[L::QF + decltype(qc)::value]; });
-  }
-  base_relax<L, COLL>(f, rho, v, omega, alpha, forced);
-  if (mp.design == LBMX_DESIGN_PBB) {
-    // PartialBouncebackCollision (src/dynamics/partialBouncebackCollision.hh:53-65)
-    T imult = T(0.5) * T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
-    static_for<1, L::HALF + 1>([&](auto dc) {
-      constexpr int d = decltype(dc)::value;
-      T a = f[d], b = f[d + L::HALF];
-      f[d] = a + imult * (b - a);
-      f[d + L::HALF] = b + imult * (a - b);
-    });
-  }
-  if constexpr (L::QT > 0) {
-    T Tn;
-    (void)fin_t;
-    thermal_relax<L>(f, j, gamma, mp, Tn);
-  }
-}
-
-// ---------------------------------------------------------------------------------------
-// Reverse collision (incompressible operators): lam <- (dC/df)^T lam,  dgam += (dC/dgamma)^T lam.
-// The forward is recomputed from f, as the reference's reverse operators do
-// (src/adjointDynamics/reversePartialBounceback.hh:18-41).
-// ---------------------------------------------------------------------------------------
-template <class L, int COLL, class T>
-__device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, const ModelParams& mp,
-                                                T (&lam)[L::Q], T& dgam) {
-  constexpr int Q = L::QF;
-  T rho, j[3];
-  T ff[L::Q];
-  static_for<0, L::Q>([&](auto qc) { ff[decltype(qc)::value] = f[decltype(qc)::value]; });
-  flow_macros<L>(ff, rho, j);
-  const T omega = T(mp.omega);
-  const bool forced = mp.design == LBMX_DESIGN_BRINKMAN;
-  T alpha = 0, beta = 1;
-  if (forced) {
-    alpha = T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
-    beta = T(1) / (T(1) + T(0.5) * alpha);
-  }
-  T v[3] = {beta * j[0], beta * j[1], beta * j[2]};
-  const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-  T gbar = 0;
-  T lamT[L::QT > 0 ? 5 : 1];
-  if constexpr (L::QT > 0) {
-    static_for<0, 5>([&](auto qc) { lamT[decltype(qc)::value] = lam[Q + decltype(qc)::value]; });
-  }
-
-  if (mp.design == LBMX_DESIGN_PBB) {
-    T g[L::Q];
-    static_for<0, L::Q>([&](auto qc) { g[decltype(qc)::value] = f[decltype(qc)::value]; });
-    base_relax<L, COLL>(g, rho, v, omega, T(0), false);
-    T imult = T(0.5) * T(mp.alpha_max) * bp_interp<T>(T(mp.interp_q), gamma);
-    T mbar = 0;
-    static_for<1, L::HALF + 1>([&](auto dc) {
-      constexpr int d = decltype(dc)::value;
-      T a = lam[d], b = lam[d + L::HALF];
-      mbar += (a - b) * (g[d + L::HALF] - g[d]);
-      lam[d] = a + imult * (b - a);
-      lam[d + L::HALF] = b + imult * (a - b);
-    });
-    gbar += mbar * T(0.5) * T(mp.alpha_max) * bp_diff<T>(T(mp.interp_q), gamma);
-  }
-
-  T rho_bar = 0, vb[3] = {0, 0, 0}, alpha_bar = 0;
-  if constexpr (!CollTraits<COLL>::mrt) {
-    const T kappa = T(1) - T(0.5) * omega;
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      T cu = cdot<L, q>(v);
-      T lq = lam[q];
-      T t = omega * T(L::w[q]) * lq;
-      rho_bar += t;
-      T common = T(9) * cu;
-      if (forced) {
-        T ks = kappa * lq * T(L::w[q]);
-        alpha_bar -= ks * (T(3) * cu + T(9) * cu * cu - T(3) * u2);
-        // d/dv of omega feq and of kappa * src
-        T c1 = t * (T(3) + common) - alpha * ks * (T(3) + T(18) * cu);
-        T c2 = -t * T(3) + alpha * ks * T(6);
-        if constexpr (L::ex[q] != 0) vb[0] += T(L::ex[q]) * c1;
-        if constexpr (L::ey[q] != 0) vb[1] += T(L::ey[q]) * c1;
-        if constexpr (L::ez[q] != 0) vb[2] += T(L::ez[q]) * c1;
-        vb[0] += c2 * v[0]; vb[1] += c2 * v[1]; vb[2] += c2 * v[2];
-      } else {
-        T c1 = t * (T(3) + common);
-        if constexpr (L::ex[q] != 0) vb[0] += T(L::ex[q]) * c1;
-        if constexpr (L::ey[q] != 0) vb[1] += T(L::ey[q]) * c1;
-        if constexpr (L::ez[q] != 0) vb[2] += T(L::ez[q]) * c1;
-        vb[0] -= T(3) * t * v[0]; vb[1] -= T(3) * t * v[1]; vb[2] -= T(3) * t * v[2];
-      }
-      lam[q] = (T(1) - omega) * lq;
-    });
-  } else {
-    T ak[Q];
-    static_for<0, Q>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      ak[k] = 0;
-      if (BasisOf<Q>::B.rate[k] != 0 || forced) {
-        T s = 0;
-        static_for<0, Q>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          if constexpr (BasisOf<Q>::B.M[k][q] != 0) s += T(BasisOf<Q>::B.M[k][q]) * lam[q];
-        });
-        ak[k] = s * T(BasisOf<Q>::B.invn2[k]);
-      }
-    });
-    // meq path: rho_bar, v_bar from b_k = s_k a_k
-    static_for<0, Q>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      if constexpr (BasisOf<Q>::B.rate[k] != 0) {
-        T bk = (BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k])) * ak[k];
-        if constexpr (BasisOf<Q>::B.a[k] != 0) rho_bar += bk * T(BasisOf<Q>::B.a[k]);
-        static_for<0, 3>([&](auto ac) {
-          constexpr int a = decltype(ac)::value;
-          T d = 0;
-          if constexpr (BasisOf<Q>::B.b[k][a] != 0) d += T(BasisOf<Q>::B.b[k][a]);
-          static_for<0, 3>([&](auto bc) {
-            constexpr int b = decltype(bc)::value;
-            if constexpr (BasisOf<Q>::B.C[k][a][b] + BasisOf<Q>::B.C[k][b][a] != 0) d += T(BasisOf<Q>::B.C[k][a][b] + BasisOf<Q>::B.C[k][b][a]) * v[b];
-          });
-          vb[a] += bk * d;
-        });
-      }
-    });
-    if (forced) {
-      // sigma_q = sum_k M_kq (1 - s_k/2) a_k ; source derivative
-      static_for<0, Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        T sg = 0;
-        static_for<0, Q>([&](auto kc) {
-          constexpr int k = decltype(kc)::value;
-          if constexpr (BasisOf<Q>::B.M[k][q] != 0) {
-            T rk = BasisOf<Q>::B.rate[k] == 0 ? T(0) : (BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]));
-            sg += T(BasisOf<Q>::B.M[k][q]) * (T(1) - T(0.5) * rk) * ak[k];
-          }
-        });
-        T cu = cdot<L, q>(v);
-        T ks = sg * T(L::w[q]);
-        alpha_bar -= ks * (T(3) * cu + T(9) * cu * cu - T(3) * u2);
-        T c1 = -alpha * ks * (T(3) + T(18) * cu);
-        if constexpr (L::ex[q] != 0) vb[0] += T(L::ex[q]) * c1;
-        if constexpr (L::ey[q] != 0) vb[1] += T(L::ey[q]) * c1;
-        if constexpr (L::ez[q] != 0) vb[2] += T(L::ez[q]) * c1;
-        T c2 = alpha * ks * T(6);
-        vb[0] += c2 * v[0]; vb[1] += c2 * v[1]; vb[2] += c2 * v[2];
-      });
-    }
-    // direct path: lam_q - sum_k M_kq s_k a_k
-    static_for<0, Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      T acc = lam[q];
-      static_for<0, Q>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        if constexpr (BasisOf<Q>::B.M[k][q] != 0 && BasisOf<Q>::B.rate[k] != 0) {
-          T sk = BasisOf<Q>::B.rate[k] == 1 ? omega : T(BasisOf<Q>::B.s[k]);
-          acc -= T(BasisOf<Q>::B.M[k][q]) * sk * ak[k];
-        }
-      });
-      lam[q] = acc;
-    });
-  }
-  T jb[3] = {beta * vb[0], beta * vb[1], beta * vb[2]};
-  if (forced) {
-    alpha_bar -= T(0.5) * beta * beta * (vb[0] * j[0] + vb[1] * j[1] + vb[2] * j[2]);
-    gbar += alpha_bar * T(mp.alpha_max) * bp_diff<T>(T(mp.interp_q), gamma);
-  }
-
-  if constexpr (L::QT > 0) {
-    constexpr int o = Q;
-    T temp = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
-    T omT = thermal_omega<T>(mp, gamma);
-    T ak[5];
-    static_for<1, 5>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      T s = 0;
-      static_for<0, 5>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if constexpr (kB5.M[k][q] != 0) s += T(kB5.M[k][q]) * lamT[q];
-      });
-      ak[k] = s * T(kB5.invn2[k]);
-    });
-    T Tbar = 0, ombar = 0;
-    static_for<1, 5>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
-      T bk = sk * ak[k];
-      Tbar += bk * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
-      if constexpr (kB5.b[k][0] != 0) jb[0] += bk * temp * T(kB5.b[k][0]);
-      if constexpr (kB5.b[k][1] != 0) jb[1] += bk * temp * T(kB5.b[k][1]);
-      if constexpr (kB5.rate[k] == 1) {
-        T mk = 0;
-        static_for<0, 5>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          if constexpr (kB5.M[k][q] != 0) mk += T(kB5.M[k][q]) * f[o + q];
-        });
-        T meq = temp * (T(kB5.a[k]) + T(kB5.b[k][0]) * j[0] + T(kB5.b[k][1]) * j[1]);
-        ombar -= ak[k] * (mk - meq);
-      }
-    });
-    static_for<0, 5>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      T acc = lamT[q];
-      static_for<1, 5>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        if constexpr (kB5.M[k][q] != 0) {
-          T sk = kB5.rate[k] == 1 ? omT : T(kB5.s[k]);
-          acc -= T(kB5.M[k][q]) * sk * ak[k];
-        }
-      });
-      lam[o + q] = acc + Tbar;
-    });
-    T dR = ramp_diff<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
-    gbar += ombar * (-omT * omT * T(5) * T(mp.tdiff) * dR);
-  }
-
-  static_for<0, Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    T add = rho_bar;
-    if constexpr (L::ex[q] != 0) add += T(L::ex[q]) * jb[0];
-    if constexpr (L::ey[q] != 0) add += T(L::ey[q]) * jb[1];
-    if constexpr (L::ez[q] != 0) add += T(L::ez[q]) * jb[2];
-    lam[q] += add;
-  });
-  dgam += gbar;
-}
-
-// ---------------------------------------------------------------------------------------
 // Boundary maps. Each piece is node-local; pieces run in insertion order after streaming
 // (src/latticeBoltzmann/LBSolver.cc:288-293). Adjoints run the transposes in reverse order.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ double bc_value(const DevBC& b, int D, int t, int x, int y, int z) {
-  double s, tt;
-  if (D == 2) { s = b.nrm[0] != 0 ? y : x; tt = 0; }
-  else if (b.nrm[0] != 0) { s = y; tt = z; }
-  else if (b.nrm[1] != 0) { s = x; tt = z; }
-  else { s = x; tt = y; }
-  double shape = 1.0;
+// prescribed value of a piece at node (x,y,z): value * profile(x,y,z) * tf, where tf is the
+// piece's time factor at the current step (computed on the host, see tfac in the sweep args)
+template <class T>
+__device__ __forceinline__ T bc_value(const DevBC& b, int D, T tf, int x, int y, int z) {
+  T s, tt;
+  if (D == 2) { s = T(b.nrm[0] != 0 ? y : x); tt = 0; }
+  else if (b.nrm[0] != 0) { s = T(y); tt = T(z); }
+  else if (b.nrm[1] != 0) { s = T(x); tt = T(z); }
+  else { s = T(x); tt = T(y); }
+  T shape = T(1);
   if (b.profile == LBMX_PROFILE_PARABOLIC) {
-    double R = b.radius[0], d = s - b.center[0];
+    const T R = T(b.radius[0]), d = s - T(b.center[0]);
     shape = (R * R - d * d) / (R * R);
   } else if (b.profile == LBMX_PROFILE_PARABOLIC_DISK) {
-    double R = b.radius[0], d0 = s - b.center[0], d1 = tt - b.center[1];
-    double r2 = d0 * d0 + d1 * d1;
-    shape = r2 <= R * R ? 1.0 - r2 / (R * R) : 0.0;
+    const T R = T(b.radius[0]), d0 = s - T(b.center[0]), d1 = tt - T(b.center[1]);
+    const T r2 = d0 * d0 + d1 * d1;
+    shape = r2 <= R * R ? T(1) - r2 / (R * R) : T(0);
   } else if (b.profile == LBMX_PROFILE_PARABOLIC_DUCT) {
-    double d0 = (s - b.center[0]) / b.radius[0], d1 = (tt - b.center[1]) / b.radius[1];
-    shape = (1.0 - d0 * d0) * (1.0 - d1 * d1);
+    const T d0 = (s - T(b.center[0])) / T(b.radius[0]), d1 = (tt - T(b.center[1])) / T(b.radius[1]);
+    shape = (T(1) - d0 * d0) * (T(1) - d1 * d1);
   }
-  double tf = 1.0;
-  if (b.time_mode == LBMX_TIME_SINE)
-    tf = b.t_offset + b.t_amp * sin(2.0 * 3.14159265358979323846 * t / b.t_period + b.t_phase);
-  return b.value * shape * tf;
+  return T(b.value) * shape * tf;
 }
 
 __device__ __forceinline__ bool in_box(const int* bx, int x, int y, int z) {
@@ -916,26 +507,26 @@ __device__ __forceinline__ int d2q5_incoming_mask(const int* n) {
 }
 
 template <class L, class T>
-__device__ __forceinline__ void bc_node(T (&f)[L::Q], const DevBC& b, int t, int x, int y, int z) {
+__device__ __forceinline__ void bc_node(T (&f)[L::Q], const DevBC& b, T tf, int x, int y, int z) {
   switch (b.type) {
     case LBMX_BC_BOUNCEBACK: bc_bounceback<L>(f, b.nrm); break;
     case LBMX_BC_ZOUHE_VELOCITY:
       if constexpr (L::D == 2) {
-        T v = T(bc_value(b, 2, t, x, y, z));
+        T v = bc_value<T>(b, 2, tf, x, y, z);
         zh_velocity<T>(f, b.nrm, b.inc, v * T(b.dir[0]), v * T(b.dir[1]));
       }
       break;
     case LBMX_BC_ZOUHE_PRESSURE:
-      if constexpr (L::D == 2) zh_pressure<T>(f, b.nrm, b.inc, T(bc_value(b, 2, t, x, y, z)));
+      if constexpr (L::D == 2) zh_pressure<T>(f, b.nrm, b.inc, bc_value<T>(b, 2, tf, x, y, z));
       break;
     case LBMX_BC_HECHT_VELOCITY:
       if constexpr (L::D == 3) {
-        T v = T(bc_value(b, 3, t, x, y, z));
+        T v = bc_value<T>(b, 3, tf, x, y, z);
         hecht_velocity<T>(f, b.inc, v * T(b.dir[0]), v * T(b.dir[1]), v * T(b.dir[2]));
       }
       break;
     case LBMX_BC_HECHT_PRESSURE:
-      if constexpr (L::D == 3) hecht_pressure<T>(f, b.inc, T(bc_value(b, 3, t, x, y, z)));
+      if constexpr (L::D == 3) hecht_pressure<T>(f, b.inc, bc_value<T>(b, 3, tf, x, y, z));
       break;
     case LBMX_BC_THERMAL_BOUNCEBACK:
       if constexpr (L::QT == 5) {
@@ -951,7 +542,7 @@ __device__ __forceinline__ void bc_node(T (&f)[L::Q], const DevBC& b, int t, int
     case LBMX_BC_THERMAL_DIRICHLET:
       if constexpr (L::QT == 5) {
         constexpr int o = L::QF;
-        T Tv = T(bc_value(b, 2, t, x, y, z));
+        T Tv = bc_value<T>(b, 2, tf, x, y, z);
         T s = f[o] + f[o + 1] + f[o + 2] + f[o + 3] + f[o + 4];
         int m = d2q5_incoming_mask(b.nrm);
         if (m & 2) f[o + 1] = Tv - (s - f[o + 1]);
@@ -1007,6 +598,39 @@ __device__ __forceinline__ void bc_node_adj(T (&l)[L::Q], const DevBC& b) {
   }
 }
 
+template <class L, int q, class T>
+__device__ __forceinline__ T cdot(const T (&v)[3]) {
+  T s = 0;
+  if constexpr (L::ex[q] != 0) s += T(L::ex[q]) * v[0];
+  if constexpr (L::ey[q] != 0) s += T(L::ey[q]) * v[1];
+  if constexpr (L::ez[q] != 0) s += T(L::ez[q]) * v[2];
+  return s;
+}
+
+template <class L, class T>
+__device__ __forceinline__ void flow_macros(const T (&f)[L::Q], T& rho, T (&j)[3]) {
+  T r = 0, jx = 0, jy = 0, jz = 0;
+  static_for<0, L::QF>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    r += f[q];
+    if constexpr (L::ex[q] == 1) jx += f[q];
+    if constexpr (L::ex[q] == -1) jx -= f[q];
+    if constexpr (L::ey[q] == 1) jy += f[q];
+    if constexpr (L::ey[q] == -1) jy -= f[q];
+    if constexpr (L::ez[q] == 1) jz += f[q];
+    if constexpr (L::ez[q] == -1) jz -= f[q];
+  });
+  rho = r; j[0] = jx; j[1] = jy; j[2] = jz;
+}
+
+enum : int { C_BGK = LBMX_BGK, C_MRT = LBMX_MRT, C_CASC = LBMX_CASCADED,
+             C_MRT_STD = LBMX_MRT_STANDARD, C_BGK_STD = LBMX_BGK_STANDARD };
+
+template <int COLL> struct CollTraits {
+  static constexpr bool inc = !(COLL == C_MRT_STD || COLL == C_BGK_STD);
+  static constexpr bool mrt = (COLL == C_MRT || COLL == C_MRT_STD);
+};
+
 // ---------------------------------------------------------------------------------------
 // Functional terms: value and gradient w.r.t. the node's pre-collision populations.
 // ---------------------------------------------------------------------------------------
@@ -1024,6 +648,26 @@ __device__ __forceinline__ T term_value(int kind, const T (&f)[L::Q], T rho, con
     }
   }
   return T(0);
+}
+
+// Gradient of a term in the form A + c_q.B on the flow populations (+ AT on the thermal ones).
+template <class T>
+__device__ __forceinline__ void term_source(int kind, T rho, const T (&j)[3], T temp, T wgt, T& A, T (&B)[3], T& AT) {
+  switch (kind) {
+    case 0: A += wgt; break;
+    case 1: B[0] += wgt; break;
+    case 2: {
+      const T pt = rho / T(3) + T(0.5) * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+      A += wgt * j[0] / T(3);
+      B[0] += wgt * (j[0] * j[0] + pt);
+      B[1] += wgt * j[0] * j[1];
+      B[2] += wgt * j[0] * j[2];
+    } break;
+    case 3:
+      B[0] += wgt * temp;
+      AT += wgt * j[0];
+      break;
+  }
 }
 
 template <class L, class T>

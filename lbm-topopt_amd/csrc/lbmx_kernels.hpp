@@ -1,55 +1,137 @@
-// Sweep kernels. Layout in HBM: structure-of-arrays, plane q at offset q*N, node
-// n = (z*ny + y)*nx + x (x fastest), so every wave reads/writes 64 consecutive nodes of one
-// population plane. Workgroups are 64 (x) x 4 (y) threads.
+// Sweep kernels over the padded structure-of-arrays layout (see Layout in lbmx_common.h).
 //
-//  k_fwd : pull-stream from the post-collision state p_{t-1} (out-of-domain sources take the
-//          node's own opposite population: mid-way bounce-back), boundary pieces in insertion
-//          order, functional of the new state, collision, store p_t.
-//  k_adj : one reverse step: recompute s_t = B_t S p_{t-1} (pull + boundaries), reverse
-//          collision on nu_t, functional source, transposed boundaries in reverse order, then
-//          the transposed stream as a push into nu_{t-1}; dJ/dgamma accumulated in FP64.
+// Addressing: the source of population q for node x is x - c_q (pull). Its element index is
+// q*plane - shift_q + off(x), shift_q = cx + cy*px + cz*px*py. The per-plane part is uniform
+// (scalar registers) and off(x) is one 32-bit lane offset shared by all planes, so each load
+// is a single global_load with a scalar base and no per-lane address arithmetic. Domain
+// edges need no special case: the sweep that produced a field also wrote, for every
+// boundary node, the mid-way bounce-back reflection of each outgoing population into the
+// halo cell it would have streamed to.
+//
+//  k_fwd : pull p_{t-1}, boundary pieces (insertion order), functional of s_t, collision,
+//          store p_t (aligned) + halo reflections at boundary nodes.
+//  k_adj : recompute s_t = B_t S p_{t-1}, read nu_t (own node; halo at boundary nodes),
+//          reverse collision, functional source, transposed boundaries (reverse order),
+//          push nu_{t-1} to the pull addresses (the transposed stream, halo included);
+//          dJ/dgamma accumulated in FP64.
+// Workgroups are 64 (x) x 4 (y) threads; the linear workgroup id is remapped so that each
+// XCD (workgroups are dealt round-robin to the 8 XCDs) receives a contiguous run of tiles and
+// x-neighbouring tiles share that XCD's L2 for the lines straddling their edge.
 #pragma once
-#include "lbmx_device.hpp"
+#include "lbmx_collision.hpp"
 
 namespace lbmx {
 
-template <class L, class P>
-__device__ __forceinline__ void pull_node(const typename P::S* __restrict__ src, long long N, int nx,
-                                          int ny, int nz, int x, int y, int z, long long n,
-                                          typename P::T (&f)[L::Q]) {
-  static_for<0, L::Q>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int cx = L::ex[q], cy = L::ey[q], cz = L::ez[q];
-    if constexpr (cx == 0 && cy == 0 && cz == 0) {
-      f[q] = P::template dec<L, q>(src[(long long)q * N + n]);
-    } else {
-      bool inside = true;
-      if constexpr (cx == 1) inside = inside && (x > 0);
-      if constexpr (cx == -1) inside = inside && (x < nx - 1);
-      if constexpr (cy == 1) inside = inside && (y > 0);
-      if constexpr (cy == -1) inside = inside && (y < ny - 1);
-      if constexpr (cz == 1) inside = inside && (z > 0);
-      if constexpr (cz == -1) inside = inside && (z < nz - 1);
-      const long long shift = (long long)cx + (long long)cy * nx + (long long)cz * nx * ny;
-      const long long off = inside ? (long long)q * N + n - shift : (long long)L::opp[q] * N + n;
-      f[q] = P::template dec<L, q>(src[off]);
-    }
-  });
+template <class L, int q>
+__device__ __forceinline__ long long shift_of(const Layout& l) {
+  return (long long)L::ex[q] + (long long)L::ey[q] * l.px + (long long)L::ez[q] * l.px * l.py;
 }
 
-template <class L, class T>
+template <class S>
+__device__ __forceinline__ S ld(const char* base, unsigned int off_bytes) {
+  return *reinterpret_cast<const S*>(base + off_bytes);
+}
+template <class S>
+__device__ __forceinline__ void st(char* base, unsigned int off_bytes, S v) {
+  *reinterpret_cast<S*>(base + off_bytes) = v;
+}
+
+// pull address base of plane q: start + (q*plane - shift_q) elements
+template <class L, int q, class S>
+__device__ __forceinline__ const char* pull_base(const void* start, const Layout& l) {
+  return static_cast<const char*>(start) + ((long long)q * l.plane - shift_of<L, q>(l)) * (long long)sizeof(S);
+}
+template <class L, int q, class S>
+__device__ __forceinline__ char* pull_base_w(void* start, const Layout& l) {
+  return static_cast<char*>(start) + ((long long)q * l.plane - shift_of<L, q>(l)) * (long long)sizeof(S);
+}
+template <int q, class S>
+__device__ __forceinline__ char* own_base_w(void* start, const Layout& l) {
+  return static_cast<char*>(start) + (long long)q * l.plane * (long long)sizeof(S);
+}
+template <int q, class S>
+__device__ __forceinline__ const char* own_base(const void* start, const Layout& l) {
+  return static_cast<const char*>(start) + (long long)q * l.plane * (long long)sizeof(S);
+}
+
+__device__ __forceinline__ unsigned int layout_off(const Layout& l, int x, int y, int z) {
+  return (unsigned int)(((z + l.oz) * l.py + (y + 1)) * l.px + l.ax + x);
+}
+
+// node coordinates of this thread from the XCD-remapped workgroup id
+__device__ __forceinline__ void tile_coords(const Geometry& g, int& x, int& y, int& z, int& tile) {
+  const int b = blockIdx.x;
+  const int nb = g.nblocks;
+  const int qd = nb >> 3, r = nb & 7;
+  const int xcd = b & 7, k = b >> 3;
+  tile = xcd < r ? xcd * (qd + 1) + k : r * (qd + 1) + (xcd - r) * qd + k;
+  const int bx = tile % g.nblk_x;
+  const int rest = tile / g.nblk_x;
+  const int by = rest % g.nblk_y;
+  z = rest / g.nblk_y;
+  x = bx * kTileX + threadIdx.x;
+  y = by * kTileY + threadIdx.y;
+}
+
+// true when x + c_q leaves the domain (q is an outgoing direction at a boundary node)
+template <class L, int q>
+__device__ __forceinline__ bool leaves(int x, int y, int z, int nx, int ny, int nz) {
+  bool o = false;
+  if constexpr (L::ex[q] == 1) o = o || x == nx - 1;
+  if constexpr (L::ex[q] == -1) o = o || x == 0;
+  if constexpr (L::ey[q] == 1) o = o || y == ny - 1;
+  if constexpr (L::ey[q] == -1) o = o || y == 0;
+  if constexpr (L::ez[q] == 1) o = o || z == nz - 1;
+  if constexpr (L::ez[q] == -1) o = o || z == 0;
+  return o;
+}
+
+template <class L>
 __device__ __forceinline__ bool on_boundary(int x, int y, int z, int nx, int ny, int nz) {
   bool b = x == 0 || x == nx - 1 || y == 0 || y == ny - 1;
   if constexpr (L::D == 3) b = b || z == 0 || z == nz - 1;
   return b;
 }
 
+template <class L, class P>
+__device__ __forceinline__ void pull_node(const void* src, const Layout& l, unsigned int off,
+                                          typename P::T (&f)[L::Q]) {
+  using S = typename P::S;
+  const unsigned int ob = off * (unsigned int)sizeof(S);
+  static_for<0, L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    f[q] = P::template dec<L, q>(ld<S>(pull_base<L, q, S>(src, l), ob));
+  });
+}
+
+// store post-collision populations and, at boundary nodes, the halo reflections
+template <class L, class P>
+__device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned int off, const typename P::T (&f)[L::Q],
+                                           bool bnd, int x, int y, int z, int nx, int ny, int nz) {
+  using S = typename P::S;
+  const unsigned int ob = off * (unsigned int)sizeof(S);
+  static_for<0, L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    st<S>(own_base_w<q, S>(dst, l), ob, P::template enc<L, q>(f[q]));
+  });
+  if (bnd) {
+    static_for<1, L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (L::ex[q] != 0 || L::ey[q] != 0 || L::ez[q] != 0) {
+        // p[q] of this node is what the pull of plane opp(q) at this node reads from x + c_q
+        if (leaves<L, q>(x, y, z, nx, ny, nz))
+          st<S>(pull_base_w<L, L::opp[q], S>(dst, l), ob, P::template enc<L, L::opp[q]>(f[q]));
+      }
+    });
+  }
+}
+
 template <class L, class T>
-__device__ __forceinline__ void apply_bcs(T (&f)[L::Q], const DevBC* __restrict__ bcs, int nbc, int t,
-                                          int x, int y, int z) {
+__device__ __forceinline__ void apply_bcs(T (&f)[L::Q], const DevBC* __restrict__ bcs, int nbc,
+                                          const double* tfac, int x, int y, int z) {
   for (int b = 0; b < nbc; ++b) {
     const DevBC& e = bcs[b];
-    if (in_box(e.box, x, y, z)) bc_node<L, T>(f, e, t, x, y, z);
+    if (in_box(e.box, x, y, z)) bc_node<L, T>(f, e, T(tfac[b]), x, y, z);
   }
 }
 
@@ -62,10 +144,8 @@ __device__ __forceinline__ void apply_bcs_adj(T (&l)[L::Q], const DevBC* __restr
   }
 }
 
-__device__ __forceinline__ bool block_meets(const int* bb) {
-  int x0 = blockIdx.x * kTileX, x1 = x0 + kTileX - 1;
-  int y0 = blockIdx.y * kTileY, y1 = y0 + kTileY - 1;
-  int z = blockIdx.z;
+__device__ __forceinline__ bool tile_meets(const int* bb, int x0, int y0, int z) {
+  int x1 = x0 + kTileX - 1, y1 = y0 + kTileY - 1;
   return !(x1 < bb[0] || x0 > bb[1] || y1 < bb[2] || y0 > bb[3] || z < bb[4] || z > bb[5]);
 }
 
@@ -91,70 +171,60 @@ __device__ __forceinline__ T node_objective(const ObjArgs& o, const T (&f)[L::Q]
   return acc * T(o.time_const);
 }
 
-template <class L, int COLL, class P>
+template <class L, int COLL, int DES, class P>
 __global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
   using T = typename P::T;
-  using S = typename P::S;
-  const int x = blockIdx.x * kTileX + threadIdx.x;
-  const int y = blockIdx.y * kTileY + threadIdx.y;
-  const int z = blockIdx.z;
+  int x, y, z, tile;
+  tile_coords(a.g, x, y, z, tile);
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-  const long long N = a.g.N;
   const bool active = x < nx && y < ny;
-  const bool objblk = a.obj.acc != nullptr && block_meets(a.obj.bbox);
+  const bool objblk = a.obj.acc != nullptr && a.mode == 0 &&
+                      tile_meets(a.obj.bbox, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
   double objv = 0.0;
   if (active) {
-    const long long n = ((long long)z * ny + y) * nx + x;
+    const unsigned int off = layout_off(a.ls, x, y, z);
+    const int n = (z * ny + y) * nx + x;
+    const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
     T f[L::Q];
     if (a.mode == 2) {
       const double* s = static_cast<const double*>(a.src);
-      static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[n * L::Q + decltype(qc)::value]); });
+      static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
     } else {
-      pull_node<L, P>(static_cast<const S*>(a.src), N, nx, ny, nz, x, y, z, n, f);
-      if (on_boundary<L, T>(x, y, z, nx, ny, nz)) apply_bcs<L, T>(f, a.bcs, a.nbc, a.t, x, y, z);
+      pull_node<L, P>(a.src, a.ls, off, f);
+      if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, a.tfac, x, y, z);
     }
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
-      static_for<0, L::Q>([&](auto qc) { out[n * L::Q + decltype(qc)::value] = double(f[decltype(qc)::value]); });
+      static_for<0, L::Q>([&](auto qc) { out[(long long)n * L::Q + decltype(qc)::value] = double(f[decltype(qc)::value]); });
     } else {
       if (objblk && in_box(a.obj.bbox, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
       const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
       T rho, j[3];
-      collide<L, COLL, T>(f, gam, a.m, rho, j);
-      S* dst = static_cast<S*>(a.dst);
-      static_for<0, L::Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        dst[(long long)q * N + n] = P::template enc<L, q>(f[q]);
-      });
+      collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+      store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz);
     }
   }
-  if (objblk && a.mode == 0) {
+  if (objblk) {
     double s = block_sum(objv);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      long long bid = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-      a.obj.acc[bid] += s;
-    }
+    if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += s;
   }
 }
 
 // Initial state: s_0 = f_eq(rho, u[, T]) from uniform or per-node macros; writes p_0 = C(s_0)
 // (IsoEquilibriumInitializationLoop2d, src/dynamics/isothermalLBLoops2d.hh:264-285).
-template <class L, int COLL, class P>
+template <class L, int COLL, int DES, class P>
 __global__ __launch_bounds__(kTileX* kTileY) void k_init(FwdArgs a, const double* macros, double rho0,
                                                          double ux0, double uy0, double uz0, double T0) {
   using T = typename P::T;
-  using S = typename P::S;
-  const int x = blockIdx.x * kTileX + threadIdx.x;
-  const int y = blockIdx.y * kTileY + threadIdx.y;
-  const int z = blockIdx.z;
-  const int nx = a.g.nx, ny = a.g.ny;
+  int x, y, z, tile;
+  tile_coords(a.g, x, y, z, tile);
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   if (x >= nx || y >= ny) return;
-  const long long N = a.g.N;
-  const long long n = ((long long)z * ny + y) * nx + x;
+  const int n = (z * ny + y) * nx + x;
   double r = rho0, u[3] = {ux0, uy0, uz0}, tmp = T0;
   if (macros) {
-    r = macros[5 * n]; u[0] = macros[5 * n + 1]; u[1] = macros[5 * n + 2]; u[2] = macros[5 * n + 3];
-    tmp = macros[5 * n + 4];
+    r = macros[5 * (long long)n]; u[0] = macros[5 * (long long)n + 1]; u[1] = macros[5 * (long long)n + 2];
+    u[2] = macros[5 * (long long)n + 3]; tmp = macros[5 * (long long)n + 4];
   }
   constexpr bool inc = CollTraits<COLL>::inc;
   double fd[L::Q];
@@ -180,100 +250,89 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_init(FwdArgs a, const double
   }
   if (a.aos_out) {
     double* out = static_cast<double*>(a.aos_out);
-    static_for<0, L::Q>([&](auto qc) { out[n * L::Q + decltype(qc)::value] = fd[decltype(qc)::value]; });
+    static_for<0, L::Q>([&](auto qc) { out[(long long)n * L::Q + decltype(qc)::value] = fd[decltype(qc)::value]; });
   }
   if (a.dst) {
     T f[L::Q];
     static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(fd[decltype(qc)::value]); });
     const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
     T rho, j[3];
-    collide<L, COLL, T>(f, gam, a.m, rho, j);
-    S* dst = static_cast<S*>(a.dst);
-    static_for<0, L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      dst[(long long)q * N + n] = P::template enc<L, q>(f[q]);
-    });
+    collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+    store_node<L, P>(a.dst, a.ls, layout_off(a.ls, x, y, z), f, on_boundary<L>(x, y, z, nx, ny, nz), x, y, z, nx,
+                     ny, nz);
   }
 }
 
-template <class L, int COLL, class P>
+template <class L, int COLL, int DES, class P>
 __global__ __launch_bounds__(kTileX* kTileY) void k_adj(AdjArgs a) {
   using T = typename P::T;
-  using S = typename P::S;
   using LS = typename P::LS;
-  const int x = blockIdx.x * kTileX + threadIdx.x;
-  const int y = blockIdx.y * kTileY + threadIdx.y;
-  const int z = blockIdx.z;
+  int x, y, z, tile;
+  tile_coords(a.g, x, y, z, tile);
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   if (x >= nx || y >= ny) return;
-  const long long N = a.g.N;
-  const long long n = ((long long)z * ny + y) * nx + x;
+  const int n = (z * ny + y) * nx + x;
+  const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
   T f[L::Q];
-  pull_node<L, P>(static_cast<const S*>(a.fwd_src), N, nx, ny, nz, x, y, z, n, f);
-  const bool bnd = on_boundary<L, T>(x, y, z, nx, ny, nz);
-  if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, a.t, x, y, z);
+  pull_node<L, P>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
+  if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, a.tfac, x, y, z);
+  const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
   T lam[L::Q];
-  const LS* lin = static_cast<const LS*>(a.lam_in);
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    lam[q] = T(lin[(long long)q * N + n]);
+    lam[q] = T(ld<LS>(own_base<q, LS>(a.lam_in, a.ll), offl));
   });
-  const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
-  T dg = 0;
-  reverse_collide<L, COLL, T>(f, gam, a.m, lam, dg);
-  a.sens[n] += double(dg);
+  if (bnd) {
+    // nu_t(x)[r] for x + c_r outside was pushed by this node into the halo cell x + c_r,
+    // plane opp(r): the pull address of plane opp(r)
+    static_for<1, L::Q>([&](auto qc) {
+      constexpr int r = decltype(qc)::value;
+      if constexpr (L::ex[r] != 0 || L::ey[r] != 0 || L::ez[r] != 0) {
+        if (leaves<L, r>(x, y, z, nx, ny, nz)) lam[r] = T(ld<LS>(pull_base<L, L::opp[r], LS>(a.lam_in, a.ll), offl));
+      }
+    });
+  }
+  T srcA = 0, srcB[3] = {0, 0, 0}, srcAT = 0;
   if (a.obj.nterm > 0 && in_box(a.obj.bbox, x, y, z)) {
     T rho, j[3];
     flow_macros<L>(f, rho, j);
+    T temp = 0;
+    if constexpr (L::QT > 0) temp = f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
     for (int m = 0; m < a.obj.nterm; ++m)
       if (in_box(a.obj.terms[m].box, x, y, z))
-        term_grad<L, T>(a.obj.terms[m].kind, f, rho, j, T(a.obj.time_const * a.obj.terms[m].weight), lam);
+        term_source<T>(a.obj.terms[m].kind, rho, j, temp, T(a.obj.time_const * a.obj.terms[m].weight), srcA, srcB,
+                       srcAT);
   }
+  const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
+  T dg = 0;
+  reverse_collide<L, COLL, DES, T>(f, gam, a.m, lam, dg, srcA, srcB, srcAT);
+  a.sens[n] += double(dg);
   if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, a.nbc, x, y, z);
-  LS* lout = static_cast<LS*>(a.lam_out);
+  // transposed stream: nu_{t-1}(x - c_q)[q] = lam[q], i.e. a store to the pull address
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    constexpr int cx = L::ex[q], cy = L::ey[q], cz = L::ez[q];
-    if constexpr (cx == 0 && cy == 0 && cz == 0) {
-      lout[(long long)q * N + n] = LS(lam[q]);
-    } else {
-      bool inside = true;
-      if constexpr (cx == 1) inside = inside && (x > 0);
-      if constexpr (cx == -1) inside = inside && (x < nx - 1);
-      if constexpr (cy == 1) inside = inside && (y > 0);
-      if constexpr (cy == -1) inside = inside && (y < ny - 1);
-      if constexpr (cz == 1) inside = inside && (z > 0);
-      if constexpr (cz == -1) inside = inside && (z < nz - 1);
-      const long long shift = (long long)cx + (long long)cy * nx + (long long)cz * nx * ny;
-      const long long off = inside ? (long long)q * N + n - shift : (long long)L::opp[q] * N + n;
-      lout[off] = LS(lam[q]);
-    }
+    st<LS>(pull_base_w<L, q, LS>(a.lam_out, a.ll), offl, LS(lam[q]));
   });
 }
 
-// ---- host-side launchers (one instantiation set per lattice translation unit) ----
-template <class L, int COLL, class P>
+// ---- host-side launchers ----
+template <class L, int COLL, int DES, class P>
 hipError_t launch_fwd_t(const FwdArgs& a, hipStream_t s) {
-  dim3 blk(kTileX, kTileY, 1);
-  dim3 grd((a.g.nx + kTileX - 1) / kTileX, (a.g.ny + kTileY - 1) / kTileY, a.g.nz);
-  hipLaunchKernelGGL((k_fwd<L, COLL, P>), grd, blk, 0, s, a);
+  hipLaunchKernelGGL((k_fwd<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), 0, s, a);
   return hipGetLastError();
 }
 
-template <class L, int COLL, class P>
+template <class L, int COLL, int DES, class P>
 hipError_t launch_init_t(const FwdArgs& a, const double* macros, const double* u, double rho0, double T0,
                          hipStream_t s) {
-  dim3 blk(kTileX, kTileY, 1);
-  dim3 grd((a.g.nx + kTileX - 1) / kTileX, (a.g.ny + kTileY - 1) / kTileY, a.g.nz);
-  hipLaunchKernelGGL((k_init<L, COLL, P>), grd, blk, 0, s, a, macros, rho0, u[0], u[1], u[2], T0);
+  hipLaunchKernelGGL((k_init<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), 0, s, a, macros, rho0,
+                     u[0], u[1], u[2], T0);
   return hipGetLastError();
 }
 
-template <class L, int COLL, class P>
+template <class L, int COLL, int DES, class P>
 hipError_t launch_adj_t(const AdjArgs& a, hipStream_t s) {
-  dim3 blk(kTileX, kTileY, 1);
-  dim3 grd((a.g.nx + kTileX - 1) / kTileX, (a.g.ny + kTileY - 1) / kTileY, a.g.nz);
-  hipLaunchKernelGGL((k_adj<L, COLL, P>), grd, blk, 0, s, a);
+  hipLaunchKernelGGL((k_adj<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), 0, s, a);
   return hipGetLastError();
 }
 

@@ -1,4 +1,4 @@
-// Host entry points of the sweep kernels, one instantiation set per lattice.
+// Host entry points of the sweep kernels, one instantiation set per (lattice, storage type).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -6,16 +6,18 @@
 
 namespace lbmx {
 
-// lattice-specific dispatchers (defined in kernels_<lattice>.hip); they return
-// hipErrorInvalidValue for an operator/precision combination that is not instantiated.
-#define LBMX_DECLARE_LAUNCHERS(SUFFIX)                                                          \
-  hipError_t launch_fwd_##SUFFIX(int coll, int prec, const FwdArgs& a, hipStream_t s);          \
-  hipError_t launch_init_##SUFFIX(int coll, int prec, const FwdArgs& a, const double* macros,   \
-                                  const double* u, double rho0, double T0, hipStream_t s);      \
-  hipError_t launch_adj_##SUFFIX(int coll, int prec, const AdjArgs& a, hipStream_t s);
-LBMX_DECLARE_LAUNCHERS(d2q9)
-LBMX_DECLARE_LAUNCHERS(d3q19)
-LBMX_DECLARE_LAUNCHERS(thermal)
+// Each returns hipErrorInvalidValue for an operator/coupling combination that is not built.
+#define LBMX_DECLARE_LAUNCHERS(SUFFIX)                                                                   \
+  hipError_t launch_fwd_##SUFFIX(int coll, int des, const FwdArgs& a, hipStream_t s);                    \
+  hipError_t launch_init_##SUFFIX(int coll, int des, const FwdArgs& a, const double* macros,             \
+                                  const double* u, double rho0, double T0, hipStream_t s);               \
+  hipError_t launch_adj_##SUFFIX(int coll, int des, const AdjArgs& a, hipStream_t s);
+#define LBMX_DECLARE_LATTICE(LAT) \
+  LBMX_DECLARE_LAUNCHERS(LAT##_f64) LBMX_DECLARE_LAUNCHERS(LAT##_f32) LBMX_DECLARE_LAUNCHERS(LAT##_f16) LBMX_DECLARE_LAUNCHERS(LAT##_bf16)
+LBMX_DECLARE_LATTICE(d2q9)
+LBMX_DECLARE_LATTICE(d3q19)
+LBMX_DECLARE_LATTICE(thermal)
+#undef LBMX_DECLARE_LATTICE
 #undef LBMX_DECLARE_LAUNCHERS
 
 }  // namespace lbmx

@@ -1,0 +1,44 @@
+// Streaming copy used as the measured HBM ceiling and as the calibration kernel for the
+// FETCH_SIZE / WRITE_SIZE counters (known bytes read and written per launch).
+#include <hip/hip_runtime.h>
+
+#include "lbmx.h"
+
+namespace lbmx {
+
+__global__ __launch_bounds__(256) void k_stream_copy(const double4* __restrict__ in, double4* __restrict__ out,
+                                                      long long n4) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n4; i += stride) out[i] = in[i];
+}
+
+}  // namespace lbmx
+
+extern "C" lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs) {
+  if (!gbs || iters < 1 || bytes < 1024) return LBMX_ERR_ARG;
+  const long long n4 = (long long)(bytes / 32);
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n4 * 32) != hipSuccess) return LBMX_ERR_MEM;
+  if (hipMalloc(&b, n4 * 32) != hipSuccess) { (void)hipFree(a); return LBMX_ERR_MEM; }
+  (void)hipMemset(a, 0, n4 * 32);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const int blocks = 256 * 32;
+  hipLaunchKernelGGL(lbmx::k_stream_copy, dim3(blocks), dim3(256), 0, nullptr, (const double4*)a, (double4*)b, n4);
+  (void)hipEventRecord(e0, nullptr);
+  for (int k = 0; k < iters; ++k)
+    hipLaunchKernelGGL(lbmx::k_stream_copy, dim3(blocks), dim3(256), 0, nullptr, (const double4*)a, (double4*)b, n4);
+  (void)hipEventRecord(e1, nullptr);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) return LBMX_ERR_DEVICE;
+  *gbs = 2.0 * n4 * 32 * iters / (ms * 1e-3) / 1e9;
+  return LBMX_OK;
+}
