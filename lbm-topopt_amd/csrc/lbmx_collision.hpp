@@ -214,6 +214,13 @@ __device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3
     // MRT: f' = f - M^-1 [S (M f - m_eq) - (I - S/2) M src]
     constexpr int Q = L::QF;
     const T inv_rho_s = inc ? T(1) : recip<T>(rho);
+    // equilibrium moments take the momentum: j (incompressible) or rho v (standard, whose
+    // quadratic terms are j_a j_b / rho: src/dynamics/MRT3d.hh:270-285)
+    T vm[3];
+    static_for<0, 3>([&](auto ac) {
+      constexpr int a = decltype(ac)::value;
+      vm[a] = inc ? v[a] : rho * v[a];
+    });
     T dm[Q];
     static_for<0, Q>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
@@ -221,7 +228,7 @@ __device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3
       if constexpr (LBMX_BAS.rate[k] != 0) {
         T mk = pair_moment<L, k>(p);
         T sk = LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]);
-        T val = sk * (mk - meq_row<L, k>(rho, v, inv_rho_s));
+        T val = sk * (mk - meq_row<L, k>(rho, vm, inv_rho_s));
         if constexpr (DES == D_BRINK && row_has_src<L, k>())
           val += (T(1) - T(0.5) * sk) * alpha * src_row<L, k>(v, u2);
         dm[k] = val * T(LBMX_BAS.invn2[k]);

@@ -42,3 +42,11 @@ extern "C" lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs) {
   *gbs = 2.0 * n4 * 32 * iters / (ms * 1e-3) / 1e9;
   return LBMX_OK;
 }
+
+extern "C" lbmx_status lbmx_device_count(int* count) {
+  if (!count) return LBMX_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return LBMX_OK;
+}

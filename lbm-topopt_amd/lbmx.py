@@ -137,6 +137,7 @@ def library():
         "lbmx_device_bytes": [C.c_void_p, P(C.c_size_t)],
         "lbmx_time_sweeps": [C.c_void_p, C.c_int, C.c_int, dp],
         "lbmx_stream_copy": [C.c_size_t, C.c_int, dp],
+        "lbmx_device_count": [P(C.c_int)],
         "lbmx_describe_boundaries": [C.c_void_p, P(C.c_int), C.c_int, P(C.c_int)],
         "lbmx_checkpoint_simulate": [C.c_int, C.c_int, P(C.c_longlong), P(C.c_int)],
     }
@@ -155,7 +156,8 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_init_equilibrium_field", "lbmx_forward", "lbmx_get_state", "lbmx_set_state",
             "lbmx_get_macros", "lbmx_get_timestep", "lbmx_checkpoint_config", "lbmx_gradient",
             "lbmx_get_sensitivity", "lbmx_get_adjoint", "lbmx_get_stats", "lbmx_device_bytes",
-            "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps", "lbmx_stream_copy"]
+            "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps", "lbmx_stream_copy",
+            "lbmx_device_count"]
 
 
 def checkpoint_simulate(nslots, nsteps):
@@ -164,6 +166,13 @@ def checkpoint_simulate(nslots, nsteps):
     mx = C.c_int(0)
     _check(library().lbmx_checkpoint_simulate(nslots, nsteps, C.byref(adv), C.byref(mx)))
     return adv.value, mx.value
+
+
+def device_count():
+    """Number of visible HIP devices."""
+    n = C.c_int(0)
+    _check(library().lbmx_device_count(C.byref(n)))
+    return n.value
 
 
 def stream_copy_gbs(nbytes=1 << 31, iters=20):

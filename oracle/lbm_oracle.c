@@ -500,6 +500,10 @@ static void base_collide(const orc_problem* p, const double* f, double* out, dou
      src/dynamics/MRT3d.hh:44-176, :222-359) */
   const orc_mrt* m = &p->mrt;
   double dm[QMAX], srcm[QMAX], src[QMAX];
+  /* the equilibrium moments take the momentum: j itself (incompressible), rho v for
+     StandardMRT3d, whose quadratic terms are j_a j_b / rho (src/dynamics/MRT3d.hh:270-285) */
+  double vm[3];
+  for (int a = 0; a < 3; ++a) vm[a] = p->incompressible ? v[a] : rho * v[a];
   if (forced)
     for (int q = 0; q < Q; ++q) src[q] = guo_src(p, q, v, alpha);
   for (int k = 0; k < Q; ++k) {
@@ -509,7 +513,7 @@ static void base_collide(const orc_problem* p, const double* f, double* out, dou
     if (m->row[k].rate != 0) {
       for (int q = 0; q < Q; ++q)
         if (m->M[k][q] != 0.) mk += m->M[k][q] * f[q];
-      dm[k] = row_s(&m->row[k], om) * (mk - row_meq(&m->row[k], rho, v, rho_s)) * m->inv_norm2[k];
+      dm[k] = row_s(&m->row[k], om) * (mk - row_meq(&m->row[k], rho, vm, rho_s)) * m->inv_norm2[k];
     }
     if (forced) {
       double sk = 0;

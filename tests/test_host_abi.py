@@ -1,0 +1,159 @@
+"""C-ABI library and host logic, without a device.
+
+* liblbmx.so loads and exports every function that include/lbmx.h declares;
+* the Python binding's symbol list is the header's;
+* boundary boxes split into the same pieces, in the same order, as the oracle's restatement
+  of the reference's splitting, with the documented corner/edge rule;
+* the checkpoint schedule replay (Wang dynamic checkpointing) delivers the right state at
+  every reverse step, never exceeds its slots, and meets closed forms and bounds.
+"""
+import os
+import re
+from math import comb
+
+import numpy as np
+import pytest
+
+import lbmx
+import oracle
+import workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lbmx.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbmx_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_symbols():
+    assert header_functions() == sorted(lbmx.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = lbmx.library()
+    missing = [n for n in header_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_error_paths_without_device():
+    lib = lbmx.library()
+    import ctypes as C
+    assert lib.lbmx_create(None, None) == lbmx.ERR_ARG
+    assert lib.lbmx_destroy(None) == lbmx.LBMX_OK
+    assert lbmx.library().lbmx_last_error() is not None
+    bad = lbmx.make_desc(0, 4)
+    h = C.c_void_p()
+    assert lib.lbmx_create(C.byref(bad), C.byref(h)) != lbmx.LBMX_OK
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.checkpoint_simulate(1, 10)
+
+
+def _pieces_engine_vs_oracle(desc, setup, *dims):
+    s = lbmx.Solver(desc)
+    o = oracle.Oracle(desc)
+    for obj in (s, o):
+        setup(obj, *dims)
+    got = s.boundaries()
+    exp = o.boundaries()
+    s.close()
+    return got, exp
+
+
+@pytest.mark.parametrize("case", ["channel", "duct", "thermal"])
+def test_boundary_splitting_matches_oracle(case):
+    if case == "duct":
+        d = lbmx.make_desc(9, 7, 6, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN)
+        got, exp = _pieces_engine_vs_oracle(d, workloads.duct3d, 9, 7, 6)
+        # 4 side walls: faces, 4 edges along x, inlet/outlet corners handled by their pieces
+        assert len(got) > 8
+    elif case == "thermal":
+        d = lbmx.make_desc(9, 7, lattice=lbmx.D2Q9_D2Q5, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN)
+        got, exp = _pieces_engine_vs_oracle(d, workloads.thermal_channel2d, 9, 7)
+    else:
+        d = lbmx.make_desc(9, 7, lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_PBB)
+        got, exp = _pieces_engine_vs_oracle(d, workloads.channel2d, 9, 7)
+    assert [(t, tuple(n), tuple(b)) for t, n, b in got] == [(t, tuple(n), tuple(b)) for t, n, b in exp]
+    # every boundary node of a piece lies on the face/edge/corner its normal names
+    nx, ny, nz = d.nx, d.ny, d.nz
+    for t, n, b in got:
+        for a, (lo, hi, N) in enumerate(((b[0], b[1], nx), (b[2], b[3], ny), (b[4], b[5], nz))):
+            if n[a] == -1:
+                assert lo == hi == 0
+            elif n[a] == 1:
+                assert lo == hi == N - 1
+            elif N > 1:
+                assert lo >= 1 and hi <= N - 2, "edge boxes exclude the corners"
+
+
+def test_unsupported_boundary_is_rejected_without_side_effects():
+    d = lbmx.make_desc(8, 6, lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_PBB)
+    s = lbmx.Solver(d)
+    n0 = len(s.boundaries())
+    with pytest.raises(lbmx.LbmxError):  # Hecht conditions are 3D only
+        s.add_boundary(lbmx.make_box(0, 0, 1, 4), lbmx.make_bc(lbmx.BC_HECHT_VELOCITY, 0.01, (1.0, 0.0, 0.0)))
+    assert len(s.boundaries()) == n0
+    s.close()
+
+
+def test_design_roundtrip_and_validation():
+    d = lbmx.make_desc(8, 6, lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_PBB)
+    s = lbmx.Solver(d)
+    g = workloads.design_field(8, 6, 1, seed=11)
+    s.set_design(g)
+    assert np.array_equal(s.get_design(), g)
+    with pytest.raises(lbmx.LbmxError):
+        s.set_design(np.zeros(5))
+    s.close()
+
+
+# ----------------------------------------------------------------------- checkpointing
+def test_checkpoint_single_slot_closed_form():
+    """Two slots leave one stored state: every reverse step restarts from state 0, so the
+    recomputation is sum_{i=1}^{n-2} i = (n-1)(n-2)/2 forward steps."""
+    for n in (3, 5, 10, 37, 100):
+        adv, mx = lbmx.checkpoint_simulate(2, n)
+        assert adv == comb(n - 1, 2)
+        assert mx <= 2
+
+
+def test_checkpoint_enough_slots_no_recompute():
+    for n in (2, 7, 30):
+        adv, mx = lbmx.checkpoint_simulate(n, n)
+        assert adv == 0 and mx <= n
+
+
+@pytest.mark.parametrize("slots", [3, 4, 6, 10, 48, 64])
+def test_checkpoint_invariants_and_bound(slots):
+    prev = None
+    for n in (10, 50, 200, 1000):
+        adv, mx = lbmx.checkpoint_simulate(slots, n)  # raises if a wrong state is delivered
+        assert mx <= slots
+        # repetition bound: with s - 1 free checkpoints every state is recomputed at most
+        # t - 1 times, t the least repetition number with C(s - 1 + t, t) >= n (Griewank)
+        t = 1
+        while comb(slots - 1 + t, t) < n:
+            t += 1
+        assert adv <= (t - 1) * n + n
+        if prev is not None:
+            assert adv >= prev[0]
+        prev = (adv, n)
+    # more slots never cost more recomputation
+    a1, _ = lbmx.checkpoint_simulate(slots, 500)
+    a2, _ = lbmx.checkpoint_simulate(slots + 1, 500)
+    assert a2 <= a1
+
+
+def test_checkpoint_headline_schedules():
+    """The checkpoint counts of the benchmark workloads stay within twice the offline optimum."""
+    def revolve_recompute(l, s):
+        t = 0
+        while comb(s + t, t) < l:
+            t += 1
+        return t * l - comb(s + t, t - 1) - l
+    for slots, n in ((64, 5000), (48, 3000)):
+        adv, mx = lbmx.checkpoint_simulate(slots, n)
+        assert mx <= slots
+        assert adv <= 2 * revolve_recompute(n - 1, slots - 1) + n
