@@ -4,8 +4,8 @@
 //
 // The flow populations are handled in "pair space": f_0, s_d = f_d + f_d' and a_d = f_d - f_d'
 // for the opposite pairs (d, d' = d + HALF). Every moment-basis row is even or odd in c, so
-// moments read only s (even rows) or a (odd rows) and the back transform updates s by -2E and
-// a by -2O: half the multiply-adds of a direct M f / M^T dm. Partial bounce-back only scales
+// moments read only s (even rows) or a (odd rows) and the back transform of a row updates
+// only s (by -2 M_k dm_k) or only a: half the multiply-adds of a direct M f / M^T dm. Partial bounce-back only scales
 // a by (1 - 2 imult). The macros are rho = f_0 + sum s_d, j = sum c_d a_d. The moments of the
 // Guo source w_q[3(c-v).F + 9(c.v)(c.F)] are contractions with compile-time tables.
 #pragma once
@@ -107,40 +107,22 @@ __device__ __forceinline__ T pair_moment(const Pairs<L, T>& p) {
   return m;
 }
 
-// x -= M^T c over the rows selected by Rows (and by parity when ParMask != 0), in pair space
-template <class L, class Rows, int ParMask, class T>
-__device__ __forceinline__ void pair_back(Pairs<L, T>& x, const T (&c)[L::QF]) {
-  constexpr int Q = L::QF;
-  if constexpr (ParMask != -1) {
-    T e0 = 0;
-    static_for<0, Q>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      if constexpr (Rows::template on<k>() && LBMX_BAS.par[k] == 1 && LBMX_BAS.M[k][0] != 0)
-        e0 += T(LBMX_BAS.M[k][0]) * c[k];
+// x -= M_k^T c for one row k, in pair space
+template <class L, int k, class T>
+__device__ __forceinline__ void pair_row_update(Pairs<L, T>& x, T c) {
+  if constexpr (LBMX_BAS.par[k] == 1) {
+    if constexpr (LBMX_BAS.M[k][0] != 0) x.f0 -= T(LBMX_BAS.M[k][0]) * c;
+    static_for<1, L::HALF + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      if constexpr (LBMX_BAS.M[k][d] != 0) x.s[d] -= T(2.0 * LBMX_BAS.M[k][d]) * c;
     });
-    x.f0 -= e0;
+  } else {
+    static_for<1, L::HALF + 1>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      if constexpr (LBMX_BAS.M[k][d] != 0) x.a[d] -= T(2.0 * LBMX_BAS.M[k][d]) * c;
+    });
   }
-  static_for<1, L::HALF + 1>([&](auto dc) {
-    constexpr int d = decltype(dc)::value;
-    T e = 0, o = 0;
-    static_for<0, Q>([&](auto kc) {
-      constexpr int k = decltype(kc)::value;
-      if constexpr (Rows::template on<k>() && LBMX_BAS.M[k][d] != 0) {
-        if constexpr (LBMX_BAS.par[k] == 1 && ParMask != -1) e += T(LBMX_BAS.M[k][d]) * c[k];
-        if constexpr (LBMX_BAS.par[k] == -1 && ParMask != 1) o += T(LBMX_BAS.M[k][d]) * c[k];
-      }
-    });
-    if constexpr (ParMask != -1) x.s[d] -= T(2) * e;
-    if constexpr (ParMask != 1) x.a[d] -= T(2) * o;
-  });
 }
-
-template <class L, int DES> struct ActiveRows {
-  template <int k> static constexpr bool on() { return row_active<L, DES, k>(); }
-};
-template <class L> struct RelaxedRows {
-  template <int k> static constexpr bool on() { return LBMX_BAS.rate[k] != 0; }
-};
 
 // m_eq of row k (src/dynamics/incompressibleMRT2d.hh:62-67, src/dynamics/MRT3d.hh:92-109)
 template <class L, int k, class T>
@@ -211,7 +193,10 @@ __device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3
       p.a[d] = a;
     });
   } else {
-    // MRT: f' = f - M^-1 [S (M f - m_eq) - (I - S/2) M src]
+    // MRT: f' = f - M^-1 [S (M f - m_eq) - (I - S/2) M src]. The rows of M are orthogonal,
+    // so relaxing them one after another, each as the rank-1 update f -= M_k^T dm_k / |M_k|^2,
+    // leaves the moments of the remaining rows unchanged and gives the simultaneous result;
+    // only the populations stay live (no vector of moments).
     constexpr int Q = L::QF;
     const T inv_rho_s = inc ? T(1) : recip<T>(rho);
     // equilibrium moments take the momentum: j (incompressible) or rho v (standard, whose
@@ -221,22 +206,19 @@ __device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3
       constexpr int a = decltype(ac)::value;
       vm[a] = inc ? v[a] : rho * v[a];
     });
-    T dm[Q];
     static_for<0, Q>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
-      dm[k] = 0;
       if constexpr (LBMX_BAS.rate[k] != 0) {
         T mk = pair_moment<L, k>(p);
         T sk = LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]);
         T val = sk * (mk - meq_row<L, k>(rho, vm, inv_rho_s));
         if constexpr (DES == D_BRINK && row_has_src<L, k>())
           val += (T(1) - T(0.5) * sk) * alpha * src_row<L, k>(v, u2);
-        dm[k] = val * T(LBMX_BAS.invn2[k]);
+        pair_row_update<L, k>(p, val * T(LBMX_BAS.invn2[k]));
       } else if constexpr (DES == D_BRINK && row_has_src<L, k>()) {
-        dm[k] = alpha * src_row<L, k>(v, u2) * T(LBMX_BAS.invn2[k]);
+        pair_row_update<L, k>(p, alpha * src_row<L, k>(v, u2) * T(LBMX_BAS.invn2[k]));
       }
     });
-    pair_back<L, ActiveRows<L, DES>, 0>(p, dm);
   }
 }
 
@@ -351,17 +333,16 @@ __device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, con
         gd[d] = pf.a[d] * (T(1) - omega) + omega * T(6.0 * L::w[d]) * cdot<L, d>(v);
       });
     } else {
-      T dm[Q];
+      Pairs<L, T> g;
+      static_for<1, H + 1>([&](auto dc) { g.a[decltype(dc)::value] = pf.a[decltype(dc)::value]; });
       static_for<0, Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        dm[k] = 0;
         if constexpr (LBMX_BAS.rate[k] != 0 && LBMX_BAS.par[k] == -1) {
           T sk = LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]);
-          dm[k] = sk * (pair_moment<L, k>(pf) - meq_row<L, k>(rho, v, T(1))) * T(LBMX_BAS.invn2[k]);
+          T dm = sk * (pair_moment<L, k>(g) - meq_row<L, k>(rho, v, T(1))) * T(LBMX_BAS.invn2[k]);
+          pair_row_update<L, k>(g, dm);
         }
       });
-      Pairs<L, T> g = pf;
-      pair_back<L, RelaxedRows<L>, -1>(g, dm);
       static_for<1, H + 1>([&](auto dc) { gd[decltype(dc)::value] = g.a[decltype(dc)::value]; });
     }
     const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
@@ -414,16 +395,15 @@ __device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, con
       pl.a[decltype(dc)::value] *= one_m;
     });
   } else {
-    T bk[Q];
+    // rows in sequence as rank-1 updates of lam (orthogonal rows: the same as all at once)
     static_for<0, Q>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
-      bk[k] = 0;
       if constexpr (row_active<L, DES, k>()) {
         const T ak = pair_moment<L, k>(pl) * T(LBMX_BAS.invn2[k]);
         const T sk = LBMX_BAS.rate[k] == 0 ? T(0) : (LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]));
         if constexpr (LBMX_BAS.rate[k] != 0) {
           const T b = sk * ak;
-          bk[k] = b;
+          pair_row_update<L, k>(pl, b);
           if constexpr (LBMX_BAS.a[k] != 0) rho_bar += b * T(LBMX_BAS.a[k]);
           static_for<0, 3>([&](auto ac) {
             constexpr int a = decltype(ac)::value;
@@ -457,7 +437,6 @@ __device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, con
         }
       }
     });
-    pair_back<L, RelaxedRows<L>, 0>(pl, bk);
   }
   T jb[3] = {beta * vb[0] + srcB[0], beta * vb[1] + srcB[1], beta * vb[2] + srcB[2]};
   rho_bar += srcA;

@@ -279,6 +279,10 @@ template <class T> __device__ __forceinline__ T ramp_diff(T p, T ck, T s) {
 // ---------------------------------------------------------------------------------------
 // Boundary maps. Each piece is node-local; pieces run in insertion order after streaming
 // (src/latticeBoltzmann/LBSolver.cc:288-293). Adjoints run the transposes in reverse order.
+// A map whose written elements depend on a runtime orientation works on a local copy and
+// writes every element back: stores to different elements on different branches would
+// otherwise be merged into one store at a runtime index, which keeps the caller's population
+// array out of registers (in scratch memory).
 // ---------------------------------------------------------------------------------------
 // prescribed value of a piece at node (x,y,z): value * profile(x,y,z) * tf, where tf is the
 // piece's time factor at the current step (computed on the host, see tfac in the sweep args)
@@ -311,6 +315,19 @@ __device__ __forceinline__ bool in_box(const int* bx, int x, int y, int z) {
 // on-grid bounce-back: q takes opp(q) when c_q.n <= 0 (incoming or tangential), per
 // OnGridBounceback2d/3d (src/boundaryConditions/bounceback2d.cc:7-193, bounceback3d.cc:8-511);
 // the (x0,y0,z0) corner of bounceback3d.cc:434-451 keeps f13 and f9 and sets f18 = f9.
+// c ? a : b on the bit patterns. Selecting between two array elements with a runtime
+// condition otherwise lets the optimiser turn the pair of loads into one load through a
+// selected address, which forces the population array into private (scratch) memory.
+__device__ __forceinline__ double bsel(bool c, double a, double b) {
+  const unsigned long long m = 0ull - (unsigned long long)c;
+  return __builtin_bit_cast(double, (__builtin_bit_cast(unsigned long long, a) & m) |
+                                        (__builtin_bit_cast(unsigned long long, b) & ~m));
+}
+__device__ __forceinline__ float bsel(bool c, float a, float b) {
+  const unsigned m = 0u - (unsigned)c;
+  return __builtin_bit_cast(float, (__builtin_bit_cast(unsigned, a) & m) | (__builtin_bit_cast(unsigned, b) & ~m));
+}
+
 template <class L, int q>
 __device__ __forceinline__ bool bb_from_opp(const int* n) {
   if constexpr (q == 0 || q >= L::QF) return false;
@@ -329,7 +346,7 @@ __device__ __forceinline__ void bc_bounceback(T (&f)[L::Q], const int* n) {
   T o[L::QF];
   static_for<0, L::QF>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    o[q] = bb_from_opp<L, q>(n) ? f[L::opp[q]] : f[q];
+    o[q] = bsel(bb_from_opp<L, q>(n), f[L::opp[q]], f[q]);
   });
   static_for<0, L::QF>([&](auto qc) { f[decltype(qc)::value] = o[decltype(qc)::value]; });
 }
@@ -339,10 +356,8 @@ __device__ __forceinline__ void bc_bounceback_adj(T (&l)[L::Q], const int* n) {
   T o[L::QF];
   static_for<0, L::QF>([&](auto qc) {
     constexpr int r = decltype(qc)::value;
-    T acc = bb_from_opp<L, r>(n) ? T(0) : l[r];
-    if constexpr (r != 0) {
-      if (bb_from_opp<L, L::opp[r]>(n)) acc += l[L::opp[r]];
-    }
+    T acc = bsel(bb_from_opp<L, r>(n), T(0), l[r]);
+    if constexpr (r != 0) acc += bsel(bb_from_opp<L, L::opp[r]>(n), l[L::opp[r]], T(0));
     o[r] = acc;
   });
   static_for<0, L::QF>([&](auto qc) { l[decltype(qc)::value] = o[decltype(qc)::value]; });
@@ -352,87 +367,99 @@ __device__ __forceinline__ void bc_bounceback_adj(T (&l)[L::Q], const int* n) {
 // standard velocity W, pressure E/W (both variants).
 template <class T>
 __device__ __forceinline__ void zh_velocity(T* f, const int* n, bool inc, T ux, T uy) {
+  T g[9];
+  for (int q = 0; q < 9; ++q) g[q] = f[q];
   if (n[0] == -1) {
     if (inc) {
-      f[1] = T(0.5) * (f[6] - f[2]) + f[5] + ux / T(6) - T(0.5) * uy;
-      f[7] = T(0.5) * (f[2] - f[6]) + f[3] + ux / T(6) + T(0.5) * uy;
-      f[8] = f[4] + T(2) / T(3) * ux;
+      g[1] = T(0.5) * (g[6] - g[2]) + g[5] + ux / T(6) - T(0.5) * uy;
+      g[7] = T(0.5) * (g[2] - g[6]) + g[3] + ux / T(6) + T(0.5) * uy;
+      g[8] = g[4] + T(2) / T(3) * ux;
     } else {
-      T rho = (f[0] + f[2] + f[6] + T(2) * (f[3] + f[4] + f[5])) / (T(1) - ux);
-      f[1] = f[5] + T(0.5) * (f[6] - f[2]) + rho * ux / T(6) - T(0.5) * rho * uy;
-      f[7] = f[3] + T(0.5) * (f[2] - f[6]) + rho * ux / T(6) + T(0.5) * rho * uy;
-      f[8] = f[4] + T(2) / T(3) * rho * ux;
+      T rho = (g[0] + g[2] + g[6] + T(2) * (g[3] + g[4] + g[5])) / (T(1) - ux);
+      g[1] = g[5] + T(0.5) * (g[6] - g[2]) + rho * ux / T(6) - T(0.5) * rho * uy;
+      g[7] = g[3] + T(0.5) * (g[2] - g[6]) + rho * ux / T(6) + T(0.5) * rho * uy;
+      g[8] = g[4] + T(2) / T(3) * rho * ux;
     }
   } else if (n[1] == 1) {
-    f[1] = T(0.5) * (f[4] - f[8]) + f[5] - uy / T(6) + T(0.5) * ux;
-    f[2] = f[6] - T(2) / T(3) * uy;
-    f[3] = T(0.5) * (f[8] - f[4]) + f[7] - uy / T(6) - T(0.5) * ux;
+    g[1] = T(0.5) * (g[4] - g[8]) + g[5] - uy / T(6) + T(0.5) * ux;
+    g[2] = g[6] - T(2) / T(3) * uy;
+    g[3] = T(0.5) * (g[8] - g[4]) + g[7] - uy / T(6) - T(0.5) * ux;
   } else {
-    f[3] = f[7] + T(0.5) * (f[6] - f[2]) - ux / T(6) - T(0.5) * uy;
-    f[4] = f[8] - T(2) / T(3) * ux;
-    f[5] = f[1] + T(0.5) * (f[2] - f[6]) - ux / T(6) + T(0.5) * uy;
+    g[3] = g[7] + T(0.5) * (g[6] - g[2]) - ux / T(6) - T(0.5) * uy;
+    g[4] = g[8] - T(2) / T(3) * ux;
+    g[5] = g[1] + T(0.5) * (g[2] - g[6]) - ux / T(6) + T(0.5) * uy;
   }
+  for (int q = 0; q < 9; ++q) f[q] = g[q];
 }
 
 template <class T>
 __device__ __forceinline__ void zh_velocity_adj(T* l, const int* n) {
+  T g[9];
+  for (int q = 0; q < 9; ++q) g[q] = l[q];
   if (n[0] == -1) {
-    l[2] += T(0.5) * (l[7] - l[1]);
-    l[6] += T(0.5) * (l[1] - l[7]);
-    l[5] += l[1]; l[3] += l[7]; l[4] += l[8];
-    l[1] = 0; l[7] = 0; l[8] = 0;
+    g[2] += T(0.5) * (g[7] - g[1]);
+    g[6] += T(0.5) * (g[1] - g[7]);
+    g[5] += g[1]; g[3] += g[7]; g[4] += g[8];
+    g[1] = 0; g[7] = 0; g[8] = 0;
   } else if (n[1] == 1) {
-    l[4] += T(0.5) * (l[1] - l[3]);
-    l[8] += T(0.5) * (l[3] - l[1]);
-    l[5] += l[1]; l[6] += l[2]; l[7] += l[3];
-    l[1] = 0; l[2] = 0; l[3] = 0;
+    g[4] += T(0.5) * (g[1] - g[3]);
+    g[8] += T(0.5) * (g[3] - g[1]);
+    g[5] += g[1]; g[6] += g[2]; g[7] += g[3];
+    g[1] = 0; g[2] = 0; g[3] = 0;
   } else {
-    l[7] += l[3];
-    l[6] += T(0.5) * (l[3] - l[5]);
-    l[2] += T(0.5) * (l[5] - l[3]);
-    l[8] += l[4]; l[1] += l[5];
-    l[3] = 0; l[4] = 0; l[5] = 0;
+    g[7] += g[3];
+    g[6] += T(0.5) * (g[3] - g[5]);
+    g[2] += T(0.5) * (g[5] - g[3]);
+    g[8] += g[4]; g[1] += g[5];
+    g[3] = 0; g[4] = 0; g[5] = 0;
   }
+  for (int q = 0; q < 9; ++q) l[q] = g[q];
 }
 
 template <class T>
 __device__ __forceinline__ void zh_pressure(T* f, const int* n, bool inc, T r0) {
+  T g[9];
+  for (int q = 0; q < 9; ++q) g[q] = f[q];
   if (n[0] == 1) {
-    T s = f[0] + f[2] + f[6] + T(2) * (f[1] + f[7] + f[8]);
+    T s = g[0] + g[2] + g[6] + T(2) * (g[1] + g[7] + g[8]);
     T u0 = inc ? s - r0 : T(-1) + s / r0;
     T sc = inc ? T(1) : r0;
-    f[4] = f[8] - T(2) / T(3) * sc * u0;
-    f[3] = f[7] - sc * u0 / T(6) + T(0.5) * (f[6] - f[2]);
-    f[5] = f[1] - sc * u0 / T(6) + T(0.5) * (f[2] - f[6]);
+    g[4] = g[8] - T(2) / T(3) * sc * u0;
+    g[3] = g[7] - sc * u0 / T(6) + T(0.5) * (g[6] - g[2]);
+    g[5] = g[1] - sc * u0 / T(6) + T(0.5) * (g[2] - g[6]);
   } else {
-    T s = f[0] + f[2] + f[6] + T(2) * (f[3] + f[4] + f[5]);
+    T s = g[0] + g[2] + g[6] + T(2) * (g[3] + g[4] + g[5]);
     T u0 = inc ? r0 - s : T(1) - s / r0;
     T sc = inc ? T(1) : r0;
-    f[1] = f[5] + sc * u0 / T(6) + T(0.5) * (f[6] - f[2]);
-    f[7] = f[3] + sc * u0 / T(6) + T(0.5) * (f[2] - f[6]);
-    f[8] = f[4] + T(2) / T(3) * sc * u0;
+    g[1] = g[5] + sc * u0 / T(6) + T(0.5) * (g[6] - g[2]);
+    g[7] = g[3] + sc * u0 / T(6) + T(0.5) * (g[2] - g[6]);
+    g[8] = g[4] + T(2) / T(3) * sc * u0;
   }
+  for (int q = 0; q < 9; ++q) f[q] = g[q];
 }
 
 template <class T>
 __device__ __forceinline__ void zh_pressure_adj(T* l, const int* n) {
+  T g[9];
+  for (int q = 0; q < 9; ++q) g[q] = l[q];
   if (n[0] == 1) {
-    T ub = -(T(2) / T(3)) * l[4] - l[3] / T(6) - l[5] / T(6);
-    l[8] += l[4]; l[7] += l[3]; l[1] += l[5];
-    T d = T(0.5) * (l[3] - l[5]);
-    l[6] += d; l[2] -= d;
-    l[0] += ub; l[2] += ub; l[6] += ub;
-    l[1] += T(2) * ub; l[7] += T(2) * ub; l[8] += T(2) * ub;
-    l[3] = 0; l[4] = 0; l[5] = 0;
+    T ub = -(T(2) / T(3)) * g[4] - g[3] / T(6) - g[5] / T(6);
+    g[8] += g[4]; g[7] += g[3]; g[1] += g[5];
+    T d = T(0.5) * (g[3] - g[5]);
+    g[6] += d; g[2] -= d;
+    g[0] += ub; g[2] += ub; g[6] += ub;
+    g[1] += T(2) * ub; g[7] += T(2) * ub; g[8] += T(2) * ub;
+    g[3] = 0; g[4] = 0; g[5] = 0;
   } else {
-    T sb = -(l[1] / T(6) + l[7] / T(6) + (T(2) / T(3)) * l[8]);
-    l[5] += l[1]; l[3] += l[7]; l[4] += l[8];
-    T d = T(0.5) * (l[1] - l[7]);
-    l[6] += d; l[2] -= d;
-    l[0] += sb; l[2] += sb; l[6] += sb;
-    l[3] += T(2) * sb; l[4] += T(2) * sb; l[5] += T(2) * sb;
-    l[1] = 0; l[7] = 0; l[8] = 0;
+    T sb = -(g[1] / T(6) + g[7] / T(6) + (T(2) / T(3)) * g[8]);
+    g[5] += g[1]; g[3] += g[7]; g[4] += g[8];
+    T d = T(0.5) * (g[1] - g[7]);
+    g[6] += d; g[2] -= d;
+    g[0] += sb; g[2] += sb; g[6] += sb;
+    g[3] += T(2) * sb; g[4] += T(2) * sb; g[5] += T(2) * sb;
+    g[1] = 0; g[7] = 0; g[8] = 0;
   }
+  for (int q = 0; q < 9; ++q) l[q] = g[q];
 }
 
 // D3Q19 Hecht & Harting (src/boundaryConditions/Hecht3d.hh): velocity west, pressure east.
@@ -507,7 +534,9 @@ __device__ __forceinline__ int d2q5_incoming_mask(const int* n) {
 }
 
 template <class L, class T>
-__device__ __forceinline__ void bc_node(T (&f)[L::Q], const DevBC& b, T tf, int x, int y, int z) {
+__device__ __forceinline__ void bc_node(T (&fio)[L::Q], const DevBC& b, T tf, int x, int y, int z) {
+  T f[L::Q];
+  static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = fio[decltype(qc)::value]; });
   switch (b.type) {
     case LBMX_BC_BOUNCEBACK: bc_bounceback<L>(f, b.nrm); break;
     case LBMX_BC_ZOUHE_VELOCITY:
@@ -553,10 +582,13 @@ __device__ __forceinline__ void bc_node(T (&f)[L::Q], const DevBC& b, T tf, int 
       break;
     default: break;
   }
+  static_for<0, L::Q>([&](auto qc) { fio[decltype(qc)::value] = f[decltype(qc)::value]; });
 }
 
 template <class L, class T>
-__device__ __forceinline__ void bc_node_adj(T (&l)[L::Q], const DevBC& b) {
+__device__ __forceinline__ void bc_node_adj(T (&lio)[L::Q], const DevBC& b) {
+  T l[L::Q];
+  static_for<0, L::Q>([&](auto qc) { l[decltype(qc)::value] = lio[decltype(qc)::value]; });
   switch (b.type) {
     case LBMX_BC_BOUNCEBACK: bc_bounceback_adj<L>(l, b.nrm); break;
     case LBMX_BC_ZOUHE_VELOCITY:
@@ -596,6 +628,7 @@ __device__ __forceinline__ void bc_node_adj(T (&l)[L::Q], const DevBC& b) {
       break;
     default: break;
   }
+  static_for<0, L::Q>([&](auto qc) { lio[decltype(qc)::value] = l[decltype(qc)::value]; });
 }
 
 template <class L, int q, class T>

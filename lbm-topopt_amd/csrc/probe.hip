@@ -6,11 +6,12 @@
 
 namespace lbmx {
 
+// one 32-byte element per thread over an exact grid: on MI355X this reaches a higher copy
+// rate than grid-stride loops, non-temporal accesses or several loads per thread
 __global__ __launch_bounds__(256) void k_stream_copy(const double4* __restrict__ in, double4* __restrict__ out,
                                                       long long n4) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n4; i += stride) out[i] = in[i];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n4) out[i] = in[i];
 }
 
 }  // namespace lbmx
@@ -25,7 +26,7 @@ extern "C" lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  const int blocks = 256 * 32;
+  const unsigned blocks = (unsigned)((n4 + 255) / 256);
   hipLaunchKernelGGL(lbmx::k_stream_copy, dim3(blocks), dim3(256), 0, nullptr, (const double4*)a, (double4*)b, n4);
   (void)hipEventRecord(e0, nullptr);
   for (int k = 0; k < iters; ++k)
