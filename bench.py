@@ -96,6 +96,20 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
     }
 
 
+def read_traffic(key):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by the
+    copy kernel) for the 384x192x192 workload, from profiles/traffic_d3q19_mrt_brinkman.txt."""
+    path = os.path.join(ROOT, "profiles", "traffic_d3q19_mrt_brinkman.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            k, _, v = line.strip().partition("=")
+            if k == key:
+                return float(v)
+    return None
+
+
 def cpu_baseline(workloads, lbmx, nx=96, ny=48, nz=48, horizon=30):
     """The CPU oracle (port of the reference's algorithm) on a bounded sample of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -142,11 +156,7 @@ def main():
                           args.sweeps)
     head = res.get("fp64", res[forms[0]])
     head_form = "fp64" if "fp64" in res else forms[0]
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_d3q19_mrt_fwd.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            traffic = json.load(fh).get(head_form)
+    traffic = read_traffic(f"{head_form}_fwd_traffic_bytes")
     roof = {"bound": "hbm", "achieved": round(head["fwd_gbs"], 1), "peak": PEAK_GBS, "unit": "GB/s",
             "frac": round(head["fwd_gbs"] / PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_fwd<D3Q19,MRT,%s> (forward sweep)" % head_form,
