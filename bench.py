@@ -63,16 +63,19 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
         log(f"{form}: warmup step {w} {time.perf_counter() - t0:.2f} s")
     times, fphase, rphase = [], [], []
     J = None
+    recompute = []
     for k in range(steps):
+        before = s.stats()["recompute_sweeps"]
         t0 = time.perf_counter()
         J, sens = s.gradient(horizon, want_sens=(k == steps - 1))
         dt = time.perf_counter() - t0
         st = s.stats()
+        recompute.append(st["recompute_sweeps"] - before)
         times.append(dt)
         fphase.append(st["forward_phase_ms"] / 1e3)
         rphase.append(st["reverse_phase_ms"] / 1e3)
         log(f"{form}: step {k} {dt:.2f} s (forward {fphase[-1]:.2f} s, reverse {rphase[-1]:.2f} s, "
-            f"recompute sweeps {st['recompute_sweeps']})")
+            f"recompute sweeps {recompute[-1]})")
     st = s.stats()
     dev_bytes = s.device_bytes()
     s.close()
@@ -92,6 +95,7 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
         "fwd_bytes_per_lu": bf,
         "adj_bytes_per_lu": ba,
         "objective": J,
+        "recompute_sweeps_per_gradient": recompute[-1],
         "device_gb": dev_bytes / 1e9,
     }
 

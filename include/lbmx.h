@@ -221,10 +221,11 @@ typedef struct {
   double adj_kernel_ms;       /* device time of adjoint sweeps in the last call          */
   double forward_phase_ms;    /* wall time of the recorded forward phase                 */
   double reverse_phase_ms;    /* wall time of the reverse phase incl. recomputation      */
+  /* counters below are cumulative since lbmx_create (difference two reads for one call) */
   long long fwd_sweeps;       /* forward sweeps launched (incl. recomputation)           */
-  long long adj_sweeps;
-  long long checkpoint_copies;
-  long long recompute_sweeps;
+  long long adj_sweeps;       /* adjoint sweeps launched                                 */
+  long long checkpoint_copies;/* state copies made for checkpoints (0: slots are swapped) */
+  long long recompute_sweeps; /* forward sweeps re-run in reverse phases                 */
 } lbmx_stats;
 lbmx_status lbmx_get_stats(lbmx_solver* s, lbmx_stats* st);
 /* Times nsweeps back-to-back sweeps with HIP events on the handle's stream. kind 0: forward

@@ -116,6 +116,7 @@ struct Engine {
       case LBMX_BGK: case LBMX_MRT: ok = true; break;
       case LBMX_BGK_STANDARD: ok = d.lattice != LBMX_D2Q9_D2Q5; break;
       case LBMX_MRT_STANDARD: ok = d.lattice == LBMX_D3Q19; break;
+      case LBMX_CASCADED: ok = d.lattice == LBMX_D2Q9 && d.design != LBMX_DESIGN_BRINKMAN; break;
       default: ok = false;
     }
     if (!ok) throw Error{LBMX_ERR_SUP, "collision operator not available for this lattice"};

@@ -273,9 +273,194 @@ __device__ __forceinline__ void eq_velocity(T rho, const T (&j)[3], T alpha, T (
   }
 }
 
-// Full node collision in place; rho, j are the pre-collision macros.
+// ---- cascaded (D2Q9) -------------------------------------------------------------------
+// IncompressibleCascaded2d<D2Q9> (src/dynamics/cascaded2d.hh:40-95): k = K(f, rho, u) with the
+// incompressible macros (u = j), then f' = f + T k (momentToDistribution). The shear moments
+// k1, k2 relax with omega, the rest with rate 1.
+template <class T>
+__device__ __forceinline__ void casc_k(const T (&f)[9], T r, T x, T y, T om, T (&k)[6]) {
+  const T u2 = x * x + y * y;
+  k[0] = (r * u2 - f[8] - f[6] - f[2] - f[4] - T(2) * (f[1] + f[3] + f[7] + f[5] - r / T(3))) / T(12);
+  k[1] = om * (f[6] + f[2] - f[8] - f[4] + r * (x * x - y * y)) / T(4);
+  k[2] = om * ((f[7] + f[3] - f[5] - f[1]) - r * x * y) / T(4);
+  k[3] = (-((f[1] + f[3] - f[7] - f[5] - T(2) * r * x * x * y + y * (r - f[6] - f[2] - f[0])) / T(4) +
+            T(0.5) * x * (f[7] - f[5] - f[1] + f[3]))) +
+         T(0.5) * y * (-T(3) * k[0] - k[1]) + T(2) * x * k[2];
+  k[4] = (-((f[3] + f[5] - f[1] - f[7] - T(2) * r * y * y * x + x * (r - f[4] - f[8] - f[0])) / T(4) +
+            T(0.5) * y * (f[7] + f[3] - f[1] - f[5]))) +
+         T(0.5) * x * (-T(3) * k[0] + k[1]) + T(2) * y * k[2];
+  k[5] = (T(0.25) * (r / T(9) - f[7] - f[5] - f[1] - f[3] +
+                     T(2) * (x * (f[7] - f[5] + f[1] - f[3]) + y * (f[7] + f[5] - f[1] - f[3])) +
+                     T(4) * x * y * (f[5] - f[7] + f[1] - f[3]) -
+                     x * x * (f[6] + f[7] + f[5] + f[2] + f[1] + f[3]) +
+                     y * y * (T(3) * r * x * x - f[8] - f[7] - f[5] - f[1] - f[3] - f[4]))) -
+         T(2) * k[0] - T(2) * x * k[4] - T(2) * y * k[3] + T(4) * x * y * k[2] - T(1.5) * k[0] * u2 +
+         T(0.5) * k[1] * (x * x - y * y);
+}
+
+// g = f + T k
+template <class T>
+__device__ __forceinline__ void casc_apply(const T (&f)[9], const T (&k)[6], T (&g)[9]) {
+  g[0] = f[0] - T(4) * k[0] + T(4) * k[5];
+  g[1] = f[1] + T(2) * k[0] + k[2] + k[3] - k[4] + k[5];
+  g[2] = f[2] - k[0] - k[1] - T(2) * k[3] - T(2) * k[5];
+  g[3] = f[3] + T(2) * k[0] - k[2] + k[3] + k[4] + k[5];
+  g[4] = f[4] - k[0] + k[1] - T(2) * k[4] - T(2) * k[5];
+  g[5] = f[5] + T(2) * k[0] + k[2] - k[3] + k[4] + k[5];
+  g[6] = f[6] - k[0] - k[1] + T(2) * k[3] - T(2) * k[5];
+  g[7] = f[7] + T(2) * k[0] - k[2] - k[3] - k[4] + k[5];
+  g[8] = f[8] - k[0] + k[1] + T(2) * k[4] - T(2) * k[5];
+}
+
+// lam <- lam + (dk/df)^T T^T lam for the explicit f terms (reverse-mode derivative of casc_k);
+// the parts through rho and u are added to rb, xb, yb.
+template <class T>
+__device__ __forceinline__ void casc_reverse(const T (&f)[9], T r, T x, T y, T om, T (&lam)[9], T& rb, T& xb,
+                                             T& yb) {
+  T k[6];
+  casc_k<T>(f, r, x, y, om, k);
+  const T* l = lam;
+  T kb0 = -T(4) * l[0] + T(2) * (l[1] + l[3] + l[5] + l[7]) - (l[2] + l[4] + l[6] + l[8]);
+  T kb1 = -l[2] + l[4] - l[6] + l[8];
+  T kb2 = l[1] - l[3] + l[5] - l[7];
+  T kb3 = l[1] - T(2) * l[2] + l[3] - l[5] + T(2) * l[6] - l[7];
+  T kb4 = -l[1] + l[3] - T(2) * l[4] + l[5] - l[7] + T(2) * l[8];
+  const T kb5 = T(4) * l[0] + (l[1] + l[3] + l[5] + l[7]) - T(2) * (l[2] + l[4] + l[6] + l[8]);
+  T fb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  T r_b = 0, x_b = 0, y_b = 0;
+  const T u2 = x * x + y * y;
+  {  // k5
+    const T b = kb5, Sb = T(0.25) * b;
+    const T A5 = f[7] - f[5] + f[1] - f[3], B5 = f[7] + f[5] - f[1] - f[3], C5 = f[5] - f[7] + f[1] - f[3];
+    const T E5 = f[6] + f[7] + f[5] + f[2] + f[1] + f[3], F5 = f[8] + f[7] + f[5] + f[1] + f[3] + f[4];
+    r_b += Sb * (T(1) / T(9) + T(3) * x * x * y * y);
+    x_b += Sb * (T(2) * A5 + T(4) * y * C5 - T(2) * x * E5 + T(6) * r * x * y * y);
+    y_b += Sb * (T(2) * B5 + T(4) * x * C5 + T(2) * y * (T(3) * r * x * x - F5));
+    const T Ab = Sb * T(2) * x, Bb = Sb * T(2) * y, Cb = Sb * T(4) * x * y, Eb = -Sb * x * x, Fb = -Sb * y * y;
+    fb[7] += -Sb + Ab + Bb - Cb + Eb + Fb;
+    fb[5] += -Sb - Ab + Bb + Cb + Eb + Fb;
+    fb[1] += -Sb + Ab - Bb + Cb + Eb + Fb;
+    fb[3] += -Sb - Ab - Bb - Cb + Eb + Fb;
+    fb[6] += Eb; fb[2] += Eb; fb[8] += Fb; fb[4] += Fb;
+    kb0 += b * (-T(2) - T(1.5) * u2);
+    kb4 += b * (-T(2) * x);
+    kb3 += b * (-T(2) * y);
+    kb2 += b * (T(4) * x * y);
+    kb1 += b * (T(0.5) * (x * x - y * y));
+    x_b += b * (-T(2) * k[4] + T(4) * y * k[2] - T(3) * k[0] * x + k[1] * x);
+    y_b += b * (-T(2) * k[3] + T(4) * x * k[2] - T(3) * k[0] * y - k[1] * y);
+  }
+  {  // k4
+    const T b = kb4, Pb = -T(0.25) * b, Qb = -T(0.5) * y * b;
+    const T Q4 = f[7] + f[3] - f[1] - f[5];
+    fb[3] += Pb; fb[5] += Pb; fb[1] -= Pb; fb[7] -= Pb;
+    r_b += Pb * (-T(2) * y * y * x + x);
+    y_b += Pb * (-T(4) * r * x * y);
+    x_b += Pb * (-T(2) * r * y * y + (r - f[4] - f[8] - f[0]));
+    fb[4] -= Pb * x; fb[8] -= Pb * x; fb[0] -= Pb * x;
+    fb[7] += Qb; fb[3] += Qb; fb[1] -= Qb; fb[5] -= Qb;
+    y_b += b * (-T(0.5) * Q4 + T(2) * k[2]);
+    x_b += b * (T(0.5) * (-T(3) * k[0] + k[1]));
+    kb0 += b * (-T(1.5) * x);
+    kb1 += b * (T(0.5) * x);
+    kb2 += b * (T(2) * y);
+  }
+  {  // k3
+    const T b = kb3, Pb = -T(0.25) * b, Qb = -T(0.5) * x * b;
+    const T Q3 = f[7] - f[5] - f[1] + f[3];
+    fb[1] += Pb; fb[3] += Pb; fb[7] -= Pb; fb[5] -= Pb;
+    r_b += Pb * (-T(2) * x * x * y + y);
+    x_b += Pb * (-T(4) * r * x * y);
+    y_b += Pb * (-T(2) * r * x * x + (r - f[6] - f[2] - f[0]));
+    fb[6] -= Pb * y; fb[2] -= Pb * y; fb[0] -= Pb * y;
+    fb[7] += Qb; fb[5] -= Qb; fb[1] -= Qb; fb[3] += Qb;
+    x_b += b * (-T(0.5) * Q3 + T(2) * k[2]);
+    y_b += b * (T(0.5) * (-T(3) * k[0] - k[1]));
+    kb0 += b * (-T(1.5) * y);
+    kb1 += b * (-T(0.5) * y);
+    kb2 += b * (T(2) * x);
+  }
+  {  // k2
+    const T b = T(0.25) * om * kb2;
+    fb[7] += b; fb[3] += b; fb[5] -= b; fb[1] -= b;
+    r_b -= b * x * y; x_b -= b * r * y; y_b -= b * r * x;
+  }
+  {  // k1
+    const T b = T(0.25) * om * kb1;
+    fb[6] += b; fb[2] += b; fb[8] -= b; fb[4] -= b;
+    r_b += b * (x * x - y * y); x_b += b * T(2) * r * x; y_b -= b * T(2) * r * y;
+  }
+  {  // k0
+    const T b = kb0 / T(12);
+    fb[8] -= b; fb[6] -= b; fb[2] -= b; fb[4] -= b;
+    fb[1] -= T(2) * b; fb[3] -= T(2) * b; fb[7] -= T(2) * b; fb[5] -= T(2) * b;
+    r_b += b * (u2 + T(2) / T(3)); x_b += b * T(2) * r * x; y_b += b * T(2) * r * y;
+  }
+  static_for<0, 9>([&](auto qc) { lam[decltype(qc)::value] += fb[decltype(qc)::value]; });
+  rb += r_b;
+  xb += x_b;
+  yb += y_b;
+}
+
+template <class L, int DES, class T>
+__device__ __forceinline__ void collide_cascaded(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
+  static_assert(L::Q == 9 && L::QF == 9, "the cascaded operator is defined for D2Q9");
+  flow_macros<L>(f, rho, j);
+  T k[6];
+  casc_k<T>(f, rho, j[0], j[1], T(mp.omega), k);
+  T g[9];
+  casc_apply<T>(f, k, g);
+  if constexpr (DES == D_PBB) {
+    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T sc = T(1) - T(2) * imult;
+    static_for<1, 5>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      const T sd = g[d] + g[d + 4], ad = (g[d] - g[d + 4]) * sc;
+      g[d] = T(0.5) * (sd + ad);
+      g[d + 4] = T(0.5) * (sd - ad);
+    });
+  }
+  static_for<0, 9>([&](auto qc) { f[decltype(qc)::value] = g[decltype(qc)::value]; });
+}
+
+template <class L, int DES, class T>
+__device__ __forceinline__ void reverse_cascaded(const T (&f)[L::Q], T gamma, const ModelParams& mp, T (&lam)[L::Q],
+                                                 T& dgam, T srcA, const T (&srcB)[3]) {
+  static_assert(L::Q == 9 && L::QF == 9, "the cascaded operator is defined for D2Q9");
+  T rho, j[3];
+  flow_macros<L>(f, rho, j);
+  const T om = T(mp.omega);
+  if constexpr (DES == D_PBB) {
+    T k[6], g[9];
+    casc_k<T>(f, rho, j[0], j[1], om, k);
+    casc_apply<T>(f, k, g);
+    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T sc = T(1) - T(2) * imult;
+    T mbar = 0;
+    static_for<1, 5>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      const T ls = lam[d] + lam[d + 4], la = lam[d] - lam[d + 4];
+      mbar -= la * (g[d] - g[d + 4]);
+      const T las = la * sc;
+      lam[d] = T(0.5) * (ls + las);
+      lam[d + 4] = T(0.5) * (ls - las);
+    });
+    dgam += mbar * T(0.5) * T(mp.alpha_max) * bp_der<T>(T(mp.interp_q), gamma);
+  }
+  T rb = srcA, xb = srcB[0], yb = srcB[1];
+  casc_reverse<T>(f, rho, j[0], j[1], om, lam, rb, xb, yb);
+  static_for<0, 9>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    T v = lam[q] + rb;
+    if constexpr (L::ex[q] != 0) v += T(L::ex[q]) * xb;
+    if constexpr (L::ey[q] != 0) v += T(L::ey[q]) * yb;
+    lam[q] = v;
+  });
+}
+
+// Moment-space operators (BGK, MRT): node collision in place; rho, j are the pre-collision macros.
 template <class L, int COLL, int DES, class T>
-__device__ __forceinline__ void collide(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
+__device__ __forceinline__ void collide_moment(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
   Pairs<L, T> p;
   to_pairs<L>(f, p);
   pair_macros<L>(p, rho, j);
@@ -303,8 +488,8 @@ __device__ __forceinline__ void collide(T (&f)[L::Q], T gamma, const ModelParams
 // (src/adjointDynamics/reversePartialBounceback.hh:18-41, reverseIncompressibleBGK2d.hh:64-121,
 // reverseIncompressibleMRT2d.hh:29-83, reverseMRT3d.hh:29-365).
 template <class L, int COLL, int DES, class T>
-__device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, const ModelParams& mp, T (&lam)[L::Q],
-                                                T& dgam, T srcA, const T (&srcB)[3], T srcAT) {
+__device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, const ModelParams& mp, T (&lam)[L::Q],
+                                               T& dgam, T srcA, const T (&srcB)[3], T srcAT) {
   static_assert(CollTraits<COLL>::inc, "reverse operators exist for the incompressible operators");
   constexpr int Q = L::QF, H = L::HALF;
   Pairs<L, T> pf;
@@ -505,5 +690,21 @@ __device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, con
 }
 
 #undef LBMX_BAS
+
+// ---- entry points -----------------------------------------------------------------------
+// Node collision in place; rho, j receive the pre-collision macros.
+template <class L, int COLL, int DES, class T>
+__device__ __forceinline__ void collide(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
+  if constexpr (COLL == C_CASC) collide_cascaded<L, DES>(f, gamma, mp, rho, j);
+  else collide_moment<L, COLL, DES>(f, gamma, mp, rho, j);
+}
+
+// lam <- (dC/df)^T lam + functional source; dgam += (dC/dgamma)^T lam.
+template <class L, int COLL, int DES, class T>
+__device__ __forceinline__ void reverse_collide(const T (&f)[L::Q], T gamma, const ModelParams& mp, T (&lam)[L::Q],
+                                                T& dgam, T srcA, const T (&srcB)[3], T srcAT) {
+  if constexpr (COLL == C_CASC) reverse_cascaded<L, DES>(f, gamma, mp, lam, dgam, srcA, srcB);
+  else reverse_moment<L, COLL, DES>(f, gamma, mp, lam, dgam, srcA, srcB, srcAT);
+}
 
 }  // namespace lbmx

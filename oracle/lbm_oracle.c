@@ -21,7 +21,10 @@
  *   reverse operators,
  *   adjoint BCs, stream    : dot-product tests against central differences of the forward
  *   functionals + gradient : central finite differences of J on random cells
- *   cascaded collision     : parity unpinned beyond conservation/rest state (see DESIGN)
+ *   cascaded collision     : conservation; shear moments relax with omega to rho u_a u_b for
+ *                            any state; at u = 0 every moment reaches its equilibrium (rate 1);
+ *                            the u-dependent terms of k0, k3, k4, k5 as coded: parity unpinned;
+ *                            its reverse: dot-product test (true transpose)
  */
 #include "lbm_oracle.h"
 
@@ -233,7 +236,10 @@ static double ramp_diff(double p, double ck, double s) {
 orc_problem* orc_create(const lbmx_desc* d) {
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return NULL;
   /* operators restated so far */
-  if (d->collision == LBMX_CASCADED || d->design == LBMX_DESIGN_DARCY) return NULL;
+  if (d->design == LBMX_DESIGN_DARCY) return NULL;
+  /* cascaded: D2Q9 only, coupled to the design through partial bounce-back (or not at all) */
+  if (d->collision == LBMX_CASCADED &&
+      (d->lattice != LBMX_D2Q9 || d->design == LBMX_DESIGN_BRINKMAN)) return NULL;
   if (d->collision == LBMX_MRT_STANDARD && d->lattice != LBMX_D3Q19) return NULL;
   if (d->lattice != LBMX_D3Q19 && d->nz != 1) return NULL;
   orc_problem* p = (orc_problem*)calloc(1, sizeof(orc_problem));
@@ -483,11 +489,151 @@ static double design_alpha(const orc_problem* p, double gamma) {
 
 /* base collision of the flow part: BGK / MRT (incompressible or standard), with the
    equilibrium velocity v and an optional Guo source (Brinkman). out may alias nothing. */
+/* IncompressibleCascaded2d<D2Q9> (src/dynamics/cascaded2d.hh:40-95): relaxation written as
+   k = K(f, rho, u), f' = f + T k (momentToDistribution, :77-95), with the incompressible
+   macros u = j (src/dynamics/isothermalMacros2d.hh:44-67). Shear moments k1, k2 relax with
+   omega, the others with rate 1. Directions follow D2Q9_EX/EY above. */
+static const double CASC_T[9][6] = {
+    {-4, 0, 0, 0, 0, 4},  {2, 0, 1, 1, -1, 1},  {-1, -1, 0, -2, 0, -2},
+    {2, 0, -1, 1, 1, 1},  {-1, 1, 0, 0, -2, -2}, {2, 0, 1, -1, 1, 1},
+    {-1, -1, 0, 2, 0, -2}, {2, 0, -1, -1, -1, 1}, {-1, 1, 0, 0, 2, -2}};
+
+/* k0..k5 as cascaded2d.hh:44-74 computes them */
+static void casc_k(const double* f, double r, double x, double y, double om, double* k) {
+  double u2 = x * x + y * y;
+  k[0] = (r * u2 - f[8] - f[6] - f[2] - f[4] - 2. * (f[1] + f[3] + f[7] + f[5] - r / 3.)) / 12.;
+  k[1] = om * (f[6] + f[2] - f[8] - f[4] + r * (x * x - y * y)) / 4.;
+  k[2] = om * ((f[7] + f[3] - f[5] - f[1]) - r * x * y) / 4.;
+  k[3] = (-((f[1] + f[3] - f[7] - f[5] - 2. * r * x * x * y + y * (r - f[6] - f[2] - f[0])) / 4. +
+            0.5 * x * (f[7] - f[5] - f[1] + f[3]))) +
+         0.5 * y * (-3. * k[0] - k[1]) + 2. * x * k[2];
+  k[4] = (-((f[3] + f[5] - f[1] - f[7] - 2. * r * y * y * x + x * (r - f[4] - f[8] - f[0])) / 4. +
+            0.5 * y * (f[7] + f[3] - f[1] - f[5]))) +
+         0.5 * x * (-3. * k[0] + k[1]) + 2. * y * k[2];
+  k[5] = (0.25 * (r / 9. - f[7] - f[5] - f[1] - f[3] +
+                  2. * (x * (f[7] - f[5] + f[1] - f[3]) + y * (f[7] + f[5] - f[1] - f[3])) +
+                  4. * x * y * (f[5] - f[7] + f[1] - f[3]) -
+                  x * x * (f[6] + f[7] + f[5] + f[2] + f[1] + f[3]) +
+                  y * y * (3. * r * x * x - f[8] - f[7] - f[5] - f[1] - f[3] - f[4]))) -
+         2. * k[0] - 2. * x * k[4] - 2. * y * k[3] + 4. * x * y * k[2] - 1.5 * k[0] * u2 +
+         0.5 * k[1] * (x * x - y * y);
+}
+
+/* Transpose of the cascaded Jacobian (reverse-mode derivative of casc_k, derived by hand;
+   the reference's ReverseIncompressibleCascaded2d, src/adjointDynamics/reverseCascaded2d.hh,
+   is not this transpose): lf = lg + (dk/df)^T T^T lg for the explicit f terms; the parts
+   through rho and u are returned in *rb, vb[0..1]. */
+static void casc_reverse(const double* f, double r, double x, double y, double om, const double* lg,
+                         double* lf, double* rb, double* vb) {
+  double k[6];
+  casc_k(f, r, x, y, om, k);
+  double kb[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 6; ++i)
+    for (int q = 0; q < 9; ++q) kb[i] += CASC_T[q][i] * lg[q];
+  double fb[9];
+  for (int q = 0; q < 9; ++q) fb[q] = 0.;
+  double r_b = 0, x_b = 0, y_b = 0;
+  const double u2 = x * x + y * y;
+  /* k5 */
+  {
+    const double b = kb[5];
+    const double A5 = f[7] - f[5] + f[1] - f[3], B5 = f[7] + f[5] - f[1] - f[3];
+    const double C5 = f[5] - f[7] + f[1] - f[3];
+    const double E5 = f[6] + f[7] + f[5] + f[2] + f[1] + f[3];
+    const double F5 = f[8] + f[7] + f[5] + f[1] + f[3] + f[4];
+    const double Sb = 0.25 * b;
+    r_b += Sb * (1. / 9. + 3. * x * x * y * y);
+    x_b += Sb * (2. * A5 + 4. * y * C5 - 2. * x * E5 + 6. * r * x * y * y);
+    y_b += Sb * (2. * B5 + 4. * x * C5 + 2. * y * (3. * r * x * x - F5));
+    const double Ab = Sb * 2. * x, Bb = Sb * 2. * y, Cb = Sb * 4. * x * y, Eb = -Sb * x * x, Fb = -Sb * y * y;
+    fb[7] += -Sb + Ab + Bb - Cb + Eb + Fb;
+    fb[5] += -Sb - Ab + Bb + Cb + Eb + Fb;
+    fb[1] += -Sb + Ab - Bb + Cb + Eb + Fb;
+    fb[3] += -Sb - Ab - Bb - Cb + Eb + Fb;
+    fb[6] += Eb; fb[2] += Eb;
+    fb[8] += Fb; fb[4] += Fb;
+    kb[0] += b * (-2. - 1.5 * u2);
+    kb[4] += b * (-2. * x);
+    kb[3] += b * (-2. * y);
+    kb[2] += b * (4. * x * y);
+    kb[1] += b * (0.5 * (x * x - y * y));
+    x_b += b * (-2. * k[4] + 4. * y * k[2] - 3. * k[0] * x + k[1] * x);
+    y_b += b * (-2. * k[3] + 4. * x * k[2] - 3. * k[0] * y - k[1] * y);
+  }
+  /* k4 = -(P4/4 + y Q4/2) + x(-3 k0 + k1)/2 + 2 y k2 */
+  {
+    const double b = kb[4];
+    const double Q4 = f[7] + f[3] - f[1] - f[5];
+    const double Pb = -0.25 * b, Qb = -0.5 * y * b;
+    fb[3] += Pb; fb[5] += Pb; fb[1] -= Pb; fb[7] -= Pb;
+    r_b += Pb * (-2. * y * y * x + x);
+    y_b += Pb * (-4. * r * x * y);
+    x_b += Pb * (-2. * r * y * y + (r - f[4] - f[8] - f[0]));
+    fb[4] -= Pb * x; fb[8] -= Pb * x; fb[0] -= Pb * x;
+    fb[7] += Qb; fb[3] += Qb; fb[1] -= Qb; fb[5] -= Qb;
+    y_b += b * (-0.5 * Q4 + 2. * k[2]);
+    x_b += b * (0.5 * (-3. * k[0] + k[1]));
+    kb[0] += b * (-1.5 * x);
+    kb[1] += b * (0.5 * x);
+    kb[2] += b * (2. * y);
+  }
+  /* k3 = -(P3/4 + x Q3/2) + y(-3 k0 - k1)/2 + 2 x k2 */
+  {
+    const double b = kb[3];
+    const double Q3 = f[7] - f[5] - f[1] + f[3];
+    const double Pb = -0.25 * b, Qb = -0.5 * x * b;
+    fb[1] += Pb; fb[3] += Pb; fb[7] -= Pb; fb[5] -= Pb;
+    r_b += Pb * (-2. * x * x * y + y);
+    x_b += Pb * (-4. * r * x * y);
+    y_b += Pb * (-2. * r * x * x + (r - f[6] - f[2] - f[0]));
+    fb[6] -= Pb * y; fb[2] -= Pb * y; fb[0] -= Pb * y;
+    fb[7] += Qb; fb[5] -= Qb; fb[1] -= Qb; fb[3] += Qb;
+    x_b += b * (-0.5 * Q3 + 2. * k[2]);
+    y_b += b * (0.5 * (-3. * k[0] - k[1]));
+    kb[0] += b * (-1.5 * y);
+    kb[1] += b * (-0.5 * y);
+    kb[2] += b * (2. * x);
+  }
+  /* k2 */
+  {
+    const double b = 0.25 * om * kb[2];
+    fb[7] += b; fb[3] += b; fb[5] -= b; fb[1] -= b;
+    r_b -= b * x * y; x_b -= b * r * y; y_b -= b * r * x;
+  }
+  /* k1 */
+  {
+    const double b = 0.25 * om * kb[1];
+    fb[6] += b; fb[2] += b; fb[8] -= b; fb[4] -= b;
+    r_b += b * (x * x - y * y); x_b += b * 2. * r * x; y_b -= b * 2. * r * y;
+  }
+  /* k0 */
+  {
+    const double b = kb[0] / 12.;
+    fb[8] -= b; fb[6] -= b; fb[2] -= b; fb[4] -= b;
+    fb[1] -= 2. * b; fb[3] -= 2. * b; fb[7] -= 2. * b; fb[5] -= 2. * b;
+    r_b += b * (u2 + 2. / 3.); x_b += b * 2. * r * x; y_b += b * 2. * r * y;
+  }
+  for (int q = 0; q < 9; ++q) lf[q] = lg[q] + fb[q];
+  *rb += r_b;
+  vb[0] += x_b;
+  vb[1] += y_b;
+}
+
 static void base_collide(const orc_problem* p, const double* f, double* out, double rho,
                          const double* v, double alpha, int forced) {
   const int Q = p->Qf;
   const double om = p->d.omega;
   double rho_s = p->incompressible ? 1. : rho;
+  if (p->d.collision == LBMX_CASCADED) {
+    double k[6];
+    casc_k(f, rho, v[0], v[1], om, k);
+    for (int q = 0; q < 9; ++q) {
+      double acc = f[q];
+      for (int i = 0; i < 6; ++i) acc += CASC_T[q][i] * k[i];
+      out[q] = acc;
+    }
+    return;
+  }
   if (p->d.collision == LBMX_BGK || p->d.collision == LBMX_BGK_STANDARD) {
     /* IncompressibleBGK2d (src/dynamics/incompressibleBGK2d.hh:38-47) */
     for (int q = 0; q < Q; ++q) {
@@ -647,7 +793,9 @@ void orc_reverse_collide_node(const orc_problem* p, const double* fin, double ga
 
   double rho_bar = 0, v_bar[3] = {0, 0, 0}, alpha_bar = 0;
   double lf[QMAX];
-  if (p->d.collision == LBMX_BGK) {
+  if (p->d.collision == LBMX_CASCADED) {
+    casc_reverse(fin, rho, v[0], v[1], om, lg, lf, &rho_bar, v_bar);
+  } else if (p->d.collision == LBMX_BGK) {
     for (int q = 0; q < Q; ++q) {
       double cq[3] = {p->ex[q], p->ey[q], p->ez[q]};
       double cu = cq[0] * v[0] + cq[1] * v[1] + cq[2] * v[2];

@@ -9,7 +9,7 @@ Each case: the problem (lattice, operator, coupling, grid, setup, steps, seed) a
 outputs J (objective summed over the steps), sens (dJ/dgamma, natural node order), and the
 state after the forward steps (array-of-structures [node][q], FP64).
 
-Run:  python tests/golden/make_golden.py
+Run:  python tests/golden/make_golden.py [case ...]
 """
 import json
 import os
@@ -35,6 +35,10 @@ CASES = {
                                       alpha_max=1.0),
     "d2q9_bgk_brinkman_channel": dict(lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_BRINKMAN, nx=32,
                                       ny=16, nz=1, setup="channel", steps=40, seed=0, omega=1.25, alpha_max=0.1),
+    # config-4 operator: PBB<IncompressibleCascaded2d, BP(q=0.1)> at tau = 0.6
+    "d2q9_cascaded_pbb_channel": dict(lattice=lbmx.D2Q9, collision=lbmx.CASCADED, design=lbmx.DESIGN_PBB, nx=32,
+                                      ny=16, nz=1, setup="channel", steps=40, seed=2, omega=1.0 / 0.6,
+                                      alpha_max=1.0),
     "d3q19_mrt_brinkman_duct": dict(lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN, nx=14,
                                     ny=10, nz=8, setup="duct", steps=20, seed=0, omega=1.6, alpha_max=0.1),
     "d2q9_d2q5_mrt_brinkman_thermal": dict(lattice=lbmx.D2Q9_D2Q5, collision=lbmx.MRT,
@@ -65,8 +69,10 @@ def init_kwargs(c):
     return dict(u0=(0.03, 0.0, 0.0), T0=0.5) if c["setup"] == "thermal" else {}
 
 
-def main():
+def main(names=None):
     for name, c in CASES.items():
+        if names and name not in names:
+            continue
         o = make_problem(oracle.Oracle, c)
         kw = init_kwargs(c)
         J, sens, _ = o.gradient(c["steps"], **kw)
@@ -79,4 +85,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -17,7 +17,7 @@ GRAD_TOL = 1e-10
 
 CASES_2D = [(lbmx.BGK, lbmx.DESIGN_PBB), (lbmx.MRT, lbmx.DESIGN_PBB),
             (lbmx.BGK, lbmx.DESIGN_BRINKMAN), (lbmx.MRT, lbmx.DESIGN_BRINKMAN),
-            (lbmx.BGK, lbmx.DESIGN_NONE)]
+            (lbmx.BGK, lbmx.DESIGN_NONE), (lbmx.CASCADED, lbmx.DESIGN_PBB), (lbmx.CASCADED, lbmx.DESIGN_NONE)]
 CASES_3D = [(lbmx.MRT, lbmx.DESIGN_PBB), (lbmx.MRT, lbmx.DESIGN_BRINKMAN), (lbmx.BGK, lbmx.DESIGN_BRINKMAN)]
 
 
@@ -47,7 +47,7 @@ def test_forward_standard_mrt3d(lbmx_mod):
     assert relmax(s.get_state(), o.f) < POP_TOL
 
 
-@pytest.mark.parametrize("coll,design", CASES_2D[:4])
+@pytest.mark.parametrize("coll,design", CASES_2D[:4] + [(lbmx.CASCADED, lbmx.DESIGN_PBB)])
 def test_gradient_2d(lbmx_mod, coll, design):
     s, o, _ = build_pair(lbmx.D2Q9, coll, design, 50, 26, time_const=1.0 / 80)
     s.checkpoint_config(0)

@@ -231,6 +231,39 @@ def test_mrt3d_basis_rates_equilibrium(coll):
         assert np.allclose(o.collide_node(fe, 1.0)[0], fe, atol=1e-16)
 
 
+# ================================================================ cascaded (D2Q9)
+def test_cascaded_closed_forms():
+    """IncompressibleCascaded2d: conserves rho and j; the shear moments relax with omega to
+    rho u_a u_b for any state (u = j, incompressible macros); at u = 0 every higher moment
+    reaches its equilibrium with rate 1 (trace 2 rho/3, third order 0, fourth order rho/9).
+    The u-dependent terms of k0, k3, k4, k5 are as coded in src/dynamics/cascaded2d.hh:44-74
+    and not fixed by these checks."""
+    om = 1.0 / 0.6
+    o = mk(lbmx.D2Q9, lbmx.CASCADED, omega=om)
+    c, _, w = lat(o)
+    cx, cy = c[:, 0], c[:, 1]
+    for _ in range(6):
+        f = near_eq(c, w, amp=0.02)
+        rho, j = f.sum(), c.T @ f
+        out, _ = o.collide_node(f, 1.0)
+        assert abs(out.sum() - rho) < 1e-15 and np.allclose(c.T @ out, j, atol=1e-16)
+        pxx, pxy = (cx ** 2 - cy ** 2) @ f, (cx * cy) @ f
+        assert abs((cx ** 2 - cy ** 2) @ out - (pxx - om * (pxx - rho * (j[0] ** 2 - j[1] ** 2)))) < 1e-15
+        assert abs((cx * cy) @ out - (pxy - om * (pxy - rho * j[0] * j[1]))) < 1e-15
+    # u = 0: remove the momentum of a random state with a combination that carries only momentum
+    for _ in range(4):
+        f = near_eq(c, w, amp=0.03)
+        j = c.T @ f
+        f = f - 3 * w * (c @ j)  # sum_q w c_q c_q = I/3 removes j exactly
+        rho = f.sum()
+        assert np.allclose(c.T @ f, 0, atol=1e-16)
+        out, _ = o.collide_node(f, 1.0)
+        assert abs((cx ** 2 + cy ** 2) @ out - 2 * rho / 3) < 1e-15
+        assert abs((cx * cy ** 2) @ out) < 1e-15 and abs((cx ** 2 * cy) @ out) < 1e-15
+        assert abs((cx ** 2 * cy ** 2) @ out - rho / 9) < 1e-15
+        assert abs((cx ** 2 - cy ** 2) @ out - (1 - om) * ((cx ** 2 - cy ** 2) @ f)) < 1e-15
+
+
 # ================================================================ design couplings
 @pytest.mark.parametrize("lattice,coll", [(lbmx.D2Q9, lbmx.BGK), (lbmx.D2Q9, lbmx.MRT), (lbmx.D3Q19, lbmx.MRT)])
 def test_pbb_pair_map(lattice, coll):
@@ -314,7 +347,8 @@ REV_CASES = [(lbmx.D2Q9, lbmx.BGK, lbmx.DESIGN_PBB), (lbmx.D2Q9, lbmx.MRT, lbmx.
              (lbmx.D2Q9, lbmx.BGK, lbmx.DESIGN_BRINKMAN), (lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_BRINKMAN),
              (lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_NONE), (lbmx.D3Q19, lbmx.BGK, lbmx.DESIGN_BRINKMAN),
              (lbmx.D3Q19, lbmx.MRT, lbmx.DESIGN_BRINKMAN), (lbmx.D3Q19, lbmx.MRT, lbmx.DESIGN_PBB),
-             (lbmx.D2Q9_D2Q5, lbmx.MRT, lbmx.DESIGN_BRINKMAN), (lbmx.D2Q9_D2Q5, lbmx.BGK, lbmx.DESIGN_PBB)]
+             (lbmx.D2Q9_D2Q5, lbmx.MRT, lbmx.DESIGN_BRINKMAN), (lbmx.D2Q9_D2Q5, lbmx.BGK, lbmx.DESIGN_PBB),
+             (lbmx.D2Q9, lbmx.CASCADED, lbmx.DESIGN_NONE), (lbmx.D2Q9, lbmx.CASCADED, lbmx.DESIGN_PBB)]
 
 
 @pytest.mark.parametrize("lattice,coll,design", REV_CASES)
@@ -493,6 +527,7 @@ def test_functional_closed_form_and_gradient(kind):
 FD_CASES = [
     (lbmx.D2Q9, lbmx.BGK, lbmx.DESIGN_PBB, (12, 8, 1), "channel"),
     (lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_BRINKMAN, (12, 8, 1), "channel"),
+    (lbmx.D2Q9, lbmx.CASCADED, lbmx.DESIGN_PBB, (12, 8, 1), "channel"),
     (lbmx.D3Q19, lbmx.MRT, lbmx.DESIGN_BRINKMAN, (8, 6, 6), "duct"),
     (lbmx.D2Q9_D2Q5, lbmx.MRT, lbmx.DESIGN_BRINKMAN, (10, 7, 1), "thermal"),
 ]
