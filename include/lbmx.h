@@ -250,6 +250,33 @@ lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, i
    steps run in the reverse phase (recomputation), *max_slots = slots in use at most. */
 lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances, int* max_slots);
 
+/* ---- batched robust-ensemble form ----
+   Several solver handles of one configuration (same lattice, operator, design coupling and
+   precision; any grids, boundary pieces, functionals and designs) advanced in lockstep: every
+   sweep of the design-gradient pass is one kernel launch over all members, and all members
+   share one checkpoint schedule. Replaces the MPISplit communicators over which the reference
+   runs robust realisations and design candidates side by side (src/core/MPISplit.cc:150-234,
+   applications/RobustPump2d/robustPump2d.cc:47,146). Built for D2Q9 incompressible BGK, MRT and
+   cascaded in the FP64 and FP32 forms. Members stay owned by their handles and must outlive
+   the batch; calls on members and on the batch must not overlap. */
+typedef struct lbmx_batch lbmx_batch;
+lbmx_status lbmx_batch_create(lbmx_solver* const* members, int count, lbmx_batch** out);
+lbmx_status lbmx_batch_destroy(lbmx_batch* b);
+/* checkpoint slots of the shared schedule (each slot holds every member's state) */
+lbmx_status lbmx_batch_checkpoint_config(lbmx_batch* b, int nslots);
+/* rows of the 64-wide workgroup tile of the batched sweeps: 1, 2 or 4 (default 4) */
+lbmx_status lbmx_batch_set_tile_rows(lbmx_batch* b, int rows);
+/* Autotuner: times nsweeps batched forward + adjoint sweeps for each tile height, keeps the
+   fastest; *rows = chosen height, ms[3] (may be NULL) = ms per forward+adjoint sweep pair for
+   heights 1, 2, 4. The members' current states are discarded. */
+lbmx_status lbmx_batch_autotune(lbmx_batch* b, int nsweeps, int* rows, double* ms);
+/* lbmx_gradient for every member: J[count] (may be NULL) and sens[count] (array of pointers to
+   per-member dJ/dgamma arrays in natural order; the array or any entry may be NULL). */
+lbmx_status lbmx_batch_gradient(lbmx_batch* b, int nsteps, double rho0, const double u0[3], double T0,
+                                double* J, double* const* sens);
+lbmx_status lbmx_batch_get_stats(lbmx_batch* b, lbmx_stats* st);
+lbmx_status lbmx_batch_device_bytes(lbmx_batch* b, size_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

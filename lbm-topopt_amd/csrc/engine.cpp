@@ -11,6 +11,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "checkpoint.hpp"
@@ -70,6 +71,85 @@ struct DeviceBuffer {
 
 // zero-fill with a kernel-free path that is safe outside graph capture
 static void zero_async(void* p, size_t bytes, hipStream_t s) { HIP_OK(hipMemsetAsync(p, 0, bytes, s)); }
+
+// The sweeps of one design-gradient pass, over opaque state buffers (a single lattice, a batch
+// of lattices in lockstep, or labelled host buffers that check the schedule).
+struct SweepOps {
+  virtual ~SweepOps() = default;
+  virtual void init(void* dst) = 0;                                              // p_0
+  virtual void fwd(const void* src, void* dst, int t, bool with_obj) = 0;       // p_t from p_{t-1}
+  virtual void adj(const void* fsrc, const void* lin, void* lout, int t) = 0;  // nu_{t-1} from nu_t
+};
+
+struct PassResult {
+  long long recompute = 0;  // forward sweeps re-run in the reverse phase
+  int max_slots = 0;        // slots in use at most
+};
+
+// Forward phase storing checkpoints in `slotbuf` (Wang dynamic checkpointing, or every state
+// when naive), then the reverse phase: the adjoint sweep at step t needs p_{t-1}, restored from
+// the nearest checkpoint and re-advanced (LBDynamicCheckpointing, src/topOpt/LBCheckpointing.cc:42-48;
+// WangCheckpointing::getForwardSolution, src/topOpt/WangCheckpointing.cc:36-126). Storing a
+// checkpoint swaps buffer handles (no copy). On return, lin holds the final adjoint nu_0.
+static PassResult checkpointed_pass(SweepOps& ops, int nsteps, bool naive, std::vector<void*>& slotbuf,
+                                    void* work0, void* work1, void*& lin, void*& lout, bool with_obj) {
+  const int slots = (int)slotbuf.size();
+  void* work[2] = {work0, work1};
+  auto other_work = [&](const void* src) -> void*& { return src == work[0] ? work[1] : work[0]; };
+  std::unique_ptr<WangSchedule> sched;
+  if (!naive) sched.reset(new WangSchedule(slots));
+  PassResult r;
+  auto used = [&]() { r.max_slots = std::max(r.max_slots, naive ? slots : slots - sched->free_slots()); };
+  ops.init(work[0]);
+  auto store = [&](int step, void*& holder) -> void* {
+    int s = naive ? step : sched->make_checkpoint(step);
+    std::swap(slotbuf[s], holder);  // the slot now owns the state; holder gets a spare buffer
+    return slotbuf[s];
+  };
+  void* src = store(0, work[0]);
+  used();
+  for (int k = 1; k <= nsteps; ++k) {
+    void*& dst = other_work(src);
+    ops.fwd(src, dst, k, with_obj);
+    if (k < nsteps) src = store(k, dst);
+    else src = dst;
+    used();
+  }
+  for (int step = nsteps; step >= 1; --step) {
+    const int index = step - 1;
+    const void* fsrc = nullptr;
+    if (naive) {
+      fsrc = slotbuf[index];
+    } else {
+      WangSchedule::Plan plan = sched->get_forward_solution(index);
+      if (plan.stored()) {
+        fsrc = slotbuf[plan.from_slot];
+      } else {
+        const void* s2 = slotbuf[plan.from_slot];
+        size_t next = 0;
+        for (int c = plan.from_step + 1; c <= plan.index; ++c) {
+          void*& dst = other_work(s2);
+          ops.fwd(s2, dst, c, false);
+          ++r.recompute;
+          if (next < plan.stores.size() && plan.stores[next].first == c) {
+            int s = plan.stores[next].second;
+            std::swap(slotbuf[s], dst);
+            s2 = slotbuf[s];
+            ++next;
+          } else {
+            s2 = dst;
+          }
+        }
+        fsrc = s2;
+      }
+      used();
+    }
+    ops.adj(fsrc, lin, lout, step);
+    std::swap(lin, lout);
+    if (!naive) sched->release();
+  }
+  return r;
+}
 
 struct Engine {
   lbmx_desc d{};
@@ -467,6 +547,13 @@ struct Engine {
     return a;
   }
 
+  // design, boundary and functional arrays on the device (no state buffers)
+  void prepare_static() {
+    touched = true;
+    upload_static();
+    red_dev.ensure(sizeof(double));
+  }
+
   void prepare() {
     touched = true;
     upload_static();
@@ -586,96 +673,58 @@ struct Engine {
     ensure_pool(2 + slots);
     std::vector<void*> slotbuf(slots);
     for (int s = 0; s < slots; ++s) slotbuf[s] = pool[2 + s]->p;
-    void* work[2] = {pool[0]->p, pool[1]->p};
-    auto other_work = [&](const void* src) -> void*& { return src == work[0] ? work[1] : work[0]; };
 
     lam_a.ensure(lam_bytes());
     lam_b.ensure(lam_bytes());
     sens_dev.ensure((size_t)N * 8);
-
-    auto tf0 = std::chrono::steady_clock::now();
-    long long ncopies = 0, nrecompute = 0;
-    // initial state p_0 = C(f_eq)
     init_rho = rho0;
     for (int k = 0; k < 3; ++k) init_u[k] = u0 ? u0[k] : 0.0;
     init_T = T0;
-    {
-      FwdArgs a = fwd_args(nullptr, work[0], 0, false);
-      HIP_OK(run_init(a, nullptr));
-    }
     const bool with_obj = obj_set && obj.nterm > 0;
-    if (with_obj) zero_acc();
-    std::unique_ptr<WangSchedule> sched;
-    if (!naive) sched.reset(new WangSchedule(slots));
-    // place p_0
-    void* src = work[0];
-    auto store = [&](int step, void*& holder) -> void* {
-      int s = naive ? step : sched->make_checkpoint(step);
-      std::swap(slotbuf[s], holder);  // the slot now owns the state; holder gets a spare buffer
-      return slotbuf[s];
-    };
-    src = store(0, work[0]);
-    for (int k = 1; k <= nsteps; ++k) {
-      void*& dst = other_work(src);
-      FwdArgs a = fwd_args(src, dst, k, with_obj);
-      HIP_OK(run_fwd(a));
-      if (k < nsteps) src = store(k, dst);
-      else src = dst;
-    }
-    stats.fwd_sweeps += nsteps;
-    double Jv = with_obj ? reduce_acc() : 0.0;
-    auto tf1 = std::chrono::steady_clock::now();
-
-    // reverse
-    zero_async(lam_a.p, lam_bytes(), stream);
-    zero_async(sens_dev.p, (size_t)N * 8, stream);
-    void* lin = lam_a.p;
-    void* lout = lam_b.p;
     ObjArgs oa = obj;
     oa.acc = nullptr;
     if (!with_obj) oa.nterm = 0;
-    for (int step = nsteps; step >= 1; --step) {
-      const int index = step - 1;  // the adjoint sweep at `step` needs p_{step-1}
-      const void* fsrc = nullptr;
-      if (naive) {
-        fsrc = slotbuf[index];
-      } else {
-        WangSchedule::Plan plan = sched->get_forward_solution(index);
-        if (plan.stored()) {
-          fsrc = slotbuf[plan.from_slot];
-        } else {
-          const void* s2 = slotbuf[plan.from_slot];
-          size_t next = 0;
-          for (int c = plan.from_step + 1; c <= plan.index; ++c) {
-            void*& dst = other_work(s2);
-            FwdArgs a = fwd_args(s2, dst, c, false);
-            HIP_OK(run_fwd(a));
-            ++nrecompute;
-            if (next < plan.stores.size() && plan.stores[next].first == c) {
-              int s = plan.stores[next].second;
-              std::swap(slotbuf[s], dst);
-              s2 = slotbuf[s];
-              ++next;
-            } else {
-              s2 = dst;
-            }
-          }
-          fsrc = s2;
-        }
+
+    struct Ops : SweepOps {
+      Engine& e;
+      const ObjArgs& oa;
+      bool reverse_started = false;
+      std::chrono::steady_clock::time_point t_fwd_end;
+      Ops(Engine& en, const ObjArgs& o) : e(en), oa(o) {}
+      void init(void* dst) override {
+        FwdArgs a = e.fwd_args(nullptr, dst, 0, false);
+        HIP_OK(e.run_init(a, nullptr));
+        if (e.obj_set && e.obj.nterm > 0) e.zero_acc();
       }
-      AdjArgs a = adj_args(fsrc, lin, lout, step, oa);
-      HIP_OK(run_adj(a));
-      std::swap(lin, lout);
-      if (!naive) sched->release();
-    }
+      void fwd(const void* src, void* dst, int t, bool with) override {
+        FwdArgs a = e.fwd_args(src, dst, t, with);
+        HIP_OK(e.run_fwd(a));
+      }
+      void adj(const void* fsrc, const void* lin, void* lout, int t) override {
+        if (!reverse_started) {
+          reverse_started = true;
+          HIP_OK(hipStreamSynchronize(e.stream));  // end of the forward phase
+          t_fwd_end = std::chrono::steady_clock::now();
+          zero_async(e.lam_a.p, e.lam_bytes(), e.stream);
+          zero_async(e.sens_dev.p, (size_t)e.N * 8, e.stream);
+        }
+        AdjArgs a = e.adj_args(fsrc, lin, lout, t, oa);
+        HIP_OK(e.run_adj(a));
+      }
+    } ops(*this, oa);
+
+    auto tf0 = std::chrono::steady_clock::now();
+    void* lin = lam_a.p;
+    void* lout = lam_b.p;
+    PassResult pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, with_obj);
     HIP_OK(hipStreamSynchronize(stream));
     auto tr1 = std::chrono::steady_clock::now();
+    const double Jv = with_obj ? reduce_acc() : 0.0;
+    stats.fwd_sweeps += nsteps + pr.recompute;
     stats.adj_sweeps += nsteps;
-    stats.recompute_sweeps += nrecompute;
-    stats.fwd_sweeps += nrecompute;
-    stats.checkpoint_copies += ncopies;
-    stats.forward_phase_ms = std::chrono::duration<double, std::milli>(tf1 - tf0).count();
-    stats.reverse_phase_ms = std::chrono::duration<double, std::milli>(tr1 - tf1).count();
+    stats.recompute_sweeps += pr.recompute;
+    stats.forward_phase_ms = std::chrono::duration<double, std::milli>(ops.t_fwd_end - tf0).count();
+    stats.reverse_phase_ms = std::chrono::duration<double, std::milli>(tr1 - ops.t_fwd_end).count();
     if (lin != lam_a.p) std::swap(lam_a.p, lam_b.p);  // final adjoint in lam_a
     grad_valid = true;
     state_kind = -1;  // the forward buffers were reused
@@ -801,6 +850,288 @@ struct Engine {
   }
 };
 
+// ---- batch: several lattices of one configuration advanced in lockstep ----
+// Every sweep of a design-gradient pass is one launch over all members (the realisations and
+// design candidates that the reference spreads over MPISplit communicators,
+// src/core/MPISplit.cc:150-234). All members share the checkpoint schedule, so a checkpoint
+// slot is one batch buffer holding every member's state at its own offset.
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct Batch {
+  std::vector<Engine*> m;  // members (owned by their handles)
+  hipStream_t stream = nullptr;
+  int ty = kTileY;         // rows of the workgroup tile
+  int nslots = 64;
+  int nblocks = 0;
+  int tf_stride = 1;
+  size_t state_total = 0, lam_total = 0;
+  std::vector<BatchInst> inst;
+  std::vector<int> blk_inst;
+  DeviceBuffer inst_dev, blk_dev, tfac_dev, lam_a, lam_b;
+  std::vector<std::unique_ptr<DeviceBuffer>> pool;
+  lbmx_stats stats{};
+
+  explicit Batch(std::vector<Engine*> members) : m(std::move(members)) {
+    if (m.empty()) throw Error{LBMX_ERR_ARG, "empty batch"};
+    const lbmx_desc& d0 = m[0]->d;
+    for (size_t i = 0; i < m.size(); ++i) {
+      const lbmx_desc& d = m[i]->d;
+      if (d.lattice != LBMX_D2Q9) throw Error{LBMX_ERR_SUP, "batched sweeps are built for D2Q9"};
+      if (d.collision != d0.collision || d.design != d0.design || d.precision != d0.precision)
+        throw Error{LBMX_ERR_ARG, "batch members must share operator, design coupling and precision"};
+      for (size_t k = 0; k < i; ++k)
+        if (m[k] == m[i]) throw Error{LBMX_ERR_ARG, "a solver appears twice in the batch"};
+    }
+    if (d0.precision != LBMX_FP32 && d0.precision != LBMX_FP64)
+      throw Error{LBMX_ERR_SUP, "batched sweeps are built for the FP64 and FP32 forms"};
+    if (d0.collision != LBMX_BGK && d0.collision != LBMX_MRT && d0.collision != LBMX_CASCADED)
+      throw Error{LBMX_ERR_SUP, "batched sweeps are built for incompressible BGK, MRT and cascaded"};
+    stream = m[0]->stream;
+  }
+
+  ~Batch() {
+    if (!pool.empty() || lam_a.p) (void)hipStreamSynchronize(stream);
+  }
+
+  Geometry geom_for(const Engine& e) const {
+    Geometry g{};
+    g.nx = e.d.nx; g.ny = e.d.ny; g.nz = e.d.nz;
+    g.nblk_x = (e.d.nx + kTileX - 1) / kTileX;
+    g.nblk_y = (e.d.ny + ty - 1) / ty;
+    g.nblocks = g.nblk_x * g.nblk_y * e.d.nz;
+    g.N = e.N;
+    return g;
+  }
+
+  // instance table, workgroup map and member device arrays for the current tile height
+  void build() {
+    inst.clear();
+    blk_inst.clear();
+    nblocks = 0;
+    tf_stride = 1;
+    size_t st = 0, lt = 0;
+    for (size_t i = 0; i < m.size(); ++i) {
+      Engine& e = *m[i];
+      e.stream = stream;
+      e.prepare_static();
+      const Geometry g = geom_for(e);
+      e.acc_dev.ensure((size_t)std::max(g.nblocks, e.grid_blocks()) * sizeof(double));
+      e.sens_dev.ensure((size_t)e.N * 8);
+      BatchInst I{};
+      I.g = g;
+      I.ls = e.ls;
+      I.ll = e.ll;
+      I.m = e.params();
+      I.obj = e.obj;
+      if (!e.obj_set) I.obj.nterm = 0;
+      I.obj.terms = static_cast<const DevTerm*>(e.term_dev.p);
+      I.obj.acc = I.obj.nterm > 0 ? static_cast<double*>(e.acc_dev.p) : nullptr;
+      I.gamma = e.gamma_dev.p;
+      I.sens = static_cast<double*>(e.sens_dev.p);
+      I.bcs = static_cast<const DevBC*>(e.bc_dev.p);
+      I.nbc = (int)e.bcs.size();
+      I.blk_begin = nblocks;
+      I.state_off = (long long)st;
+      I.lam_off = (long long)lt;
+      st += align256(e.state_bytes());
+      lt += align256(e.lam_bytes());
+      for (int b = 0; b < g.nblocks; ++b) blk_inst.push_back((int)i);
+      nblocks += g.nblocks;
+      tf_stride = std::max(tf_stride, I.nbc);
+      inst.push_back(I);
+    }
+    if (st != state_total) pool.clear();
+    state_total = st;
+    lam_total = lt;
+    inst_dev.ensure(inst.size() * sizeof(BatchInst));
+    HIP_OK(hipMemcpyAsync(inst_dev.p, inst.data(), inst.size() * sizeof(BatchInst), hipMemcpyHostToDevice, stream));
+    blk_dev.ensure(blk_inst.size() * sizeof(int));
+    HIP_OK(hipMemcpyAsync(blk_dev.p, blk_inst.data(), blk_inst.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+  }
+
+  void ensure_pool(size_t count) {
+    while (pool.size() < count) {
+      pool.emplace_back(new DeviceBuffer());
+      pool.back()->ensure(state_total);
+    }
+  }
+
+  // time factors of every member's pieces for t = 0..nsteps
+  void upload_tfac(int nsteps) {
+    const size_t per = (size_t)m.size() * tf_stride;
+    std::vector<double> h((size_t)(nsteps + 1) * per, 1.0);
+    for (int t = 0; t <= nsteps; ++t)
+      for (size_t i = 0; i < m.size(); ++i) m[i]->time_factors(t, &h[t * per + i * tf_stride]);
+    tfac_dev.ensure(h.size() * sizeof(double));
+    HIP_OK(hipMemcpyAsync(tfac_dev.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+  }
+
+  hipError_t launch(int kind, const void* src, void* dst, const void* lin, void* lout, int t, bool with_obj) {
+    BatchStep st{};
+    st.src = src;
+    st.dst = dst;
+    st.lam_in = lin;
+    st.lam_out = lout;
+    st.inst = static_cast<const BatchInst*>(inst_dev.p);
+    st.blk_inst = static_cast<const int*>(blk_dev.p);
+    st.tfac = static_cast<const double*>(tfac_dev.p) + (size_t)t * m.size() * tf_stride;
+    st.tf_stride = tf_stride;
+    st.nblocks = nblocks;
+    st.t = t;
+    st.with_obj = with_obj ? 1 : 0;
+    const lbmx_desc& d = m[0]->d;
+    if (d.precision == LBMX_FP64) return launch_batch_d2q9_f64(d.collision, d.design, ty, kind, st, stream);
+    return launch_batch_d2q9_f32(d.collision, d.design, ty, kind, st, stream);
+  }
+
+  void init_all(void* dst, double rho0, const double* u0, double T0) {
+    for (size_t i = 0; i < m.size(); ++i) {
+      Engine& e = *m[i];
+      e.init_rho = rho0;
+      for (int k = 0; k < 3; ++k) e.init_u[k] = u0 ? u0[k] : 0.0;
+      e.init_T = T0;
+      FwdArgs a = e.fwd_args(nullptr, static_cast<char*>(dst) + inst[i].state_off, 0, false);
+      HIP_OK(e.run_init(a, nullptr));
+    }
+  }
+
+  void zero_accs() {
+    for (size_t i = 0; i < m.size(); ++i)
+      if (inst[i].obj.acc) zero_async(inst[i].obj.acc, (size_t)inst[i].g.nblocks * sizeof(double), stream);
+  }
+
+  double reduce_acc(size_t i) {
+    if (!inst[i].obj.acc) return 0.0;
+    std::vector<double> h(inst[i].g.nblocks);
+    HIP_OK(hipMemcpyAsync(h.data(), inst[i].obj.acc, h.size() * 8, hipMemcpyDeviceToHost, stream));
+    HIP_OK(hipStreamSynchronize(stream));
+    double s = 0;
+    for (double v : h) s += v;  // fixed order: deterministic
+    return s;
+  }
+
+  void gradient(int nsteps, double rho0, const double* u0, double T0, double* J, double* const* sens) {
+    if (nsteps < 1) throw Error{LBMX_ERR_ARG, "nsteps must be >= 1"};
+    build();
+    const bool naive = nslots <= 0;
+    const int slots = naive ? nsteps : std::min(nslots, std::max(2, nsteps));
+    ensure_pool(2 + slots);
+    std::vector<void*> slotbuf(slots);
+    for (int k = 0; k < slots; ++k) slotbuf[k] = pool[2 + k]->p;
+    lam_a.ensure(lam_total);
+    lam_b.ensure(lam_total);
+    upload_tfac(nsteps);
+
+    struct Ops : SweepOps {
+      Batch& b;
+      double rho0, T0;
+      const double* u0;
+      bool reverse_started = false;
+      std::chrono::steady_clock::time_point t_fwd_end;
+      Ops(Batch& bb, double r, const double* u, double T) : b(bb), rho0(r), T0(T), u0(u) {}
+      void init(void* dst) override {
+        b.init_all(dst, rho0, u0, T0);
+        b.zero_accs();
+      }
+      void fwd(const void* src, void* dst, int t, bool with) override {
+        HIP_OK(b.launch(0, src, dst, nullptr, nullptr, t, with));
+      }
+      void adj(const void* fsrc, const void* lin, void* lout, int t) override {
+        if (!reverse_started) {
+          reverse_started = true;
+          HIP_OK(hipStreamSynchronize(b.stream));  // end of the forward phase
+          t_fwd_end = std::chrono::steady_clock::now();
+          zero_async(b.lam_a.p, b.lam_total, b.stream);
+          for (auto& I : b.inst) zero_async(I.sens, (size_t)I.g.N * 8, b.stream);
+        }
+        HIP_OK(b.launch(1, fsrc, nullptr, lin, lout, t, true));
+      }
+    } ops(*this, rho0, u0, T0);
+
+    auto tf0 = std::chrono::steady_clock::now();
+    void* lin = lam_a.p;
+    void* lout = lam_b.p;
+    PassResult pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true);
+    HIP_OK(hipStreamSynchronize(stream));
+    auto tr1 = std::chrono::steady_clock::now();
+    if (lin != lam_a.p) std::swap(lam_a.p, lam_b.p);
+    long long nodes = 0;
+    for (size_t i = 0; i < m.size(); ++i) {
+      if (J) J[i] = reduce_acc(i);
+      if (sens && sens[i]) HIP_OK(hipMemcpy(sens[i], inst[i].sens, (size_t)m[i]->N * 8, hipMemcpyDeviceToHost));
+      m[i]->grad_valid = false;  // the members' own adjoint buffers were not used
+      m[i]->state_kind = -1;
+      nodes += m[i]->N;
+    }
+    (void)nodes;
+    stats.fwd_sweeps += nsteps + pr.recompute;
+    stats.adj_sweeps += nsteps;
+    stats.recompute_sweeps += pr.recompute;
+    stats.forward_phase_ms = std::chrono::duration<double, std::milli>(ops.t_fwd_end - tf0).count();
+    stats.reverse_phase_ms = std::chrono::duration<double, std::milli>(tr1 - ops.t_fwd_end).count();
+  }
+
+  // Times nsweeps batched forward and adjoint sweeps for each tile height and keeps the fastest.
+  int autotune(int nsweeps, double* ms_out) {
+    if (nsweeps < 1) throw Error{LBMX_ERR_ARG, "nsweeps must be >= 1"};
+    const int cand[3] = {1, 2, 4};
+    double best = 1e300;
+    int best_ty = ty;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    for (int c = 0; c < 3; ++c) {
+      ty = cand[c];
+      build();
+      ensure_pool(2);
+      lam_a.ensure(lam_total);
+      lam_b.ensure(lam_total);
+      upload_tfac(1);
+      const double u0[3] = {0, 0, 0};
+      init_all(pool[0]->p, 1.0, u0, 0.0);
+      zero_async(lam_a.p, lam_total, stream);
+      zero_async(lam_b.p, lam_total, stream);
+      for (auto& I : inst) zero_async(I.sens, (size_t)I.g.N * 8, stream);
+      void* a = pool[0]->p;
+      void* b = pool[1]->p;
+      HIP_OK(launch(0, a, b, nullptr, nullptr, 1, false));  // warm-up
+      HIP_OK(hipEventRecord(e0, stream));
+      for (int k = 0; k < nsweeps; ++k) {
+        HIP_OK(launch(0, a, b, nullptr, nullptr, 1, false));
+        std::swap(a, b);
+      }
+      void* li = lam_a.p;
+      void* lo = lam_b.p;
+      for (int k = 0; k < nsweeps; ++k) {
+        HIP_OK(launch(1, a, nullptr, li, lo, 1, true));
+        std::swap(li, lo);
+      }
+      HIP_OK(hipEventRecord(e1, stream));
+      HIP_OK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms_out) ms_out[c] = ms / nsweeps;
+      if (ms < best) {
+        best = ms;
+        best_ty = cand[c];
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ty = best_ty;
+    for (Engine* e : m) e->state_kind = -1;
+    return ty;
+  }
+
+  size_t device_bytes() const {
+    size_t b = inst_dev.bytes + blk_dev.bytes + tfac_dev.bytes + lam_a.bytes + lam_b.bytes;
+    for (auto& p : pool) b += p->bytes;
+    return b;
+  }
+};
+
 }  // namespace lbmx
 
 using lbmx::Engine;
@@ -808,6 +1139,10 @@ using lbmx::Error;
 
 struct lbmx_solver {
   std::unique_ptr<Engine> e;
+};
+
+struct lbmx_batch {
+  std::unique_ptr<lbmx::Batch> b;
 };
 
 template <class F>
@@ -997,69 +1332,118 @@ lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, i
   return LBMX_OK;
 }
 
-// Replays the schedule of Engine::gradient with integer-labelled buffers: a buffer holds the
-// step of the state it contains; advancing from label k writes label k+1.
+// Runs the gradient's checkpointed pass (the same code as lbmx_gradient) on labelled host
+// buffers: a buffer holds the step of the state it contains; every forward sweep must read
+// p_{t-1} and every adjoint sweep must receive p_{t-1}.
 lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances, int* max_slots) {
   if (!advances || !max_slots || nsteps < 1 || nslots < 2) {
     lbmx::g_last_error = "bad argument";
     return LBMX_ERR_ARG;
   }
   return guarded([&] {
-    const int slots = std::min(nslots, std::max(2, nsteps));
-    lbmx::WangSchedule sched(slots);
-    std::vector<int> bufs(slots + 2, -1);  // label per physical buffer
-    std::vector<int> slotbuf(slots);
-    for (int k = 0; k < slots; ++k) slotbuf[k] = 2 + k;
-    int work[2] = {0, 1};
-    auto other = [&](int src) -> int& { return src == work[0] ? work[1] : work[0]; };
-    bufs[work[0]] = 0;
-    int src;
-    {
-      int s0 = sched.make_checkpoint(0);
-      std::swap(slotbuf[s0], work[0]);
-      src = slotbuf[s0];
-    }
-    int maxused = 1;
-    for (int k = 1; k < nsteps; ++k) {
-      int& dst = other(src);
-      bufs[dst] = bufs[src] + 1;
-      int s = sched.make_checkpoint(k);
-      std::swap(slotbuf[s], dst);
-      src = slotbuf[s];
-      maxused = std::max(maxused, slots - sched.free_slots());
-    }
-    long long adv = 0;
-    for (int step = nsteps; step >= 1; --step) {
-      int index = step - 1;
-      auto plan = sched.get_forward_solution(index);
-      int fsrc;
-      if (plan.stored()) {
-        fsrc = slotbuf[plan.from_slot];
-      } else {
-        int s2 = slotbuf[plan.from_slot];
-        if (bufs[s2] != plan.from_step) throw Error{LBMX_ERR_STATE, "restart buffer holds the wrong state"};
-        size_t next = 0;
-        for (int c = plan.from_step + 1; c <= plan.index; ++c) {
-          int& dst = other(s2);
-          bufs[dst] = bufs[s2] + 1;
-          ++adv;
-          if (next < plan.stores.size() && plan.stores[next].first == c) {
-            std::swap(slotbuf[plan.stores[next].second], dst);
-            s2 = slotbuf[plan.stores[next].second];
-            ++next;
-          } else {
-            s2 = dst;
-          }
-        }
-        fsrc = s2;
+    struct Labels : lbmx::SweepOps {
+      std::unordered_map<const void*, int> label;
+      int get(const void* p) const {
+        auto it = label.find(p);
+        return it == label.end() ? -1 : it->second;
       }
-      if (bufs[fsrc] != index) throw Error{LBMX_ERR_STATE, "delivered the wrong state at step " + std::to_string(step)};
-      maxused = std::max(maxused, slots - sched.free_slots());
-      sched.release();
-    }
-    *advances = adv;
-    *max_slots = maxused;
+      void init(void* dst) override { label[dst] = 0; }
+      void fwd(const void* src, void* dst, int t, bool) override {
+        if (get(src) != t - 1) throw Error{LBMX_ERR_STATE, "forward sweep to step " + std::to_string(t) + " read a wrong state"};
+        label[dst] = t;
+      }
+      void adj(const void* fsrc, const void*, void*, int t) override {
+        if (get(fsrc) != t - 1) throw Error{LBMX_ERR_STATE, "delivered the wrong state at step " + std::to_string(t)};
+      }
+    } ops;
+    const int slots = std::min(nslots, std::max(2, nsteps));
+    std::vector<char> storage(slots + 4);  // distinct addresses stand for buffers
+    std::vector<void*> slotbuf(slots);
+    for (int k = 0; k < slots; ++k) slotbuf[k] = &storage[k];
+    void* lin = &storage[slots + 2];
+    void* lout = &storage[slots + 3];
+    lbmx::PassResult r =
+        lbmx::checkpointed_pass(ops, nsteps, false, slotbuf, &storage[slots], &storage[slots + 1], lin, lout, false);
+    *advances = r.recompute;
+    *max_slots = r.max_slots;
   });
+}
+
+lbmx_status lbmx_batch_create(lbmx_solver* const* members, int count, lbmx_batch** out) {
+  if (!members || !out || count < 1) {
+    lbmx::g_last_error = "bad argument";
+    return LBMX_ERR_ARG;
+  }
+  return guarded([&] {
+    std::vector<Engine*> v;
+    for (int i = 0; i < count; ++i) {
+      if (!members[i] || !members[i]->e) throw Error{LBMX_ERR_ARG, "null solver handle in batch"};
+      v.push_back(members[i]->e.get());
+    }
+    auto* b = new lbmx_batch;
+    try {
+      b->b.reset(new lbmx::Batch(std::move(v)));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+lbmx_status lbmx_batch_destroy(lbmx_batch* b) {
+  delete b;
+  return LBMX_OK;
+}
+
+#define CHECK_BATCH(b)                                       \
+  if (!(b) || !(b)->b) {                                     \
+    lbmx::g_last_error = "null batch handle";                \
+    return LBMX_ERR_ARG;                                     \
+  }
+
+lbmx_status lbmx_batch_checkpoint_config(lbmx_batch* b, int nslots) {
+  CHECK_BATCH(b);
+  b->b->nslots = nslots;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_batch_set_tile_rows(lbmx_batch* b, int rows) {
+  CHECK_BATCH(b);
+  if (rows != 1 && rows != 2 && rows != 4) {
+    lbmx::g_last_error = "tile rows must be 1, 2 or 4";
+    return LBMX_ERR_ARG;
+  }
+  b->b->ty = rows;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_batch_autotune(lbmx_batch* b, int nsweeps, int* rows, double* ms) {
+  CHECK_BATCH(b);
+  return guarded([&] {
+    int r = b->b->autotune(nsweeps, ms);
+    if (rows) *rows = r;
+  });
+}
+
+lbmx_status lbmx_batch_gradient(lbmx_batch* b, int nsteps, double rho0, const double u0[3], double T0, double* J,
+                                double* const* sens) {
+  CHECK_BATCH(b);
+  return guarded([&] { b->b->gradient(nsteps, rho0, u0, T0, J, sens); });
+}
+
+lbmx_status lbmx_batch_get_stats(lbmx_batch* b, lbmx_stats* st) {
+  CHECK_BATCH(b);
+  if (!st) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  *st = b->b->stats;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_batch_device_bytes(lbmx_batch* b, size_t* bytes) {
+  CHECK_BATCH(b);
+  if (!bytes) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  *bytes = b->b->device_bytes();
+  return LBMX_OK;
 }
 
 lbmx_status lbmx_get_stats(lbmx_solver* s, lbmx_stats* st) {

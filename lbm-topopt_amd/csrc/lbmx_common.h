@@ -109,6 +109,36 @@ struct AdjArgs {
   ObjArgs obj;
 };
 
+// One instance of a batch: its static description for the batched sweeps.
+struct BatchInst {
+  Geometry g;             // grid and tiles (for the batch's tile height)
+  Layout ls, ll;          // layouts of its state and adjoint fields
+  ModelParams m;
+  ObjArgs obj;            // functional terms and per-tile accumulators of this instance
+  const void* gamma;      // design field (element type of the precision form)
+  double* sens;           // dJ/dgamma accumulator
+  const DevBC* bcs;
+  int nbc;
+  int blk_begin;          // first global workgroup of this instance
+  long long state_off;    // byte offset of the instance in the batch state buffers
+  long long lam_off;      // byte offset in the batch adjoint buffers
+};
+
+// Per-launch arguments of a batched sweep.
+struct BatchStep {
+  const void* src;        // forward: p_{t-1}; adjoint: the forward state p_{t-1}
+  void* dst;              // forward: p_t
+  const void* lam_in;     // adjoint: nu_t
+  void* lam_out;          // adjoint: nu_{t-1}
+  const BatchInst* inst;  // device table of instances
+  const int* blk_inst;    // instance of each (XCD-remapped) global workgroup
+  const double* tfac;     // time factors of this step: [instance][tf_stride]
+  int tf_stride;
+  int nblocks;            // total workgroups
+  int t;                  // time of the new state (forward) / reverse step (adjoint)
+  int with_obj;           // forward: accumulate the functional; adjoint: add its source
+};
+
 inline Layout make_layout(int nx, int ny, int nz, int dim, int elem_bytes) {
   Layout l{};
   l.ax = 128 / elem_bytes;

@@ -144,19 +144,22 @@ __device__ __forceinline__ void apply_bcs_adj(T (&l)[L::Q], const DevBC* __restr
   }
 }
 
+// TY: rows of the workgroup tile (kTileX x TY threads, one wave per row)
+template <int TY>
 __device__ __forceinline__ bool tile_meets(const int* bb, int x0, int y0, int z) {
-  int x1 = x0 + kTileX - 1, y1 = y0 + kTileY - 1;
+  int x1 = x0 + kTileX - 1, y1 = y0 + TY - 1;
   return !(x1 < bb[0] || x0 > bb[1] || y1 < bb[2] || y0 > bb[3] || z < bb[4] || z > bb[5]);
 }
 
+template <int TY>
 __device__ __forceinline__ double block_sum(double v) {
-  __shared__ double part[kTileY];
+  __shared__ double part[TY];
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if (threadIdx.x == 0) part[threadIdx.y] = v;
   __syncthreads();
   double s = 0;
   if (threadIdx.x == 0 && threadIdx.y == 0)
-    for (int w = 0; w < kTileY; ++w) s += part[w];
+    for (int w = 0; w < TY; ++w) s += part[w];
   return s;
 }
 
@@ -171,15 +174,15 @@ __device__ __forceinline__ T node_objective(const ObjArgs& o, const T (&f)[L::Q]
   return acc * T(o.time_const);
 }
 
-template <class L, int COLL, int DES, class P>
-__global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
+// One forward sweep over the node (x,y,z) of workgroup tile `tile`; called by every thread of
+// the workgroup (the objective reduction synchronises it). tfac: time factor per boundary piece.
+template <class L, int COLL, int DES, class P, int TY>
+__device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, int x, int y, int z, int tile) {
   using T = typename P::T;
-  int x, y, z, tile;
-  tile_coords(a.g, x, y, z, tile);
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool active = x < nx && y < ny;
   const bool objblk = a.obj.acc != nullptr && a.mode == 0 &&
-                      tile_meets(a.obj.bbox, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
+                      tile_meets<TY>(a.obj.bbox, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
   double objv = 0.0;
   if (active) {
     const unsigned int off = layout_off(a.ls, x, y, z);
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
       static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
     } else {
       pull_node<L, P>(a.src, a.ls, off, f);
-      if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, a.tfac, x, y, z);
+      if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, tfac, x, y, z);
     }
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
@@ -205,9 +208,16 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
     }
   }
   if (objblk) {
-    double s = block_sum(objv);
+    double s = block_sum<TY>(objv);
     if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += s;
   }
+}
+
+template <class L, int COLL, int DES, class P>
+__global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
+  int x, y, z, tile;
+  tile_coords(a.g, x, y, z, tile);
+  fwd_node<L, COLL, DES, P, kTileY>(a, a.tfac, x, y, z, tile);
 }
 
 // Initial state: s_0 = f_eq(rho, u[, T]) from uniform or per-node macros; writes p_0 = C(s_0)
@@ -263,19 +273,18 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_init(FwdArgs a, const double
   }
 }
 
+// One adjoint sweep over node (x,y,z); tfac: time factor per boundary piece at step a.t.
 template <class L, int COLL, int DES, class P>
-__global__ __launch_bounds__(kTileX* kTileY) void k_adj(AdjArgs a) {
+__device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, int x, int y, int z) {
   using T = typename P::T;
   using LS = typename P::LS;
-  int x, y, z, tile;
-  tile_coords(a.g, x, y, z, tile);
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   if (x >= nx || y >= ny) return;
   const int n = (z * ny + y) * nx + x;
   const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
   T f[L::Q];
   pull_node<L, P>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
-  if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, a.tfac, x, y, z);
+  if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, tfac, x, y, z);
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
   T lam[L::Q];
   static_for<0, L::Q>([&](auto qc) {
@@ -315,6 +324,84 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_adj(AdjArgs a) {
   });
 }
 
+template <class L, int COLL, int DES, class P>
+__global__ __launch_bounds__(kTileX* kTileY) void k_adj(AdjArgs a) {
+  int x, y, z, tile;
+  tile_coords(a.g, x, y, z, tile);
+  adj_node<L, COLL, DES, P>(a, a.tfac, x, y, z);
+}
+
+// ---- batched sweeps: one launch covers every instance of a batch ------------------------
+// The global workgroup id is remapped across XCDs as for a single grid, then mapped to
+// (instance, local tile) through blk_inst; each instance's static description (grid,
+// layouts, offsets into the batch buffers, design, sensitivity, boundary and functional
+// arrays) comes from a device table, the time factors from a per-step table.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int qd = nb >> 3, r = nb & 7;
+  const int xcd = b & 7, k = b >> 3;
+  return xcd < r ? xcd * (qd + 1) + k : r * (qd + 1) + (xcd - r) * qd + k;
+}
+
+template <int TY>
+__device__ __forceinline__ const BatchInst& batch_locate(const BatchStep& st, int& inst, int& x, int& y, int& z,
+                                                         int& tile) {
+  const int b = xcd_remap(blockIdx.x, st.nblocks);
+  inst = st.blk_inst[b];
+  const BatchInst& I = st.inst[inst];
+  tile = b - I.blk_begin;
+  const int bx = tile % I.g.nblk_x;
+  const int rest = tile / I.g.nblk_x;
+  const int by = rest % I.g.nblk_y;
+  z = rest / I.g.nblk_y;
+  x = bx * kTileX + threadIdx.x;
+  y = by * TY + threadIdx.y;
+  return I;
+}
+
+template <class L, int COLL, int DES, class P, int TY>
+__global__ __launch_bounds__(kTileX* TY) void k_fwd_batch(BatchStep st) {
+  int i, x, y, z, tile;
+  const BatchInst& I = batch_locate<TY>(st, i, x, y, z, tile);
+  FwdArgs a;
+  a.src = static_cast<const char*>(st.src) + I.state_off;
+  a.dst = static_cast<char*>(st.dst) + I.state_off;
+  a.gamma = I.gamma;
+  a.bcs = I.bcs;
+  a.nbc = I.nbc;
+  a.t = st.t;
+  a.mode = 0;
+  a.aos_out = nullptr;
+  a.g = I.g;
+  a.ls = I.ls;
+  a.m = I.m;
+  a.obj = I.obj;
+  if (!st.with_obj) a.obj.acc = nullptr;
+  fwd_node<L, COLL, DES, P, TY>(a, st.tfac + (long long)i * st.tf_stride, x, y, z, tile);
+}
+
+template <class L, int COLL, int DES, class P, int TY>
+__global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
+  int i, x, y, z, tile;
+  const BatchInst& I = batch_locate<TY>(st, i, x, y, z, tile);
+  AdjArgs a;
+  a.fwd_src = static_cast<const char*>(st.src) + I.state_off;
+  a.lam_in = static_cast<const char*>(st.lam_in) + I.lam_off;
+  a.lam_out = static_cast<char*>(st.lam_out) + I.lam_off;
+  a.gamma = I.gamma;
+  a.sens = I.sens;
+  a.bcs = I.bcs;
+  a.nbc = I.nbc;
+  a.t = st.t;
+  a.g = I.g;
+  a.ls = I.ls;
+  a.ll = I.ll;
+  a.m = I.m;
+  a.obj = I.obj;
+  a.obj.acc = nullptr;
+  if (!st.with_obj) a.obj.nterm = 0;
+  adj_node<L, COLL, DES, P>(a, st.tfac + (long long)i * st.tf_stride, x, y, z);
+}
+
 // ---- host-side launchers ----
 template <class L, int COLL, int DES, class P>
 hipError_t launch_fwd_t(const FwdArgs& a, hipStream_t s) {
@@ -333,6 +420,15 @@ hipError_t launch_init_t(const FwdArgs& a, const double* macros, const double* u
 template <class L, int COLL, int DES, class P>
 hipError_t launch_adj_t(const AdjArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_adj<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class L, int COLL, int DES, class P, int TY>
+hipError_t launch_batch_t(int kind, const BatchStep& st, hipStream_t s) {
+  if (kind == 0)
+    hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);
+  else
+    hipLaunchKernelGGL((k_adj_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);
   return hipGetLastError();
 }
 

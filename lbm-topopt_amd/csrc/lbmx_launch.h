@@ -20,4 +20,8 @@ LBMX_DECLARE_LATTICE(thermal)
 #undef LBMX_DECLARE_LATTICE
 #undef LBMX_DECLARE_LAUNCHERS
 
+// Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
+hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+hipError_t launch_batch_d2q9_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+
 }  // namespace lbmx

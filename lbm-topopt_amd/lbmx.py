@@ -138,6 +138,14 @@ def library():
         "lbmx_time_sweeps": [C.c_void_p, C.c_int, C.c_int, dp],
         "lbmx_stream_copy": [C.c_size_t, C.c_int, dp],
         "lbmx_device_count": [P(C.c_int)],
+        "lbmx_batch_create": [P(C.c_void_p), C.c_int, P(C.c_void_p)],
+        "lbmx_batch_destroy": [C.c_void_p],
+        "lbmx_batch_checkpoint_config": [C.c_void_p, C.c_int],
+        "lbmx_batch_set_tile_rows": [C.c_void_p, C.c_int],
+        "lbmx_batch_autotune": [C.c_void_p, C.c_int, P(C.c_int), dp],
+        "lbmx_batch_gradient": [C.c_void_p, C.c_int, C.c_double, dp, C.c_double, dp, P(dp)],
+        "lbmx_batch_get_stats": [C.c_void_p, P(Stats)],
+        "lbmx_batch_device_bytes": [C.c_void_p, P(C.c_size_t)],
         "lbmx_describe_boundaries": [C.c_void_p, P(C.c_int), C.c_int, P(C.c_int)],
         "lbmx_checkpoint_simulate": [C.c_int, C.c_int, P(C.c_longlong), P(C.c_int)],
     }
@@ -157,7 +165,9 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_get_macros", "lbmx_get_timestep", "lbmx_checkpoint_config", "lbmx_gradient",
             "lbmx_get_sensitivity", "lbmx_get_adjoint", "lbmx_get_stats", "lbmx_device_bytes",
             "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps", "lbmx_stream_copy",
-            "lbmx_device_count"]
+            "lbmx_device_count", "lbmx_batch_create", "lbmx_batch_destroy", "lbmx_batch_checkpoint_config",
+            "lbmx_batch_set_tile_rows", "lbmx_batch_autotune", "lbmx_batch_gradient", "lbmx_batch_get_stats",
+            "lbmx_batch_device_bytes"]
 
 
 def checkpoint_simulate(nslots, nsteps):
@@ -321,3 +331,60 @@ class Solver:
         b = C.c_size_t(0)
         _check(self.lib.lbmx_device_bytes(self.h, C.byref(b)))
         return b.value
+
+
+class Batch:
+    """Batched robust-ensemble form: member Solvers (same lattice/operator/coupling/precision)
+    advanced in lockstep, one launch per sweep over all members. Members must outlive it."""
+
+    def __init__(self, members):
+        self.lib = library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        self.h = C.c_void_p()
+        _check(self.lib.lbmx_batch_create(arr, len(self.members), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.lbmx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def checkpoint_config(self, nslots):
+        _check(self.lib.lbmx_batch_checkpoint_config(self.h, nslots))
+
+    def set_tile_rows(self, rows):
+        _check(self.lib.lbmx_batch_set_tile_rows(self.h, rows))
+
+    def autotune(self, nsweeps=10):
+        """Returns (chosen tile rows, ms per forward+adjoint sweep pair for rows 1, 2, 4)."""
+        r = C.c_int(0)
+        ms = (C.c_double * 3)()
+        _check(self.lib.lbmx_batch_autotune(self.h, nsweeps, C.byref(r), ms))
+        return r.value, list(ms)
+
+    def gradient(self, nsteps, rho0=1.0, u0=(0.0, 0.0, 0.0), T0=0.0, want_sens=True):
+        """Returns (J per member, list of dJ/dgamma arrays or None)."""
+        n = len(self.members)
+        J = np.zeros(n)
+        u = np.asarray(u0, dtype=np.float64)
+        sens = [np.empty(m.nodes) for m in self.members] if want_sens else None
+        ptrs = (C.POINTER(C.c_double) * n)(*[_dptr(a) for a in sens]) if want_sens else None
+        _check(self.lib.lbmx_batch_gradient(self.h, nsteps, rho0, _dptr(u), T0, _dptr(J), ptrs))
+        return J, sens
+
+    def stats(self):
+        st = Stats()
+        _check(self.lib.lbmx_batch_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in Stats._fields_}
+
+    def device_bytes(self):
+        b = C.c_size_t(0)
+        _check(self.lib.lbmx_batch_device_bytes(self.h, C.byref(b)))
+        return b.value
+

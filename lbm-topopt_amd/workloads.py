@@ -106,3 +106,38 @@ def thermal_channel2d(obj, nx, ny, u_max=0.05, T_in=1.0, T_wall=0.0, time_const=
     obj.add_boundary(B(0, 0, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_DIRICHLET, T_in))
     obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_BOUNCEBACK))
     obj.set_objective(lbmx.make_objective(lbmx.OBJ_HEAT_FLUX, B(nx - 1, nx - 1, 1, ny - 2), None, time_const))
+
+
+def heaviside_project(g, beta=8.0, eta=0.5):
+    """Smoothed Heaviside (tanh) projection of a filtered design,
+    (tanh(beta eta) + tanh(beta (g - eta))) / (tanh(beta eta) + tanh(beta (1 - eta)))
+    (ProjectionFilter2d::filterDesign, src/topOpt/projectionFilter2d.cc:27-47)."""
+    g = np.asarray(g, dtype=np.float64)
+    return (np.tanh(beta * eta) + np.tanh(beta * (g - eta))) / (np.tanh(beta * eta) + np.tanh(beta * (1.0 - eta)))
+
+
+def heaviside_project_grad(g, beta=8.0, eta=0.5):
+    """d(projection)/dg, the factor filterSensitivities applies (projectionFilter2d.cc:65-67)."""
+    g = np.asarray(g, dtype=np.float64)
+    return beta * (1.0 - np.tanh(beta * (g - eta)) ** 2) / (np.tanh(beta * eta) + np.tanh(beta * (1.0 - eta)))
+
+
+# Robust ensemble of the batched form: 3 projection realisations (eroded / nominal / dilated)
+# x 8 design candidates on uneven grids, two of each size.
+ENSEMBLE_ETAS = (0.3, 0.5, 0.7)
+ENSEMBLE_GRIDS = ((256, 128), (256, 128), (512, 256), (512, 256), (768, 384), (768, 384), (1024, 512), (1024, 512))
+
+
+def ensemble_members(make_solver, grids=ENSEMBLE_GRIDS, etas=ENSEMBLE_ETAS, beta=8.0, steps=3000, seed0=1000):
+    """Builds the robust ensemble: for candidate c (grid grids[c], design seed seed0 + c) and
+    realisation eta, a channel problem whose design is the projected field. make_solver(nx, ny)
+    returns an object with the Solver setup interface. Returns [(solver, c, eta), ...]."""
+    out = []
+    for c, (nx, ny) in enumerate(grids):
+        raw = design_field(nx, ny, 1, seed=seed0 + c)
+        for eta in etas:
+            s = make_solver(nx, ny)
+            channel2d(s, nx, ny, time_const=1.0 / steps)
+            s.set_design(heaviside_project(raw, beta, eta))
+            out.append((s, c, eta))
+    return out

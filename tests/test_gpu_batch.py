@@ -1,0 +1,96 @@
+"""Batched robust-ensemble form against single-instance gradients, through the C ABI.
+
+Each member of a batch must get exactly the gradient it gets alone: the batched sweep runs
+the same node code on the same data, so dJ/dgamma is bitwise equal; J is equal to rounding
+(its per-tile partial sums are grouped by the batch's tile height).
+"""
+import numpy as np
+import pytest
+
+import lbmx
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 40
+
+
+def member(nx, ny, coll, design, prec, seed, eta, sine=False, objective=lbmx.OBJ_PRESSURE_DROP):
+    d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=coll, design=design, precision=prec,
+                       omega=1.0 / 0.6, interp_q=0.1, alpha_max=1.0 if design == lbmx.DESIGN_PBB else 0.1)
+    s = lbmx.Solver(d)
+    workloads.channel2d(s, nx, ny, objective=objective, time_const=1.0 / STEPS,
+                        modulation=(17.0, 0.4) if sine else None)
+    s.set_design(workloads.heaviside_project(workloads.design_field(nx, ny, 1, seed=seed), 8.0, eta))
+    return s
+
+
+def build(coll, design, prec):
+    # uneven grids (not multiples of the tile), different designs, boundary time modes and functionals
+    spec = [((64, 32), 0.3, False, lbmx.OBJ_PRESSURE_DROP), ((96, 40), 0.5, True, lbmx.OBJ_PRESSURE_DROP),
+            ((130, 50), 0.7, False, lbmx.OBJ_OUTFLOW), ((70, 21), 0.5, True, lbmx.OBJ_DISSIPATED_POWER)]
+    return [member(nx, ny, coll, design, prec, 1000 + i, eta, sine, obj)
+            for i, ((nx, ny), eta, sine, obj) in enumerate(spec)]
+
+
+def singles(members, slots):
+    out = []
+    for m in members:
+        m.checkpoint_config(slots)
+        out.append(m.gradient(STEPS))
+    return out
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4])
+@pytest.mark.parametrize("prec", [lbmx.FP64, lbmx.FP32])
+def test_batch_equals_single_instances(lbmx_mod, prec, rows):
+    members = build(lbmx.CASCADED, lbmx.DESIGN_PBB, prec)
+    ref = singles(members, 6)
+    b = lbmx.Batch(members)
+    b.set_tile_rows(rows)
+    b.checkpoint_config(6)
+    J, sens = b.gradient(STEPS)
+    for i, (Js, gs) in enumerate(ref):
+        assert abs(J[i] - Js) <= 1e-13 * abs(Js), (i, J[i], Js)
+        assert np.array_equal(sens[i], gs), (i, float(np.max(np.abs(sens[i] - gs))))
+    st = b.stats()
+    assert st["adj_sweeps"] == STEPS
+    b.close()
+
+
+@pytest.mark.parametrize("coll,design", [(lbmx.BGK, lbmx.DESIGN_BRINKMAN), (lbmx.MRT, lbmx.DESIGN_PBB),
+                                         (lbmx.MRT, lbmx.DESIGN_NONE)])
+def test_batch_other_operators(lbmx_mod, coll, design):
+    members = build(coll, design, lbmx.FP64)
+    ref = singles(members, 0)
+    b = lbmx.Batch(members)
+    b.checkpoint_config(0)
+    J, sens = b.gradient(STEPS)
+    for i, (Js, gs) in enumerate(ref):
+        assert abs(J[i] - Js) <= 1e-13 * abs(Js)
+        assert np.array_equal(sens[i], gs)
+    b.close()
+
+
+def test_batch_autotune_and_repeat(lbmx_mod):
+    members = build(lbmx.CASCADED, lbmx.DESIGN_PBB, lbmx.FP32)
+    ref = singles(members, 5)
+    b = lbmx.Batch(members)
+    b.checkpoint_config(5)
+    rows, ms = b.autotune(4)
+    assert rows in (1, 2, 4) and all(v > 0 for v in ms)
+    for _ in range(2):  # the batch is reusable; results do not depend on the tile height
+        J, sens = b.gradient(STEPS)
+        for i, (Js, gs) in enumerate(ref):
+            assert abs(J[i] - Js) <= 1e-13 * abs(Js)
+            assert np.array_equal(sens[i], gs)
+    b.close()
+
+
+def test_batch_rejects_mixed_members(lbmx_mod):
+    a = member(64, 32, lbmx.CASCADED, lbmx.DESIGN_PBB, lbmx.FP32, 1, 0.5)
+    c = member(64, 32, lbmx.MRT, lbmx.DESIGN_PBB, lbmx.FP32, 2, 0.5)
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.Batch([a, c])
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.Batch([a, a])
