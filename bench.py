@@ -100,6 +100,52 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
     }
 
 
+def run_ensemble(lbmx, workloads, steps, slots, warmup):
+    """Robust ensemble (24 members: 3 projection realisations x 8 candidates on uneven grids,
+    PBB<IncompressibleCascaded2d> at tau = 0.6, FP32): batched gradient vs the same 24
+    gradients as sequential single-instance runs. Aggregate MLUPS = total nodes x steps / s."""
+    def mk(nx, ny):
+        return lbmx.Solver(lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=lbmx.CASCADED,
+                                          design=lbmx.DESIGN_PBB, precision=lbmx.FP32, omega=1.0 / 0.6,
+                                          interp_q=0.1, alpha_max=1.0))
+    members = [s for s, _, _ in workloads.ensemble_members(mk, steps=steps)]
+    nodes = sum(m.nodes for m in members)
+    b = lbmx.Batch(members)
+    b.checkpoint_config(slots)
+    rows, tune_ms = b.autotune(10)
+    log(f"ensemble: {len(members)} members, {nodes} nodes, autotuned tile rows {rows} "
+        f"(ms per fwd+adj sweep for rows 1/2/4: {', '.join(f'{v:.3f}' for v in tune_ms)})")
+    for _ in range(warmup):
+        b.gradient(steps, want_sens=False)
+    t0 = time.perf_counter()
+    Jb, _ = b.gradient(steps)
+    tb = time.perf_counter() - t0
+    st = b.stats()
+    log(f"ensemble: batched gradient {tb:.2f} s (forward {st['forward_phase_ms'] / 1e3:.2f} s, "
+        f"reverse {st['reverse_phase_ms'] / 1e3:.2f} s)")
+    for m in members:
+        m.checkpoint_config(slots)
+    for _ in range(warmup):
+        for m in members:
+            m.gradient(steps, want_sens=False)
+    t0 = time.perf_counter()
+    Js = [m.gradient(steps)[0] for m in members]
+    ts = time.perf_counter() - t0
+    log(f"ensemble: sequential single-instance gradients {ts:.2f} s")
+    jerr = max(abs(a - c) / abs(c) for a, c in zip(Jb, Js))
+    dev = b.device_bytes()
+    b.close()
+    for m in members:
+        m.close()
+    return {"members": len(members), "nodes": nodes, "time_steps": steps, "checkpoints": slots,
+            "tile_rows": rows, "autotune_ms_rows_1_2_4": [round(v, 4) for v in tune_ms],
+            "batched_s": round(tb, 3), "sequential_s": round(ts, 3),
+            "batched_mlups": round(nodes * steps / tb / 1e6, 1),
+            "sequential_mlups": round(nodes * steps / ts / 1e6, 1),
+            "speedup": round(ts / tb, 3), "max_rel_J_diff_batched_vs_single": jerr,
+            "batch_device_gb": round(dev / 1e9, 2)}
+
+
 def read_traffic(key):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by the
     copy kernel) for the 384x192x192 workload, from profiles/traffic_d3q19_mrt_brinkman.txt."""
@@ -145,6 +191,9 @@ def main():
     ap.add_argument("--sweeps", type=int, default=50)
     ap.add_argument("--forms", type=str, default="fp64,fp16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=3000,
+                    help="time steps of the batched robust-ensemble measurement (0: skip it)")
+    ap.add_argument("--ensemble-slots", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but "
@@ -188,6 +237,8 @@ def main():
     }
     if "fp16" in res and "fp64" in res:
         out["fp16_over_fp64"] = round(res["fp16"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
+    if args.ensemble_steps > 0:
+        out["ensemble"] = run_ensemble(lbmx, workloads, args.ensemble_steps, args.ensemble_slots, min(args.warmup, 1))
     out["stream_copy_gbs"] = round(lbmx.stream_copy_gbs(), 1)
     log(f"stream copy ceiling {out['stream_copy_gbs']} GB/s")
     if not args.no_cpu_baseline:

@@ -157,3 +157,45 @@ def test_checkpoint_headline_schedules():
         adv, mx = lbmx.checkpoint_simulate(slots, n)
         assert mx <= slots
         assert adv <= 2 * revolve_recompute(n - 1, slots - 1) + n
+
+
+# ----------------------------------------------------------------------- robust ensemble inputs
+def test_heaviside_projection_closed_forms():
+    g = np.linspace(0.0, 1.0, 101)
+    for eta in workloads.ENSEMBLE_ETAS:
+        p = workloads.heaviside_project(g, 8.0, eta)
+        assert abs(p[0]) < 1e-15 and abs(p[-1] - 1.0) < 1e-15
+        assert np.all(np.diff(p) > 0)
+        # p(eta) = tanh(beta eta) / (tanh(beta eta) + tanh(beta (1 - eta)))
+        pe = workloads.heaviside_project(np.array([eta]), 8.0, eta)[0]
+        assert abs(pe - np.tanh(8 * eta) / (np.tanh(8 * eta) + np.tanh(8 * (1 - eta)))) < 1e-15
+        d = workloads.heaviside_project_grad(g, 8.0, eta)
+        fd = (workloads.heaviside_project(g + 1e-6, 8.0, eta) - workloads.heaviside_project(g - 1e-6, 8.0, eta)) / 2e-6
+        assert np.max(np.abs(fd - d)) < 1e-8
+
+
+def test_ensemble_members_layout():
+    class Stub:
+        def __init__(self, nx, ny):
+            self.nx, self.ny, self.bcs, self.obj, self.design = nx, ny, [], None, None
+
+        def add_boundary(self, box, bc):
+            self.bcs.append((box, bc))
+
+        def set_objective(self, o):
+            self.obj = o
+
+        def set_design(self, g):
+            self.design = np.asarray(g)
+
+    ms = workloads.ensemble_members(Stub, steps=3000)
+    assert len(ms) == 24
+    assert sum(s.nx * s.ny for s, _, _ in ms) == 5898240
+    for s, c, eta in ms:
+        assert (s.nx, s.ny) == workloads.ENSEMBLE_GRIDS[c] and eta in workloads.ENSEMBLE_ETAS
+        assert s.design.size == s.nx * s.ny and len(s.bcs) == 4 and s.obj is not None
+        assert abs(s.obj.time_const - 1.0 / 3000) < 1e-18
+    # the three realisations of one candidate project the same raw field: a larger threshold
+    # erodes (dilated eta = 0.3 >= nominal 0.5 >= eroded 0.7 pointwise)
+    a, b, c = ms[0][0].design, ms[1][0].design, ms[2][0].design
+    assert np.all(a >= b) and np.all(b >= c) and not np.array_equal(a, c)
