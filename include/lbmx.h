@@ -266,6 +266,11 @@ lbmx_status lbmx_batch_destroy(lbmx_batch* b);
 lbmx_status lbmx_batch_checkpoint_config(lbmx_batch* b, int nslots);
 /* rows of the 64-wide workgroup tile of the batched sweeps: 1, 2 or 4 (default 4) */
 lbmx_status lbmx_batch_set_tile_rows(lbmx_batch* b, int rows);
+/* enable != 0: the first lbmx_batch_gradient captures its whole launch sequence into a HIP
+   graph and later calls with the same steps, slots, tile height and buffers replay it (designs
+   may change in between; they are read from the same device arrays). Forward/reverse phase
+   times are then not separated in the stats. */
+lbmx_status lbmx_batch_set_graph(lbmx_batch* b, int enable);
 /* Autotuner: times nsweeps batched forward + adjoint sweeps for each tile height, keeps the
    fastest; *rows = chosen height, ms[3] (may be NULL) = ms per forward+adjoint sweep pair for
    heights 1, 2, 4. The members' current states are discarded. */

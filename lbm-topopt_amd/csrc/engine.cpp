@@ -69,8 +69,8 @@ struct DeviceBuffer {
   }
 };
 
-// zero-fill with a kernel-free path that is safe outside graph capture
-static void zero_async(void* p, size_t bytes, hipStream_t s) { HIP_OK(hipMemsetAsync(p, 0, bytes, s)); }
+// zero fill by a kernel (also valid inside a captured graph)
+static void zero_async(void* p, size_t bytes, hipStream_t s) { HIP_OK(fill_zero(p, bytes, s)); }
 
 // The sweeps of one design-gradient pass, over opaque state buffers (a single lattice, a batch
 // of lattices in lockstep, or labelled host buffers that check the schedule).
@@ -200,7 +200,8 @@ struct Engine {
       default: ok = false;
     }
     if (!ok) throw Error{LBMX_ERR_SUP, "collision operator not available for this lattice"};
-    if (d.design == LBMX_DESIGN_DARCY) throw Error{LBMX_ERR_SUP, "DarcyMRT2d coupling is not available"};
+    if (d.design == LBMX_DESIGN_DARCY && !(d.lattice == LBMX_D2Q9 && d.collision == LBMX_MRT))
+      throw Error{LBMX_ERR_SUP, "the DarcyMRT2d coupling is defined for D2Q9 MRT"};
     inc = !(d.collision == LBMX_MRT_STANDARD || d.collision == LBMX_BGK_STANDARD);
     N = (long long)d.nx * d.ny * d.nz;
     if (N >= (1LL << 31)) throw Error{LBMX_ERR_SUP, "grids of 2^31 nodes or more are not supported"};
@@ -665,6 +666,8 @@ struct Engine {
   // ---- gradient: forward with checkpoints, reverse sweep with recomputation ----
   void gradient(int nsteps, double rho0, const double* u0, double T0, double* J, double* sens, size_t n) {
     if (!inc) throw Error{LBMX_ERR_SUP, "no adjoint for the standard (compressible) operators"};
+    if (d.design == LBMX_DESIGN_DARCY)
+      throw Error{LBMX_ERR_SUP, "DarcyMRT2d has no reverse operator (ReverseDarcyMRT2d is empty)"};
     if (nsteps < 1) throw Error{LBMX_ERR_ARG, "nsteps must be >= 1"};
     if (sens && (long long)n != N) throw Error{LBMX_ERR_ARG, "sensitivity length != nodes"};
     prepare();
@@ -794,7 +797,8 @@ struct Engine {
   double time_sweeps(int kind, int nsweeps) {
     require_state();
     if (nsweeps < 1) throw Error{LBMX_ERR_ARG, "nsweeps must be >= 1"};
-    if (kind == 1 && !inc) throw Error{LBMX_ERR_SUP, "no adjoint for the standard (compressible) operators"};
+    if (kind == 1 && (!inc || d.design == LBMX_DESIGN_DARCY))
+      throw Error{LBMX_ERR_SUP, "no adjoint sweep for this operator"};
     prepare();
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
@@ -870,6 +874,11 @@ struct Batch {
   DeviceBuffer inst_dev, blk_dev, tfac_dev, lam_a, lam_b;
   std::vector<std::unique_ptr<DeviceBuffer>> pool;
   lbmx_stats stats{};
+  std::vector<hipStream_t> member_streams;  // restored when the batch goes away
+  bool use_graph = false;                   // replay the captured gradient pass
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> gkey;              // what the captured graph was built for
+  long long graph_recompute = 0;
 
   explicit Batch(std::vector<Engine*> members) : m(std::move(members)) {
     if (m.empty()) throw Error{LBMX_ERR_ARG, "empty batch"};
@@ -884,13 +893,46 @@ struct Batch {
     }
     if (d0.precision != LBMX_FP32 && d0.precision != LBMX_FP64)
       throw Error{LBMX_ERR_SUP, "batched sweeps are built for the FP64 and FP32 forms"};
+    if (d0.design == LBMX_DESIGN_DARCY) throw Error{LBMX_ERR_SUP, "DarcyMRT2d has no reverse operator"};
     if (d0.collision != LBMX_BGK && d0.collision != LBMX_MRT && d0.collision != LBMX_CASCADED)
       throw Error{LBMX_ERR_SUP, "batched sweeps are built for incompressible BGK, MRT and cascaded"};
-    stream = m[0]->stream;
+    // a stream of its own (default priority): graph capture needs a non-null stream
+    HIP_OK(hipStreamCreate(&stream));
+    for (Engine* e : m) member_streams.push_back(e->stream);
   }
 
   ~Batch() {
-    if (!pool.empty() || lam_a.p) (void)hipStreamSynchronize(stream);
+    (void)hipStreamSynchronize(stream);
+    drop_graph();
+    for (size_t i = 0; i < m.size(); ++i) m[i]->stream = member_streams[i];
+    (void)hipStreamDestroy(stream);
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gkey.clear();
+  }
+
+  // every address and size the captured launch sequence depends on
+  std::vector<long long> graph_key(int nsteps, int slots, bool naive, double rho0, const double* u0, double T0) const {
+    std::vector<long long> k = {nsteps, slots, naive ? 1 : 0, ty, nblocks, tf_stride};
+    auto bits = [&](double v) {
+      long long b;
+      std::memcpy(&b, &v, sizeof b);
+      k.push_back(b);
+    };
+    bits(rho0); bits(T0);  // the initial state is part of the captured launches
+    for (int a = 0; a < 3; ++a) bits(u0 ? u0[a] : 0.0);
+    auto ptr = [&](const void* p) { k.push_back((long long)(uintptr_t)p); };
+    for (auto& p : pool) ptr(p->p);
+    ptr(lam_a.p); ptr(lam_b.p); ptr(inst_dev.p); ptr(blk_dev.p); ptr(tfac_dev.p);
+    for (size_t i = 0; i < m.size(); ++i) {
+      ptr(inst[i].obj.acc); ptr(inst[i].sens); ptr(inst[i].gamma); ptr(inst[i].bcs); ptr(inst[i].obj.terms);
+      k.push_back(inst[i].state_off);
+      k.push_back(inst[i].lam_off);
+    }
+    return k;
   }
 
   Geometry geom_for(const Engine& e) const {
@@ -1028,6 +1070,7 @@ struct Batch {
       Batch& b;
       double rho0, T0;
       const double* u0;
+      bool capturing = false;
       bool reverse_started = false;
       std::chrono::steady_clock::time_point t_fwd_end;
       Ops(Batch& bb, double r, const double* u, double T) : b(bb), rho0(r), T0(T), u0(u) {}
@@ -1041,7 +1084,7 @@ struct Batch {
       void adj(const void* fsrc, const void* lin, void* lout, int t) override {
         if (!reverse_started) {
           reverse_started = true;
-          HIP_OK(hipStreamSynchronize(b.stream));  // end of the forward phase
+          if (!capturing) HIP_OK(hipStreamSynchronize(b.stream));  // end of the forward phase
           t_fwd_end = std::chrono::steady_clock::now();
           zero_async(b.lam_a.p, b.lam_total, b.stream);
           for (auto& I : b.inst) zero_async(I.sens, (size_t)I.g.N * 8, b.stream);
@@ -1053,10 +1096,42 @@ struct Batch {
     auto tf0 = std::chrono::steady_clock::now();
     void* lin = lam_a.p;
     void* lout = lam_b.p;
-    PassResult pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true);
+    PassResult pr;
+    if (!use_graph) {
+      pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true);
+    } else {
+      // the launch sequence of a pass is fixed by (steps, slots, tile height) and the buffer
+      // addresses: capture it once, replay it on later calls
+      std::vector<long long> key = graph_key(nsteps, slots, naive, rho0, u0, T0);
+      if (!gexec || key != gkey) {
+        drop_graph();
+        ops.capturing = true;
+        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+          pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true);
+        } catch (...) {
+          (void)hipStreamEndCapture(stream, &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          throw;
+        }
+        HIP_OK(hipStreamEndCapture(stream, &graph));
+        hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+          gexec = nullptr;
+          throw Error{LBMX_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)};
+        }
+        gkey = key;
+        graph_recompute = pr.recompute;
+      }
+      pr.recompute = graph_recompute;
+      tf0 = std::chrono::steady_clock::now();
+      HIP_OK(hipGraphLaunch(gexec, stream));
+      ops.t_fwd_end = tf0;  // phases are not separated inside a replay
+    }
     HIP_OK(hipStreamSynchronize(stream));
     auto tr1 = std::chrono::steady_clock::now();
-    if (lin != lam_a.p) std::swap(lam_a.p, lam_b.p);
     long long nodes = 0;
     for (size_t i = 0; i < m.size(); ++i) {
       if (J) J[i] = reduce_acc(i);
@@ -1415,6 +1490,13 @@ lbmx_status lbmx_batch_set_tile_rows(lbmx_batch* b, int rows) {
     return LBMX_ERR_ARG;
   }
   b->b->ty = rows;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_batch_set_graph(lbmx_batch* b, int enable) {
+  CHECK_BATCH(b);
+  b->b->use_graph = enable != 0;
+  if (!enable) b->b->drop_graph();
   return LBMX_OK;
 }
 

@@ -13,7 +13,8 @@
 
 namespace lbmx {
 
-enum : int { D_NONE = LBMX_DESIGN_NONE, D_PBB = LBMX_DESIGN_PBB, D_BRINK = LBMX_DESIGN_BRINKMAN };
+enum : int { D_NONE = LBMX_DESIGN_NONE, D_PBB = LBMX_DESIGN_PBB, D_BRINK = LBMX_DESIGN_BRINKMAN,
+             D_DARCY = LBMX_DESIGN_DARCY };
 
 #define LBMX_BAS BasisOf<L::QF>::B
 
@@ -458,6 +459,31 @@ __device__ __forceinline__ void reverse_cascaded(const T (&f)[L::Q], T gamma, co
   });
 }
 
+// DarcyMRT2d forcing (src/dynamics/darcyMRT2d.hh:33-59), added to the MRT output: drag
+// F = -amax BP_q(gamma) rho u + body force with the unshifted incompressible macros u = j,
+// through the operator's closed-form polynomials (MRT rates s1 = s2 = 1.4).
+template <class T>
+__device__ __forceinline__ void darcy_force(T (&f)[9], T rho, const T (&j)[3], T gamma, const ModelParams& mp) {
+  const T s1 = T(1.4), s2 = T(1.4), om = T(mp.omega);
+  const T ux = j[0], uy = j[1];
+  const T im = -T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma) * rho;
+  const T Fx = im * ux + T(mp.body[0]), Fy = im * uy + T(mp.body[1]);
+  const T o3 = T(1) / T(3), o6 = T(1) / T(6), o12 = T(1) / T(12), o24 = T(1) / T(24);
+  const T ax = (-T(4) * s1 + T(2) * s2 + T(4)) * ux;           // common x-part of the diagonal terms
+  const T by = (s1 - T(0.5) * s2 - T(1)) * uy;
+  const T cy = (s1 - T(2) * s2 - T(3) * om + T(8));
+  const T dd = (s1 - T(2) * s2 + T(3) * om - T(4));
+  f[0] += o3 * (s1 + s2 - T(4)) * (Fx * ux + Fy * uy);
+  f[1] += o24 * (ax + T(2) + (T(3) * om - T(6)) * uy) * Fx - o6 * Fy * ((-T(0.75) * om + T(1.5)) * ux + T(0.5) + by);
+  f[2] += o12 * (-T(4) + cy * uy) * Fy + o12 * Fx * ux * dd;
+  f[3] += o24 * (ax - T(2) + (-T(3) * om + T(6)) * uy) * Fx - o6 * Fy * ((T(0.75) * om - T(1.5)) * ux + T(0.5) + by);
+  f[4] += o12 * (-T(4) + cy * ux) * Fx + o12 * Fy * uy * dd;
+  f[5] += o24 * (ax - T(2) + (T(3) * om - T(6)) * uy) * Fx - o6 * Fy * ((-T(0.75) * om + T(1.5)) * ux - T(0.5) + by);
+  f[6] += o12 * (T(4) + cy * uy) * Fy + o12 * Fx * ux * dd;
+  f[7] += o24 * (ax + T(2) + (-T(3) * om + T(6)) * uy) * Fx - o6 * Fy * ((T(0.75) * om - T(1.5)) * ux - T(0.5) + by);
+  f[8] += o12 * (T(4) + cy * ux) * Fx + o12 * Fy * uy * dd;
+}
+
 // Moment-space operators (BGK, MRT): node collision in place; rho, j are the pre-collision macros.
 template <class L, int COLL, int DES, class T>
 __device__ __forceinline__ void collide_moment(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
@@ -477,6 +503,10 @@ __device__ __forceinline__ void collide_moment(T (&f)[L::Q], T gamma, const Mode
     static_for<1, L::HALF + 1>([&](auto dc) { p.a[decltype(dc)::value] *= sc; });
   }
   from_pairs<L>(p, f);
+  if constexpr (DES == D_DARCY) {
+    static_assert(L::Q == 9 && CollTraits<COLL>::mrt && CollTraits<COLL>::inc, "DarcyMRT2d is D2Q9 MRT");
+    darcy_force<T>(f, rho, j, gamma, mp);
+  }
   if constexpr (L::QT > 0) thermal_relax<L>(f, j, gamma, mp);
 }
 
@@ -491,6 +521,7 @@ template <class L, int COLL, int DES, class T>
 __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, const ModelParams& mp, T (&lam)[L::Q],
                                                T& dgam, T srcA, const T (&srcB)[3], T srcAT) {
   static_assert(CollTraits<COLL>::inc, "reverse operators exist for the incompressible operators");
+  static_assert(DES != D_DARCY, "DarcyMRT2d has no reverse operator");
   constexpr int Q = L::QF, H = L::HALF;
   Pairs<L, T> pf;
   to_pairs<L>(f, pf);

@@ -20,6 +20,9 @@ LBMX_DECLARE_LATTICE(thermal)
 #undef LBMX_DECLARE_LATTICE
 #undef LBMX_DECLARE_LAUNCHERS
 
+// Zero fill of a device range (bytes a multiple of 4) by a kernel.
+hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
+
 // Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
 hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 hipError_t launch_batch_d2q9_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);

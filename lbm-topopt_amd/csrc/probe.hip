@@ -2,6 +2,8 @@
 // FETCH_SIZE / WRITE_SIZE counters (known bytes read and written per launch).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lbmx.h"
 
 namespace lbmx {
@@ -51,3 +53,23 @@ extern "C" lbmx_status lbmx_device_count(int* count) {
   *count = n;
   return LBMX_OK;
 }
+
+namespace lbmx {
+
+// Zero fill by a kernel (usable inside captured graphs, where memset nodes are avoided).
+__global__ __launch_bounds__(256) void k_fill_zero(unsigned int* __restrict__ p, long long nwords) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) p[i] = 0u;
+}
+
+hipError_t fill_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if (bytes % 4 != 0) return hipErrorInvalidValue;
+  const long long nwords = (long long)(bytes / 4);
+  const long long want = (nwords + 255) / 256;
+  const unsigned blocks = (unsigned)std::min<long long>(want, 65536);
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, s, static_cast<unsigned int*>(p), nwords);
+  return hipGetLastError();
+}
+
+}  // namespace lbmx

@@ -142,6 +142,7 @@ def library():
         "lbmx_batch_destroy": [C.c_void_p],
         "lbmx_batch_checkpoint_config": [C.c_void_p, C.c_int],
         "lbmx_batch_set_tile_rows": [C.c_void_p, C.c_int],
+        "lbmx_batch_set_graph": [C.c_void_p, C.c_int],
         "lbmx_batch_autotune": [C.c_void_p, C.c_int, P(C.c_int), dp],
         "lbmx_batch_gradient": [C.c_void_p, C.c_int, C.c_double, dp, C.c_double, dp, P(dp)],
         "lbmx_batch_get_stats": [C.c_void_p, P(Stats)],
@@ -166,7 +167,7 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_get_sensitivity", "lbmx_get_adjoint", "lbmx_get_stats", "lbmx_device_bytes",
             "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps", "lbmx_stream_copy",
             "lbmx_device_count", "lbmx_batch_create", "lbmx_batch_destroy", "lbmx_batch_checkpoint_config",
-            "lbmx_batch_set_tile_rows", "lbmx_batch_autotune", "lbmx_batch_gradient", "lbmx_batch_get_stats",
+            "lbmx_batch_set_tile_rows", "lbmx_batch_set_graph", "lbmx_batch_autotune", "lbmx_batch_gradient", "lbmx_batch_get_stats",
             "lbmx_batch_device_bytes"]
 
 
@@ -360,6 +361,9 @@ class Batch:
 
     def set_tile_rows(self, rows):
         _check(self.lib.lbmx_batch_set_tile_rows(self.h, rows))
+
+    def set_graph(self, enable=True):
+        _check(self.lib.lbmx_batch_set_graph(self.h, 1 if enable else 0))
 
     def autotune(self, nsweeps=10):
         """Returns (chosen tile rows, ms per forward+adjoint sweep pair for rows 1, 2, 4)."""

@@ -15,6 +15,8 @@
  *   thermal MRT (D2Q5)     : moment-space relaxation with the D2Q5 basis; T conserved
  *   PBB + BP               : closed-form 2x2 pair map; gamma=1 identity; gamma=0 pair mean
  *   Brinkman (Guo forcing) : alpha=0 reduces to the base operator; momentum change = F
+ *   DarcyMRT2d forcing     : mass 0, momentum change F, moment rows (1 - s_k/2) x the Guo
+ *                            source moments (q rows: the full source) -- the polynomials as coded
  *   pull stream            : brute-force permutation on tiny grids
  *   boundary conditions    : prescribed u / rho / T recovered at the node; bounce-back
  *                            against the rule "incoming <- opposite, tangential swapped"
@@ -236,7 +238,8 @@ static double ramp_diff(double p, double ck, double s) {
 orc_problem* orc_create(const lbmx_desc* d) {
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return NULL;
   /* operators restated so far */
-  if (d->design == LBMX_DESIGN_DARCY) return NULL;
+  /* DarcyMRT2d: D2Q9 incompressible MRT only, forward only (its reverse is empty) */
+  if (d->design == LBMX_DESIGN_DARCY && (d->lattice != LBMX_D2Q9 || d->collision != LBMX_MRT)) return NULL;
   /* cascaded: D2Q9 only, coupled to the design through partial bounce-back (or not at all) */
   if (d->collision == LBMX_CASCADED &&
       (d->lattice != LBMX_D2Q9 || d->design == LBMX_DESIGN_BRINKMAN)) return NULL;
@@ -619,6 +622,26 @@ static void casc_reverse(const double* f, double r, double x, double y, double o
   vb[1] += y_b;
 }
 
+/* DarcyMRT2d forcing (src/dynamics/darcyMRT2d.hh:38-58): added to the MRT output, with the
+   unshifted incompressible macros u = j and F = -amax BP(gamma) rho u + body force. */
+static void darcy_forcing(double om, double ux, double uy, double Fx, double Fy, double* f) {
+  const double s1 = 1.4, s2 = 1.4;
+  const double o3 = 1. / 3., o6 = 1. / 6., o12 = 1. / 12., o24 = 1. / 24.;
+  f[0] += o3 * (s1 + s2 - 4.) * (Fx * ux + Fy * uy);
+  f[1] += o24 * ((-4. * s1 + 2. * s2 + 4.) * ux + 2. + (3. * om - 6.) * uy) * Fx -
+          o6 * Fy * ((-0.75 * om + 1.5) * ux + 0.5 + (s1 - 0.5 * s2 - 1.) * uy);
+  f[2] += o12 * (-4. + (s1 - 2. * s2 - 3. * om + 8.) * uy) * Fy + o12 * Fx * ux * (s1 - 2. * s2 + 3. * om - 4.);
+  f[3] += o24 * ((-4. * s1 + 2. * s2 + 4.) * ux - 2. + (-3. * om + 6.) * uy) * Fx -
+          o6 * Fy * ((0.75 * om - 1.5) * ux + 0.5 + (s1 - 0.5 * s2 - 1.) * uy);
+  f[4] += o12 * (-4. + (s1 - 2. * s2 - 3. * om + 8.) * ux) * Fx + o12 * Fy * uy * (s1 - 2. * s2 + 3. * om - 4.);
+  f[5] += o24 * ((-4. * s1 + 2. * s2 + 4.) * ux - 2. + (3. * om - 6.) * uy) * Fx -
+          o6 * Fy * ((-0.75 * om + 1.5) * ux - 0.5 + (s1 - 0.5 * s2 - 1.) * uy);
+  f[6] += o12 * (4. + (s1 - 2. * s2 - 3. * om + 8.) * uy) * Fy + o12 * Fx * ux * (s1 - 2. * s2 + 3. * om - 4.);
+  f[7] += o24 * ((-4. * s1 + 2. * s2 + 4.) * ux + 2. + (-3. * om + 6.) * uy) * Fx -
+          o6 * Fy * ((0.75 * om - 1.5) * ux - 0.5 + (s1 - 0.5 * s2 - 1.) * uy);
+  f[8] += o12 * (4. + (s1 - 2. * s2 - 3. * om + 8.) * ux) * Fx + o12 * Fy * uy * (s1 - 2. * s2 + 3. * om - 4.);
+}
+
 static void base_collide(const orc_problem* p, const double* f, double* out, double rho,
                          const double* v, double alpha, int forced) {
   const int Q = p->Qf;
@@ -716,6 +739,10 @@ void orc_collide_node(const orc_problem* p, const double* fin, double* fout, dou
     double beta = p->incompressible ? 1. / (1. + 0.5 * alpha) : 1. / (rho * (1. + 0.5 * alpha));
     for (int a = 0; a < 3; ++a) v[a] = beta * j[a];
     base_collide(p, fin, fout, rho, v, alpha, 1);
+  } else if (design == LBMX_DESIGN_DARCY) {
+    base_collide(p, fin, fout, rho, j, 0., 0);
+    const double im = -p->d.alpha_max * bp(p->d.interp_q, gamma) * rho;
+    darcy_forcing(p->d.omega, j[0], j[1], im * j[0] + p->d.body_force[0], im * j[1] + p->d.body_force[1], fout);
   } else {
     double v[3];
     for (int a = 0; a < 3; ++a) v[a] = p->incompressible ? j[a] : j[a] / rho;
@@ -1316,7 +1343,7 @@ void orc_forward(const orc_problem* p, double* f, int nsteps, int* t, double* J)
 int orc_gradient(const orc_problem* p, int nsteps, double rho0, const double* u0, double T0,
                  double* J, double* sens, double* adj) {
   const int Q = p->Q;
-  if (!p->incompressible) return LBMX_ERR_SUP;
+  if (!p->incompressible || p->d.design == LBMX_DESIGN_DARCY) return LBMX_ERR_SUP;
   size_t len = p->nodes * Q;
   double* states = (double*)malloc((size_t)(nsteps + 1) * len * sizeof(double));
   if (!states) return LBMX_ERR_MEM;

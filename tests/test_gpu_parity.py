@@ -104,3 +104,25 @@ def test_set_state_roundtrip(lbmx_mod):
     s.forward(10); o.forward(10)
     assert s.timestep() == 35
     assert relmax(s.get_state(), o.f) < POP_TOL
+
+
+def test_forward_darcy_mrt(lbmx_mod):
+    """DarcyMRT2d (forward only, as the reference): drag plus a constant body force."""
+    import oracle
+    import workloads
+    nx, ny = 70, 34
+    kw = dict(lattice=lbmx.D2Q9, collision=lbmx.MRT, design=lbmx.DESIGN_DARCY, omega=1.25, interp_q=0.1,
+              alpha_max=0.05, body_force=(2e-6, -1e-6, 0.0))
+    s = lbmx.Solver(lbmx.make_desc(nx, ny, **kw))
+    o = oracle.Oracle(lbmx.make_desc(nx, ny, **kw))
+    g = workloads.design_field(nx, ny, 1, seed=21)
+    for obj in (s, o):
+        workloads.channel2d(obj, nx, ny, time_const=1.0 / 120)
+        obj.set_design(g)
+    s.init_equilibrium(); o.init_equilibrium()
+    Js = s.forward(120); Jo = o.forward(120)
+    assert relmax(s.get_state(), o.f) < POP_TOL
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+    with pytest.raises(lbmx.LbmxError):
+        s.gradient(5)
+

@@ -318,6 +318,36 @@ def test_brinkman_guo_forcing(lattice, coll):
             assert np.allclose(np.einsum("q,qa,qb->ab", out, c, c), ex, atol=1e-15)
 
 
+def test_darcy_mrt_forcing():
+    """DarcyMRT2d (src/dynamics/darcyMRT2d.hh:33-59): the MRT collision with the unshifted
+    macros u = j, plus a forcing term for F = -alpha_max BP_q(gamma) rho u + body force. Pinned
+    in moment space against the textbook Guo source S = w (3 c.F + 9 (c.u)(c.F) - 3 u.F): the
+    forcing changes mass by 0, momentum by F, and each other Lallemand-Luo row by (1 - s_k/2)
+    times the source moment, except the energy-flux rows q, which receive the full source."""
+    om, amax, q, body = 1.3, 0.4, 0.1, (2e-4, -1e-4, 0.0)
+    d = lbmx.make_desc(4, 4, lattice=lbmx.D2Q9, collision=lbmx.MRT, design=lbmx.DESIGN_DARCY, omega=om,
+                       interp_q=q, alpha_max=amax, body_force=body)
+    o = oracle.Oracle(d)
+    base = mk(lbmx.D2Q9, lbmx.MRT, omega=om)
+    c, _, w = lat(o)
+    rows = basis_d2q9(c)
+    s = {k: _rate(v, om) for k, v in RATES_D2Q9.items()}
+    for g in (0.0, 0.35, 1.0):
+        f = near_eq(c, w)
+        rho, j = f.sum(), c.T @ f
+        add = o.collide_node(f, g)[0] - base.collide_node(f, g)[0]
+        F = -amax * bp_closed(q, g) * rho * j + np.array(body)
+        guo = w * (3 * (c @ F) + 9 * (c @ j) * (c @ F) - 3 * (j @ F))
+        # add is a difference of two O(0.1) outputs: roundoff of a few 1e-17
+        assert abs(add.sum()) < 5e-16
+        assert np.allclose(c.T @ add, F, rtol=0, atol=5e-16)
+        for name, row in rows.items():
+            fac = 1.0 if name in ("qx", "qy") else 1.0 - s[name] / 2
+            assert abs(row @ add - fac * (row @ guo)) < 5e-16, (g, name)
+    with pytest.raises(lbmx.LbmxError):  # the reference has no reverse for this operator
+        o.gradient(3)
+
+
 # ================================================================ thermal (D2Q5 MRT, ramp)
 def test_thermal_mrt_and_ramp():
     D, ck = 0.02, 0.1
