@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lbm-topopt_amd"))
 
@@ -33,10 +35,7 @@ def log(msg):
 
 def bytes_per_lu(form, Q=19):
     """Algorithmic bytes per lattice update (forward, adjoint)."""
-    if form == "fp64":
-        sf, sl, sg = 8, 8, 8
-    else:
-        sf, sl, sg = 2, 4, 4
+    sf, sl, sg = {"fp64": (8, 8, 8), "fp32": (4, 4, 4), "fp16": (2, 4, 4), "bf16": (2, 4, 4)}[form]
     return 2 * Q * sf + sg, Q * sf + 2 * Q * sl + sg + 2 * 8
 
 
@@ -95,6 +94,7 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
         "fwd_bytes_per_lu": bf,
         "adj_bytes_per_lu": ba,
         "objective": J,
+        "_sens": sens,
         "recompute_sweeps_per_gradient": recompute[-1],
         "device_gb": dev_bytes / 1e9,
     }
@@ -123,6 +123,13 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup):
     st = b.stats()
     log(f"ensemble: batched gradient {tb:.2f} s (forward {st['forward_phase_ms'] / 1e3:.2f} s, "
         f"reverse {st['reverse_phase_ms'] / 1e3:.2f} s)")
+    b.set_graph(True)
+    b.gradient(steps, want_sens=False)  # captures the pass into a HIP graph
+    t0 = time.perf_counter()
+    Jg, _ = b.gradient(steps)
+    tg = time.perf_counter() - t0
+    b.set_graph(False)
+    log(f"ensemble: batched gradient replayed from a HIP graph {tg:.2f} s")
     for m in members:
         m.checkpoint_config(slots)
     for _ in range(warmup):
@@ -132,15 +139,16 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup):
     Js = [m.gradient(steps)[0] for m in members]
     ts = time.perf_counter() - t0
     log(f"ensemble: sequential single-instance gradients {ts:.2f} s")
-    jerr = max(abs(a - c) / abs(c) for a, c in zip(Jb, Js))
+    jerr = max(max(abs(a - c) / abs(c) for a, c in zip(Jb, Js)), max(abs(a - c) / abs(c) for a, c in zip(Jg, Js)))
     dev = b.device_bytes()
     b.close()
     for m in members:
         m.close()
     return {"members": len(members), "nodes": nodes, "time_steps": steps, "checkpoints": slots,
             "tile_rows": rows, "autotune_ms_rows_1_2_4": [round(v, 4) for v in tune_ms],
-            "batched_s": round(tb, 3), "sequential_s": round(ts, 3),
+            "batched_s": round(tb, 3), "batched_graph_s": round(tg, 3), "sequential_s": round(ts, 3),
             "batched_mlups": round(nodes * steps / tb / 1e6, 1),
+            "batched_graph_mlups": round(nodes * steps / tg / 1e6, 1),
             "sequential_mlups": round(nodes * steps / ts / 1e6, 1),
             "speedup": round(ts / tb, 3), "max_rel_J_diff_batched_vs_single": jerr,
             "batch_device_gb": round(dev / 1e9, 2)}
@@ -189,7 +197,7 @@ def main():
     ap.add_argument("--slots", type=int, default=48)
     ap.add_argument("--grid", type=str, default="384x192x192")
     ap.add_argument("--sweeps", type=int, default=50)
-    ap.add_argument("--forms", type=str, default="fp64,fp16")
+    ap.add_argument("--forms", type=str, default="fp64,fp16,fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=3000,
                     help="time steps of the batched robust-ensemble measurement (0: skip it)")
@@ -207,6 +215,21 @@ def main():
     for f in forms:
         res[f] = run_form(lbmx, workloads, f, nx, ny, nz, args.horizon, args.slots, args.steps, args.warmup,
                           args.sweeps)
+    # accuracy of the reduced-storage forms against the FP64 plain form on the same workload
+    if "fp64" in res:
+        g64, J64 = res["fp64"]["_sens"], res["fp64"]["objective"]
+        for f in forms:
+            if f == "fp64":
+                continue
+            g, J = res[f]["_sens"], res[f]["objective"]
+            res[f]["vs_fp64"] = {
+                "objective_rel_err": abs(J - J64) / abs(J64),
+                "gradient_rel_err_max_norm": float(np.max(np.abs(g - g64)) / np.max(np.abs(g64))),
+                "gradient_rel_err_l2": float(np.linalg.norm(g - g64) / np.linalg.norm(g64)),
+                "mlups_ratio": res[f]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"]}
+            log(f"{f} vs fp64: {res[f]['vs_fp64']}")
+    for f in forms:
+        res[f].pop("_sens", None)
     head = res.get("fp64", res[forms[0]])
     head_form = "fp64" if "fp64" in res else forms[0]
     traffic = read_traffic(f"{head_form}_fwd_traffic_bytes")
@@ -237,6 +260,8 @@ def main():
     }
     if "fp16" in res and "fp64" in res:
         out["fp16_over_fp64"] = round(res["fp16"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
+    if "fp32" in res and "fp64" in res:
+        out["fp32_over_fp64"] = round(res["fp32"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
     if args.ensemble_steps > 0:
         out["ensemble"] = run_ensemble(lbmx, workloads, args.ensemble_steps, args.ensemble_slots, min(args.warmup, 1))
     out["stream_copy_gbs"] = round(lbmx.stream_copy_gbs(), 1)

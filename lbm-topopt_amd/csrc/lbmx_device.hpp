@@ -208,18 +208,42 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float x) {
   return static_cast<unsigned short>(u >> 16);
 }
 
+// Stochastic rounding of the reduced storage forms. Rounding to nearest commits the same
+// error every step on a nearly steady state, a persistent bias that the conserved modes do
+// not damp; rounding up or down with probability given by the discarded bits makes the error
+// zero-mean. The random bits are a hash of (node, step) spread over the directions, so a
+// recomputed sweep reproduces the stored state exactly.
+__device__ __forceinline__ unsigned int mix32(unsigned int x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// 16 random bits for direction q from the node's two hashed words (a window of one word;
+// the codecs use its low bits)
+template <int q>
+__device__ __forceinline__ unsigned int sr_bits(unsigned int r0, unsigned int r1) {
+  constexpr unsigned int sh = (unsigned int)(((q >> 1) * 3) % 17);
+  return ((q & 1) ? r1 : r0) >> sh;
+}
+
 struct PrecF64 {
   using S = double;   // state storage
   using T = double;   // arithmetic
   using G = double;   // design field storage
   using LS = double;  // adjoint storage
   static constexpr bool shifted = false;
+  static constexpr bool stochastic = false;
   template <class L, int q> __device__ static T dec(S v) { return v; }
   template <class L, int q> __device__ static S enc(T v) { return v; }
 };
 struct PrecF32 {
   using S = float; using T = float; using G = float; using LS = float;
   static constexpr bool shifted = false;
+  static constexpr bool stochastic = false;
   template <class L, int q> __device__ static T dec(S v) { return v; }
   template <class L, int q> __device__ static S enc(T v) { return v; }
 };
@@ -234,6 +258,21 @@ struct PrecF16 {
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
     return static_cast<_Float16>(__builtin_fmaf(v, iw, -1.0f));
   }
+  static constexpr bool stochastic = true;
+  // FP16 keeps 10 of the 23 fraction bits: 13 random bits added to the FP32 pattern, then a
+  // round-toward-zero conversion (two values per instruction) drops them. Exact stochastic
+  // rounding for FP16-normal values; below 2^-14 the conversion truncates to the subnormal grid.
+  template <class L, int q> __device__ static float sr_ready(T v, unsigned int rnd) {
+    constexpr float iw = static_cast<float>(1.0 / L::w[q]);
+    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -1.0f)) + (rnd & 0x1fffu));
+  }
+  __device__ static void pack2(float a, float b, S& ea, S& eb) {
+    // the packed pair read as one 32-bit word (element-wise access of the half vector has
+    // been observed to yield the low half twice)
+    const unsigned int u = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(a, b));
+    ea = __builtin_bit_cast(_Float16, static_cast<unsigned short>(u & 0xffffu));
+    eb = __builtin_bit_cast(_Float16, static_cast<unsigned short>(u >> 16));
+  }
 };
 struct PrecBF16 {
   using S = unsigned short; using T = float; using G = float; using LS = float;
@@ -245,6 +284,16 @@ struct PrecBF16 {
   template <class L, int q> __device__ static S enc(T v) {
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
     return f32_to_bf16(__builtin_fmaf(v, iw, -1.0f));
+  }
+  static constexpr bool stochastic = true;
+  // BF16 keeps the top 16 bits of the FP32 pattern: 16 random bits added, then truncated
+  template <class L, int q> __device__ static float sr_ready(T v, unsigned int rnd) {
+    constexpr float iw = static_cast<float>(1.0 / L::w[q]);
+    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -1.0f)) + (rnd & 0xffffu));
+  }
+  __device__ static void pack2(float a, float b, S& ea, S& eb) {
+    ea = static_cast<unsigned short>(__float_as_uint(a) >> 16);
+    eb = static_cast<unsigned short>(__float_as_uint(b) >> 16);
   }
 };
 

@@ -104,23 +104,57 @@ __device__ __forceinline__ void pull_node(const void* src, const Layout& l, unsi
   });
 }
 
-// store post-collision populations and, at boundary nodes, the halo reflections
+// store post-collision populations and, at boundary nodes, the halo reflections. tstep seeds
+// the stochastic rounding of the reduced storage forms (with the node offset).
 template <class L, class P>
 __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned int off, const typename P::T (&f)[L::Q],
-                                           bool bnd, int x, int y, int z, int nx, int ny, int nz) {
+                                           bool bnd, int x, int y, int z, int nx, int ny, int nz, int tstep) {
   using S = typename P::S;
   const unsigned int ob = off * (unsigned int)sizeof(S);
-  static_for<0, L::Q>([&](auto qc) {
+  unsigned int r0 = 0, r1 = 0;
+  if constexpr (P::stochastic) {
+    const unsigned int tkey = mix32((unsigned int)tstep * 0x9e3779b9u + 0x632be5abu);  // uniform per launch
+    r0 = (off ^ tkey) * 0x9e3779b1u;
+    r0 ^= r0 >> 15;
+    r1 = r0 * 0x85ebca6bu;
+    r1 ^= r1 >> 13;
+  }
+  // encoding of direction q (stochastic rounding: two directions per conversion instruction)
+  auto enc1 = [&](auto qc) -> S {
     constexpr int q = decltype(qc)::value;
-    st<S>(own_base_w<q, S>(dst, l), ob, P::template enc<L, q>(f[q]));
-  });
+    if constexpr (P::stochastic) {
+      S e0, e1;
+      P::pack2(P::template sr_ready<L, q>(f[q], sr_bits<q>(r0, r1)), 0.0f, e0, e1);
+      return e0;
+    } else {
+      return P::template enc<L, q>(f[q]);
+    }
+  };
+  if constexpr (P::stochastic) {
+    static_for<0, (L::Q + 1) / 2>([&](auto pc) {
+      constexpr int q = 2 * decltype(pc)::value;
+      if constexpr (q + 1 < L::Q) {
+        S e0, e1;
+        P::pack2(P::template sr_ready<L, q>(f[q], sr_bits<q>(r0, r1)),
+                 P::template sr_ready<L, q + 1>(f[q + 1], sr_bits<q + 1>(r0, r1)), e0, e1);
+        st<S>(own_base_w<q, S>(dst, l), ob, e0);
+        st<S>(own_base_w<q + 1, S>(dst, l), ob, e1);
+      } else {
+        st<S>(own_base_w<q, S>(dst, l), ob, enc1(IC<q>{}));
+      }
+    });
+  } else {
+    static_for<0, L::Q>([&](auto qc) { st<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc1(qc)); });
+  }
   if (bnd) {
     static_for<1, L::Q>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (L::ex[q] != 0 || L::ey[q] != 0 || L::ez[q] != 0) {
-        // p[q] of this node is what the pull of plane opp(q) at this node reads from x + c_q
-        if (leaves<L, q>(x, y, z, nx, ny, nz))
-          st<S>(pull_base_w<L, L::opp[q], S>(dst, l), ob, P::template enc<L, L::opp[q]>(f[q]));
+        // p[q] of this node is what the pull of plane opp(q) at this node reads from x + c_q;
+        // opposite directions share a weight, so the encoding (recomputed: the rounding bits
+        // are deterministic) is the stored one
+        static_assert(L::w[q] == L::w[L::opp[q]], "opposite directions must share a weight");
+        if (leaves<L, q>(x, y, z, nx, ny, nz)) st<S>(pull_base_w<L, L::opp[q], S>(dst, l), ob, enc1(qc));
       }
     });
   }
@@ -204,7 +238,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
       const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
       T rho, j[3];
       collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
-      store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz);
+      store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
     }
   }
   if (objblk) {
@@ -269,7 +303,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_init(FwdArgs a, const double
     T rho, j[3];
     collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
     store_node<L, P>(a.dst, a.ls, layout_off(a.ls, x, y, z), f, on_boundary<L>(x, y, z, nx, ny, nz), x, y, z, nx,
-                     ny, nz);
+                     ny, nz, 0);
   }
 }
 

@@ -141,3 +141,10 @@ def ensemble_members(make_solver, grids=ENSEMBLE_GRIDS, etas=ENSEMBLE_ETAS, beta
             s.set_design(heaviside_project(raw, beta, eta))
             out.append((s, c, eta))
     return out
+
+
+def config1_channel(obj, nx=1024, ny=512, steps=5000, period=1000.0, amp=0.5):
+    """Unsteady channel: the inlet amplitude is modulated by 1 + amp sin(2 pi t / period)
+    (the functor pattern of applications/RobustPump2d/robustPump2d.cc:14-19); objective: the
+    time-integrated dissipated power (total-pressure flux in minus out), timeConst = 1/steps."""
+    channel2d(obj, nx, ny, objective=lbmx.OBJ_DISSIPATED_POWER, time_const=1.0 / steps, modulation=(period, amp))

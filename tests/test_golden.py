@@ -23,12 +23,13 @@ import make_golden  # noqa: E402
 FILES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
 NAMES = [os.path.splitext(os.path.basename(f))[0] for f in FILES]
 
-# reduced-storage forms: relative error of J and max-norm relative error of dJ/dgamma
-# against the FP64 vectors (FP32 collide and FP64 objective/sensitivity accumulation).
-# FP16 shifted storage is held to the gradient target 1e-4 (measured 5.6e-5 .. 7.4e-5 on
-# these cases); FP32 (unshifted) measured J 2e-6 .. 4e-6, gradient <= 1e-6; BF16 shifted
-# measured J <= 3.3e-4, gradient <= 6.4e-4.
-FORM_TOL = {lbmx.FP32: (1e-5, 5e-6), lbmx.FP16_SHIFTED: (1e-4, 1e-4), lbmx.BF16_SHIFTED: (1e-3, 2e-3)}
+# reduced-storage forms on these short runs (20-40 steps): relative error of J and max-norm
+# relative error of dJ/dgamma against the FP64 vectors (FP32 collide, FP64 objective and
+# sensitivity accumulation, stochastic rounding of the FP16/BF16 storage). Measured: FP32
+# J 1e-6 .. 4e-6, gradient <= 1e-6; FP16 J <= 7e-5, gradient <= 1.3e-4; BF16 J <= 7e-5,
+# gradient <= 8.4e-4. The 1e-4 gradient target of the FP16 form is asserted on the
+# 384 x 192 x 192, 3000-step workload itself (tests/test_gpu_fullsize.py).
+FORM_TOL = {lbmx.FP32: (1e-5, 5e-6), lbmx.FP16_SHIFTED: (2e-4, 3e-4), lbmx.BF16_SHIFTED: (1e-3, 2e-3)}
 
 
 def load(name):

@@ -1,0 +1,125 @@
+"""Full-size checks through size-independent properties (the oracle is too slow at these
+sizes): the configuration-1 gradient against central finite differences on 16 seed-chosen
+cells, mass conservation and schedule invariance at the 384 x 192 x 192 D3Q19 size."""
+import numpy as np
+import pytest
+
+import lbmx
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _config1(precision):
+    nx, ny = 1024, 512
+    d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_BRINKMAN,
+                       precision=precision, omega=1.0 / 0.8, interp_q=0.1, alpha_max=0.1)
+    s = lbmx.Solver(d)
+    workloads.config1_channel(s, nx, ny, steps=5000)
+    g = workloads.design_field(nx, ny, 1, seed=0)
+    s.set_design(g)
+    s.checkpoint_config(64)
+    return s, g
+
+
+def test_config1_gradient_vs_central_differences(lbmx_mod):
+    """1024 x 512 D2Q9 BGK-Brinkman, sinusoidal inlet (period 1000), time-integrated dissipated
+    power, 5000 steps, 64 HBM checkpoints: dJ/dgamma against central differences on 16 cells."""
+    steps = 5000
+    s, g = _config1(lbmx.FP64)
+    J, sens = s.gradient(steps)
+    rng = np.random.default_rng(0)
+    cells = rng.choice(g.size, 16, replace=False)
+    h = 1e-4
+    fd = []
+    for c in cells:
+        vals = []
+        for sgn in (1, -1):
+            gg = g.copy()
+            gg[c] += sgn * h
+            s.set_design(gg)
+            s.init_equilibrium()
+            vals.append(s.forward(steps))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    s.set_design(g)
+    fd = np.array(fd)
+    scale = np.max(np.abs(sens))
+    err = np.max(np.abs(fd - sens[cells])) / scale
+    print(f"config 1: J = {J:.12e}, max |dJ/dgamma| = {scale:.3e}, FD rel. error on 16 cells = {err:.2e}")
+    assert err < 1e-6
+    st = s.stats()
+    assert st["recompute_sweeps"] > 0
+    s.close()
+
+
+def test_config1_fp16_form_gradient_error(lbmx_mod):
+    s64, _ = _config1(lbmx.FP64)
+    J64, g64 = s64.gradient(5000)
+    s64.close()
+    s16, _ = _config1(lbmx.FP16_SHIFTED)
+    J16, g16 = s16.gradient(5000)
+    s16.close()
+    ej = abs(J16 - J64) / abs(J64)
+    eg = np.max(np.abs(g16 - g64)) / np.max(np.abs(g64))
+    print(f"config 1 FP16-shifted form: J rel. error {ej:.2e}, gradient rel. max-norm error {eg:.2e}")
+    assert ej < 1e-3 and eg < 1e-2
+
+
+def _duct(nsteps_obj=1.0, bcs=True, precision=lbmx.FP64):
+    nx, ny, nz = 384, 192, 192
+    d = lbmx.make_desc(nx, ny, nz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
+                       precision=precision, omega=1.6, interp_q=0.1, alpha_max=0.1)
+    s = lbmx.Solver(d)
+    if bcs:
+        workloads.duct3d(s, nx, ny, nz, u_in=0.04, time_const=nsteps_obj)
+    s.set_design(workloads.design_field(nx, ny, nz, seed=0))
+    return s
+
+
+def test_fullsize_d3q19_mass_conservation(lbmx_mod):
+    """Without boundary pieces the domain edges reflect (mid-way bounce-back of the pull) and
+    the collision, Brinkman drag included, conserves mass: total mass is invariant."""
+    s = _duct(bcs=False)
+    s.init_equilibrium(1.0, (0.03, 0.01, -0.02))
+    m0 = s.get_macros().reshape(-1, 5)[:, 0].sum()
+    s.forward(100)
+    m1 = s.get_macros().reshape(-1, 5)[:, 0].sum()
+    print(f"total mass {m0:.15e} -> {m1:.15e}")
+    assert abs(m1 - m0) <= 1e-12 * abs(m0)
+    s.close()
+
+
+def test_fullsize_d3q19_checkpointed_equals_stored(lbmx_mod):
+    """At 384 x 192 x 192: the gradient with recomputation (5 slots) is bitwise equal to the
+    one that keeps every state (30 slots >= 30 steps), and repeatable."""
+    steps = 30
+    s = _duct(1.0 / steps)
+    s.checkpoint_config(30)
+    Ja, ga = s.gradient(steps)
+    s.checkpoint_config(5)
+    Jb, gb = s.gradient(steps)
+    Jc, gc = s.gradient(steps)
+    assert s.stats()["recompute_sweeps"] > 0
+    assert Ja == Jb == Jc
+    assert np.array_equal(ga, gb) and np.array_equal(gb, gc)
+    s.close()
+
+
+def test_config3_fp16_form_meets_gradient_target(lbmx_mod):
+    """The FP16-shifted storage form (FP32 collide, stochastic rounding, FP64 objective and
+    sensitivity) on the 384 x 192 x 192 D3Q19 MRT Brinkman duct, 3000 steps, 48 checkpoints:
+    relative gradient error <= 1e-4 against the FP64 plain form (max-norm)."""
+    steps = 3000
+    out = {}
+    for prec in (lbmx.FP64, lbmx.FP16_SHIFTED):
+        s = _duct(1.0 / steps, precision=prec)
+        s.checkpoint_config(48)
+        out[prec] = s.gradient(steps)
+        s.close()
+    (J64, g64), (J16, g16) = out[lbmx.FP64], out[lbmx.FP16_SHIFTED]
+    ej = abs(J16 - J64) / abs(J64)
+    eg = np.max(np.abs(g16 - g64)) / np.max(np.abs(g64))
+    e2 = np.linalg.norm(g16 - g64) / np.linalg.norm(g64)
+    print(f"config 3 FP16 form: J rel. error {ej:.2e}, gradient rel. error max-norm {eg:.2e}, L2 {e2:.2e}")
+    assert eg <= 1e-4 and ej <= 1e-4
+
