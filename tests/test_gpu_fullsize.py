@@ -62,7 +62,7 @@ def test_config1_fp16_form_gradient_error(lbmx_mod):
     ej = abs(J16 - J64) / abs(J64)
     eg = np.max(np.abs(g16 - g64)) / np.max(np.abs(g64))
     print(f"config 1 FP16-shifted form: J rel. error {ej:.2e}, gradient rel. max-norm error {eg:.2e}")
-    assert ej < 1e-3 and eg < 1e-2
+    assert ej < 1e-4 and eg < 2e-4  # measured 1.3e-5 and 9.7e-5
 
 
 def _duct(nsteps_obj=1.0, bcs=True, precision=lbmx.FP64):
