@@ -169,23 +169,43 @@ def read_traffic(key):
 
 
 def cpu_baseline(workloads, lbmx, nx=96, ny=48, nz=48, horizon=30):
-    """The CPU oracle (port of the reference's algorithm) on a bounded sample of the workload."""
+    """The CPU oracle (port of the reference's algorithm) on a bounded sample of the workload,
+    on the host's cores (up to 16) and on one core (a smaller sample)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: E402  (baseline leg only)
+
+    def run(threads, gx, gy, gz):
+        d = lbmx.make_desc(gx, gy, gz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
+                           omega=1.6, interp_q=0.1, alpha_max=0.1)
+        o = oracle.Oracle(d, threads=threads)
+        workloads.duct3d(o, gx, gy, gz, u_in=0.04, time_const=1.0 / horizon)
+        o.set_design(workloads.design_field(gx, gy, gz, seed=0))
+        t0 = time.perf_counter()
+        o.gradient(horizon)
+        dt = time.perf_counter() - t0
+        return gx * gy * gz * horizon / dt / 1e6, dt
+
     threads = min(16, os.cpu_count() or 1)
-    d = lbmx.make_desc(nx, ny, nz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
-                       omega=1.6, interp_q=0.1, alpha_max=0.1)
-    o = oracle.Oracle(d, threads=threads)
-    workloads.duct3d(o, nx, ny, nz, u_in=0.04, time_const=1.0 / horizon)
-    o.set_design(workloads.design_field(nx, ny, nz, seed=0))
-    t0 = time.perf_counter()
-    o.gradient(horizon)
-    dt = time.perf_counter() - t0
-    v = nx * ny * nz * horizon / dt / 1e6
+    v, dt = run(threads, nx, ny, nz)
     log(f"cpu baseline: {v:.3f} MLUPS forward+adjoint on {threads} threads ({dt:.1f} s)")
+    v1, dt1 = run(1, nx // 2, ny // 2, nz // 2)
+    log(f"cpu baseline: {v1:.3f} MLUPS forward+adjoint on 1 thread ({dt1:.1f} s)")
     return {"value": v, "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"D3Q19 MRT Brinkman duct {nx}x{ny}x{nz}, {horizon} steps forward+adjoint "
-                      f"(every state stored), FP64, OpenMP over nodes"}
+                      f"(every state stored), FP64, OpenMP over nodes",
+            "single_core": {"value": v1, "sample": f"{nx // 2}x{ny // 2}x{nz // 2}, {horizon} steps"},
+            "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():

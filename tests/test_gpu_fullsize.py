@@ -123,3 +123,55 @@ def test_config3_fp16_form_meets_gradient_target(lbmx_mod):
     print(f"config 3 FP16 form: J rel. error {ej:.2e}, gradient rel. error max-norm {eg:.2e}, L2 {e2:.2e}")
     assert eg <= 1e-4 and ej <= 1e-4
 
+
+def test_config0_parity_full_size(lbmx_mod):
+    """Configuration 0 at its full size: D2Q9 BGK with Brinkman porosity (alpha_max = 0.1,
+    q = 0.1), 256 x 128 channel, tau = 0.8, 2000 forward steps from rest, against the oracle."""
+    import os
+    import oracle
+    nx, ny, steps = 256, 128, 2000
+    kw = dict(lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_BRINKMAN, omega=1.0 / 0.8, interp_q=0.1,
+              alpha_max=0.1)
+    s = lbmx.Solver(lbmx.make_desc(nx, ny, **kw))
+    o = oracle.Oracle(lbmx.make_desc(nx, ny, **kw), threads=min(16, os.cpu_count() or 1))
+    g = workloads.design_field(nx, ny, 1, seed=0)
+    for obj in (s, o):
+        workloads.channel2d(obj, nx, ny, time_const=1.0 / steps)
+        obj.set_design(g)
+    s.init_equilibrium(); o.init_equilibrium()
+    Js = s.forward(steps); Jo = o.forward(steps)
+    f = s.get_state()
+    err = float(np.max(np.abs(f - o.f)) / np.max(np.abs(o.f)))
+    print(f"config 0: populations rel. max-norm error {err:.2e}, J {Js:.15e} vs {Jo:.15e}")
+    assert err < 1e-12 and abs(Js - Jo) <= 1e-10 * abs(Jo)
+    s.close()
+
+
+def test_config2_thermal_forms_full_size(lbmx_mod):
+    """Configuration 2 at full size: D2Q9 MRT + D2Q5 MRT thermal, 2048 x 1024, 4000 steps,
+    k_s/k_f = 10 (ramp C_k = 0.1), hot inlet T = 1, cold walls T = 0, outlet heat flux; the
+    BF16- and FP16-shifted storage forms against the FP64 form. The run starts from a moving,
+    uniformly warm state (u = 0.03, T = 0.5): from rest, heat from the inlet does not reach
+    the outlet 2047 cells away within 4000 steps and the objective would be zero."""
+    nx, ny, steps = 2048, 1024, 4000
+    out = {}
+    for prec in (lbmx.FP64, lbmx.FP16_SHIFTED, lbmx.BF16_SHIFTED):
+        d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9_D2Q5, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
+                           precision=prec, omega=1.25, interp_q=0.1, alpha_max=0.1, thermal_diffusivity=0.02,
+                           ramp_p=1.0, ramp_ck=0.1)
+        s = lbmx.Solver(d)
+        workloads.thermal_channel2d(s, nx, ny, time_const=1.0 / steps)
+        s.set_design(workloads.design_field(nx, ny, 1, seed=0))
+        s.checkpoint_config(64)
+        out[prec] = s.gradient(steps, u0=(0.03, 0.0, 0.0), T0=0.5)
+        s.close()
+    J64, g64 = out[lbmx.FP64]
+    assert np.isfinite(J64) and abs(J64) > 0
+    for prec, (tj, tg) in ((lbmx.FP16_SHIFTED, (1e-3, 1e-3)), (lbmx.BF16_SHIFTED, (1e-2, 1e-2))):
+        J, g = out[prec]
+        ej = abs(J - J64) / abs(J64)
+        eg = np.max(np.abs(g - g64)) / np.max(np.abs(g64))
+        e2 = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+        print(f"config 2 precision {prec}: J rel. error {ej:.2e}, gradient rel. error max-norm {eg:.2e}, L2 {e2:.2e}")
+        assert ej <= tj and eg <= tg
+
