@@ -282,6 +282,66 @@ lbmx_status lbmx_batch_gradient(lbmx_batch* b, int nsteps, double rho0, const do
 lbmx_status lbmx_batch_get_stats(lbmx_batch* b, lbmx_stats* st);
 lbmx_status lbmx_batch_device_bytes(lbmx_batch* b, size_t* bytes);
 
+/* ---- design pipeline: the steps either side of gamma and dJ/dgamma ----
+   Design-box mapping (FullToDesignMapping2d/3d, src/topOpt/fullToDesignMapping2d.cc:3-151,
+   fullToDesignMapping3d.cc), cone density filter (DensityFilter2d/3d,
+   src/topOpt/densityFilter2d.cc:3-146, densityFilter3d.cc:60-139), tanh projection
+   (ProjectionFilter2d/3d, src/topOpt/projectionFilter2d.cc:27-87, projectionFilter3d.hh) and
+   the two-stage chain (DoubleProjectionFilter2d, src/topOpt/doubleProjectionFilter2d.cc:28-48),
+   on the device. Fields are FP64 in natural order; the full grid is non-periodic.
+
+   Density filter of one stage with radius R: a row inside filter_box holds the weights
+   R - |d| of the in-grid neighbours at offsets d with |d| <= R; every other row is the
+   identity; the result is (W x) / (W 1). Its sensitivity map is W^T (s / (W 1)). A projecting
+   stage then maps the filtered value x inside filter_box to
+   (tanh(beta eta) + tanh(beta (x - eta))) / (tanh(beta eta) + tanh(beta (1 - eta))). */
+#define LBMX_FILTER_MAX_RADIUS 16
+typedef struct {
+  int nx, ny, nz;          /* full lattice grid (nz = 1 in 2D) */
+  lbmx_box design_box;     /* optimisation variables: natural order over this box */
+  lbmx_box filter_box;     /* rows that are filtered and projected */
+  int nstages;             /* 1 (DensityFilter / ProjectionFilter) or 2 (DoubleProjectionFilter) */
+  int radius[2];           /* cone radius per stage, 1..LBMX_FILTER_MAX_RADIUS */
+  int project[2];          /* nonzero: the stage projects after its density filter */
+  double beta[2], eta[2];  /* projection sharpness and threshold per stage */
+} lbmx_filter_desc;
+typedef struct lbmx_filter lbmx_filter;
+/* DensityFilter2d::DensityFilter2d + setUp, ProjectionFilter2d::ProjectionFilter2d,
+   DoubleProjectionFilter2d::DoubleProjectionFilter2d, FullToDesignMapping2d::FullToDesignMapping2d.
+   Allocates on the current device; the background field starts at 1 (fluid). */
+lbmx_status lbmx_filter_create(const lbmx_filter_desc* desc, lbmx_filter** out);
+lbmx_status lbmx_filter_destroy(lbmx_filter* f);
+/* ProjectionFilter2d::setBeta/setEta (projectionFilter2d.hh:121-122),
+   DoubleProjectionFilter2d::setFirst/SecondBeta/Eta (doubleProjectionFilter2d.hh:73-76) */
+lbmx_status lbmx_filter_set_projection(lbmx_filter* f, int stage, double beta, double eta);
+/* increaseBeta: every stage's beta *= factor (doubleProjectionFilter2d.hh:77-81) */
+lbmx_status lbmx_filter_increase_beta(lbmx_filter* f, double factor);
+/* Full-grid values outside design_box, n = nx*ny*nz: LBSolver::createDesignVec copies the
+   solver's material field (src/latticeBoltzmann/LBSolver.cc:318-331) before
+   mapDesignToFull inserts the design box. */
+lbmx_status lbmx_filter_set_background(lbmx_filter* f, const double* full, size_t n);
+/* mapDesignToFull + filterDesign: design (nd = design-box nodes) -> physical field (nf = full
+   nodes; physical may be NULL). Caches each stage's density-filtered field for the next
+   sensitivity call, as ProjectionFilter2d keeps densityFilteredField. */
+lbmx_status lbmx_filter_design(lbmx_filter* f, const double* design, size_t nd, double* physical, size_t nf);
+/* The same, then LBSolver::setFieldFromDesignVec (LBSolver.cc:333-348) on the device: the
+   solver's design field is written in HBM without a host round trip. */
+lbmx_status lbmx_filter_design_to_solver(lbmx_filter* f, const double* design, size_t nd, lbmx_solver* s);
+/* filterSensitivities + mapFullToDesign for one full-grid field (objective or constraint):
+   stages in reverse order, each the projection derivative at the cached filtered field, then
+   the transposed density filter; out = the design-box entries (nd). */
+lbmx_status lbmx_filter_sensitivities(lbmx_filter* f, const double* full_sens, size_t nf, double* design_sens,
+                                      size_t nd);
+/* The same on the solver's device-resident dJ/dgamma of its last lbmx_gradient. */
+lbmx_status lbmx_filter_solver_sensitivities(lbmx_filter* f, lbmx_solver* s, double* design_sens, size_t nd);
+/* Intermediate fields of the last lbmx_filter_design (outputIntermediateFields,
+   projectionFilter2d.cc:89-103): which = 2*stage for the stage's density-filtered field,
+   2*stage + 1 for its output; n = full nodes. */
+lbmx_status lbmx_filter_get_field(lbmx_filter* f, int which, double* out, size_t n);
+/* Milliseconds per call of the last lbmx_filter_design / sensitivity call on the device
+   (HIP events around its kernels): ms[0] design, ms[1] sensitivities. */
+lbmx_status lbmx_filter_timing(lbmx_filter* f, double ms[2]);
+
 #ifdef __cplusplus
 }
 #endif

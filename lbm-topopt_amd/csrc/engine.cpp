@@ -15,25 +15,12 @@
 #include <vector>
 
 #include "checkpoint.hpp"
+#include "host_common.hpp"
 #include "lbmx.h"
 #include "lbmx_common.h"
 #include "lbmx_launch.h"
 
 namespace lbmx {
-
-static thread_local std::string g_last_error;
-
-struct Error {
-  int code;
-  std::string msg;
-};
-
-#define HIP_OK(expr)                                                                         \
-  do {                                                                                       \
-    hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess)                                                                    \
-      throw Error{LBMX_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)};       \
-  } while (0)
 
 static size_t prec_state_bytes(int prec) {
   switch (prec) {
@@ -44,30 +31,6 @@ static size_t prec_state_bytes(int prec) {
 }
 static size_t prec_lam_bytes(int prec) { return prec == LBMX_FP64 ? 8 : 4; }
 static size_t prec_gamma_bytes(int prec) { return prec == LBMX_FP64 ? 8 : 4; }
-
-struct DeviceBuffer {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DeviceBuffer() = default;
-  DeviceBuffer(const DeviceBuffer&) = delete;
-  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-  ~DeviceBuffer() { reset(); }
-  void reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  void ensure(size_t b) {
-    if (bytes >= b && p) return;
-    reset();
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) {
-      p = nullptr;
-      throw Error{LBMX_ERR_MEM, "hipMalloc of " + std::to_string(b) + " bytes failed: " + hipGetErrorString(e)};
-    }
-    bytes = b;
-  }
-};
 
 // zero fill by a kernel (also valid inside a captured graph)
 static void zero_async(void* p, size_t bytes, hipStream_t s) { HIP_OK(fill_zero(p, bytes, s)); }
@@ -163,11 +126,13 @@ struct Engine {
   bool terms_dirty = true;
   Layout ls{}, ll{};                // padded layouts of state and adjoint fields
   std::vector<double> gamma_host;   // design field
-  bool gamma_dirty = true;
+  bool gamma_dirty = true;          // host copy newer than the device copy
+  bool gamma_host_stale = false;    // device copy newer than the host copy (set on the device)
   ObjArgs obj{};                    // acc pointer filled at launch
   bool obj_set = false;
 
   // device
+  DeviceBuffer gamma64_dev;  // FP64 copy of a design set on the device, for the FP32-design forms
   DeviceBuffer gamma_dev, bc_dev, term_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
   std::vector<std::unique_ptr<DeviceBuffer>> pool;  // state buffers: [0],[1] working; rest slots
   bool bc_dirty = true;
@@ -423,6 +388,33 @@ struct Engine {
     if ((long long)n != N) throw Error{LBMX_ERR_ARG, "design vector length != number of nodes"};
     std::copy(g, g + n, gamma_host.begin());
     gamma_dirty = true;
+    gamma_host_stale = false;
+  }
+
+  // design from a device FP64 array (natural order), ordered on this engine's stream
+  void set_design_device(const double* dev) {
+    touched = true;
+    gamma_dev.ensure((size_t)N * prec_gamma_bytes(d.precision));
+    if (d.precision == LBMX_FP64) {
+      HIP_OK(hipMemcpyAsync(gamma_dev.p, dev, (size_t)N * 8, hipMemcpyDeviceToDevice, stream));
+    } else {
+      gamma64_dev.ensure((size_t)N * 8);
+      HIP_OK(hipMemcpyAsync(gamma64_dev.p, dev, (size_t)N * 8, hipMemcpyDeviceToDevice, stream));
+      HIP_OK(convert_f64_f32(static_cast<float*>(gamma_dev.p), dev, N, stream));
+    }
+    gamma_dirty = false;
+    gamma_host_stale = true;
+  }
+
+  void get_design(double* g, size_t n) {
+    if ((long long)n != N) throw Error{LBMX_ERR_ARG, "design vector length != number of nodes"};
+    if (gamma_host_stale) {
+      const void* src = d.precision == LBMX_FP64 ? gamma_dev.p : gamma64_dev.p;
+      HIP_OK(hipStreamSynchronize(stream));
+      HIP_OK(hipMemcpy(gamma_host.data(), src, (size_t)N * 8, hipMemcpyDeviceToHost));
+      gamma_host_stale = false;
+    }
+    std::copy(gamma_host.begin(), gamma_host.end(), g);
   }
 
   // ---- device preparation ----
@@ -847,7 +839,7 @@ struct Engine {
   }
 
   size_t device_bytes() const {
-    size_t b = gamma_dev.bytes + bc_dev.bytes + term_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
+    size_t b = gamma64_dev.bytes + gamma_dev.bytes + bc_dev.bytes + term_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
                lam_b.bytes + raw_aos.bytes;
     for (auto& p : pool) b += p->bytes;
     return b;
@@ -1220,22 +1212,34 @@ struct lbmx_batch {
   std::unique_ptr<lbmx::Batch> b;
 };
 
-template <class F>
-static lbmx_status guarded(F&& f) {
-  try {
-    f();
-    return LBMX_OK;
-  } catch (const Error& e) {
-    lbmx::g_last_error = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    lbmx::g_last_error = "host allocation failed";
-    return LBMX_ERR_MEM;
-  } catch (const std::exception& e) {
-    lbmx::g_last_error = e.what();
-    return LBMX_ERR_STATE;
-  }
+using lbmx::guarded;
+
+namespace lbmx {
+
+static Engine& engine_of(lbmx_solver* s) {
+  if (!s || !s->e) throw Error{LBMX_ERR_ARG, "null solver handle"};
+  return *s->e;
 }
+
+hipStream_t solver_stream(lbmx_solver* s) { return engine_of(s).stream; }
+
+long long solver_nodes(lbmx_solver* s, int dims[3]) {
+  Engine& e = engine_of(s);
+  dims[0] = e.d.nx;
+  dims[1] = e.d.ny;
+  dims[2] = e.d.nz;
+  return e.N;
+}
+
+void solver_set_design_device(lbmx_solver* s, const double* dev) { engine_of(s).set_design_device(dev); }
+
+const double* solver_sensitivity_device(lbmx_solver* s) {
+  Engine& e = engine_of(s);
+  if (!e.grad_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
+  return static_cast<const double*>(e.sens_dev.p);
+}
+
+}  // namespace lbmx
 
 #define CHECK_HANDLE(s)                                      \
   if (!(s) || !(s)->e) {                                     \
@@ -1291,10 +1295,7 @@ lbmx_status lbmx_set_design(lbmx_solver* s, const double* gamma, size_t n) {
 lbmx_status lbmx_get_design(lbmx_solver* s, double* gamma, size_t n) {
   CHECK_HANDLE(s);
   if (!gamma) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
-  return guarded([&] {
-    if ((long long)n != s->e->N) throw Error{LBMX_ERR_ARG, "design vector length != number of nodes"};
-    std::copy(s->e->gamma_host.begin(), s->e->gamma_host.end(), gamma);
-  });
+  return guarded([&] { s->e->get_design(gamma, n); });
 }
 
 lbmx_status lbmx_set_objective(lbmx_solver* s, const lbmx_objective* obj) {

@@ -23,6 +23,35 @@ LBMX_DECLARE_LATTICE(thermal)
 // Zero fill of a device range (bytes a multiple of 4) by a kernel.
 hipError_t fill_zero(void* p, size_t bytes, hipStream_t s);
 
+// FP64 -> FP32 conversion of n values (design fields of the reduced-storage forms).
+hipError_t convert_f64_f32(float* out, const double* in, long long n, hipStream_t s);
+
+// Design pipeline kernels (natural-order FP64 fields on the full grid).
+struct ConeArgs {
+  const double* in;    // field to filter (forward) or s / (W 1) (transposed)
+  double* out;
+  const double* den;   // forward: divide by den inside the box (nullptr: no division)
+  const double* wt;    // weights [(dz+R)(2R+1) + dy+R][dx+R], (2R+1)^3 entries (2D: dz = 0 only)
+  const int* rx;       // per (dz, dy): largest |dx| with dx^2+dy^2+dz^2 <= R^2, or -1
+  int nx, ny, nz, R;
+  int box[6];          // filter box, inclusive, clipped to the grid
+  int transposed;      // 0: out = W in (rows in the box), 1: out = W^T in
+};
+hipError_t launch_cone(const ConeArgs& a, hipStream_t s);
+// out[i] = inside design box ? design[box-local i] : background[i]
+hipError_t launch_map_to_full(double* out, const double* design, const double* background, int nx, int ny, int nz,
+                              const int box[6], hipStream_t s);
+// design[box-local i] = full[i] over the design box
+hipError_t launch_extract_box(double* design, const double* full, int nx, int ny, int nz, const int box[6],
+                              hipStream_t s);
+// out = inside box ? (tb + tanh(beta (x - eta))) / tsum : x
+hipError_t launch_project(double* out, const double* x, int nx, int ny, int nz, const int box[6], double beta,
+                          double eta, double tb, double tsum, hipStream_t s);
+// out = (inside box && project ? sens * (1 - tanh(beta (x - eta))^2) beta / tsum : sens) / den
+hipError_t launch_sens_scale(double* out, const double* sens, const double* x, const double* den, int nx, int ny,
+                             int nz, const int box[6], int project, double beta, double eta, double tsum,
+                             hipStream_t s);
+
 // Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
 hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 hipError_t launch_batch_d2q9_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);

@@ -55,6 +55,16 @@ class Objective(C.Structure):
                 ("box_a", Box), ("box_b", Box)]
 
 
+class FilterDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("design_box", Box), ("filter_box", Box), ("nstages", C.c_int),
+                ("radius", C.c_int * 2), ("project", C.c_int * 2),
+                ("beta", C.c_double * 2), ("eta", C.c_double * 2)]
+
+
+FILTER_MAX_RADIUS = 16
+
+
 class Stats(C.Structure):
     _fields_ = [("fwd_kernel_ms", C.c_double), ("adj_kernel_ms", C.c_double),
                 ("forward_phase_ms", C.c_double), ("reverse_phase_ms", C.c_double),
@@ -149,6 +159,17 @@ def library():
         "lbmx_batch_device_bytes": [C.c_void_p, P(C.c_size_t)],
         "lbmx_describe_boundaries": [C.c_void_p, P(C.c_int), C.c_int, P(C.c_int)],
         "lbmx_checkpoint_simulate": [C.c_int, C.c_int, P(C.c_longlong), P(C.c_int)],
+        "lbmx_filter_create": [P(FilterDesc), P(C.c_void_p)],
+        "lbmx_filter_destroy": [C.c_void_p],
+        "lbmx_filter_set_projection": [C.c_void_p, C.c_int, C.c_double, C.c_double],
+        "lbmx_filter_increase_beta": [C.c_void_p, C.c_double],
+        "lbmx_filter_set_background": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_filter_design": [C.c_void_p, dp, C.c_size_t, dp, C.c_size_t],
+        "lbmx_filter_design_to_solver": [C.c_void_p, dp, C.c_size_t, C.c_void_p],
+        "lbmx_filter_sensitivities": [C.c_void_p, dp, C.c_size_t, dp, C.c_size_t],
+        "lbmx_filter_solver_sensitivities": [C.c_void_p, C.c_void_p, dp, C.c_size_t],
+        "lbmx_filter_get_field": [C.c_void_p, C.c_int, dp, C.c_size_t],
+        "lbmx_filter_timing": [C.c_void_p, dp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -168,7 +189,10 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps", "lbmx_stream_copy",
             "lbmx_device_count", "lbmx_batch_create", "lbmx_batch_destroy", "lbmx_batch_checkpoint_config",
             "lbmx_batch_set_tile_rows", "lbmx_batch_set_graph", "lbmx_batch_autotune", "lbmx_batch_gradient", "lbmx_batch_get_stats",
-            "lbmx_batch_device_bytes"]
+            "lbmx_batch_device_bytes", "lbmx_filter_create", "lbmx_filter_destroy", "lbmx_filter_set_projection",
+            "lbmx_filter_increase_beta", "lbmx_filter_set_background", "lbmx_filter_design",
+            "lbmx_filter_design_to_solver", "lbmx_filter_sensitivities", "lbmx_filter_solver_sensitivities",
+            "lbmx_filter_get_field", "lbmx_filter_timing"]
 
 
 def checkpoint_simulate(nslots, nsteps):
@@ -392,3 +416,86 @@ class Batch:
         _check(self.lib.lbmx_batch_device_bytes(self.h, C.byref(b)))
         return b.value
 
+
+
+def make_filter_desc(nx, ny, nz, design_box, filter_box=None, stages=((8, True, 1.0, 0.5),)):
+    """lbmx_filter_desc for a chain of (radius, project, beta, eta) stages (1 or 2)."""
+    d = FilterDesc()
+    d.nx, d.ny, d.nz = nx, ny, nz
+    d.design_box = design_box
+    d.filter_box = filter_box if filter_box is not None else design_box
+    d.nstages = len(stages)
+    for k, (r, proj, beta, eta) in enumerate(stages):
+        d.radius[k] = r
+        d.project[k] = 1 if proj else 0
+        d.beta[k] = beta
+        d.eta[k] = eta
+    return d
+
+
+class Filter:
+    """Design pipeline on the device: design-box mapping, density filter and tanh projection
+    stages (DensityFilter2d/3d, ProjectionFilter2d/3d, DoubleProjectionFilter2d) and their
+    sensitivity map. Fields are FP64 numpy arrays in natural order."""
+
+    def __init__(self, desc):
+        self.lib = library()
+        self.desc = desc
+        self.h = C.c_void_p()
+        _check(self.lib.lbmx_filter_create(C.byref(desc), C.byref(self.h)))
+        self.n_full = desc.nx * desc.ny * desc.nz
+        b = desc.design_box
+        self.n_design = (b.x1 - b.x0 + 1) * (b.y1 - b.y0 + 1) * (b.z1 - b.z0 + 1)
+
+    def close(self):
+        if self.h:
+            self.lib.lbmx_filter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_projection(self, stage, beta, eta):
+        _check(self.lib.lbmx_filter_set_projection(self.h, stage, beta, eta))
+
+    def increase_beta(self, factor):
+        _check(self.lib.lbmx_filter_increase_beta(self.h, factor))
+
+    def set_background(self, full):
+        a = np.ascontiguousarray(full, dtype=np.float64)
+        _check(self.lib.lbmx_filter_set_background(self.h, _dptr(a), a.size))
+
+    def design(self, design):
+        """Physical (filtered and projected) full-grid field of a design-box vector."""
+        d = np.ascontiguousarray(design, dtype=np.float64)
+        out = np.empty(self.n_full)
+        _check(self.lib.lbmx_filter_design(self.h, _dptr(d), d.size, _dptr(out), out.size))
+        return out
+
+    def design_to_solver(self, design, solver):
+        d = np.ascontiguousarray(design, dtype=np.float64)
+        _check(self.lib.lbmx_filter_design_to_solver(self.h, _dptr(d), d.size, solver.h))
+
+    def sensitivities(self, full_sens):
+        s = np.ascontiguousarray(full_sens, dtype=np.float64)
+        out = np.empty(self.n_design)
+        _check(self.lib.lbmx_filter_sensitivities(self.h, _dptr(s), s.size, _dptr(out), out.size))
+        return out
+
+    def solver_sensitivities(self, solver):
+        out = np.empty(self.n_design)
+        _check(self.lib.lbmx_filter_solver_sensitivities(self.h, solver.h, _dptr(out), out.size))
+        return out
+
+    def field(self, which):
+        out = np.empty(self.n_full)
+        _check(self.lib.lbmx_filter_get_field(self.h, which, _dptr(out), out.size))
+        return out
+
+    def timing(self):
+        ms = (C.c_double * 2)()
+        _check(self.lib.lbmx_filter_timing(self.h, ms))
+        return ms[0], ms[1]

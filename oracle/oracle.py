@@ -57,6 +57,13 @@ def library():
         "orc_forward": ([vp, dp, C.c_int, ip, dp], None),
         "orc_gradient": ([vp, C.c_int, C.c_double, dp, C.c_double, dp, dp, dp], C.c_int),
         "orc_macros": ([vp, dp, dp], None),
+        "odf_denominator": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, dp], None),
+        "odf_density_forward": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, dp], None),
+        "odf_density_transpose": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, dp], None),
+        "odf_project": ([C.c_int, C.c_int, C.c_int, ip, C.c_double, C.c_double, dp, dp], None),
+        "odf_project_sens": ([C.c_int, C.c_int, C.c_int, ip, C.c_double, C.c_double, dp, dp], None),
+        "odf_map_to_full": ([C.c_int, C.c_int, C.c_int, ip, dp, dp, dp], None),
+        "odf_extract": ([C.c_int, C.c_int, C.c_int, ip, dp, dp], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -199,3 +206,83 @@ class Oracle:
         m = np.empty(self.nodes * 5)
         self.lib.orc_macros(self.h, _d(f), _d(m))
         return m.reshape(self.nodes, 5)
+
+
+def _box6(b):
+    """(x0, x1, y0, y1, z0, z1) from an lbmx.Box or a 6-sequence, as a C int array."""
+    if hasattr(b, "x0"):
+        b = (b.x0, b.x1, b.y0, b.y1, b.z0, b.z1)
+    return (C.c_int * 6)(*[int(v) for v in b])
+
+
+class FilterChain:
+    """Design pipeline restated on the CPU: FullToDesignMapping2d + DensityFilter2d/3d and
+    ProjectionFilter2d stages (src/topOpt/fullToDesignMapping2d.cc, densityFilter2d.cc,
+    projectionFilter2d.cc, doubleProjectionFilter2d.cc). stages: (radius, project, beta, eta)."""
+
+    def __init__(self, nx, ny, nz, design_box, filter_box=None, stages=((8, True, 1.0, 0.5),), background=None):
+        self.lib = library()
+        self.dims = (nx, ny, nz)
+        self.n = nx * ny * nz
+        self.dbox = _box6(design_box)
+        self.fbox = _box6(filter_box if filter_box is not None else design_box)
+        self.stages = [list(s) for s in stages]
+        self.background = np.ones(self.n) if background is None else np.array(background, dtype=np.float64)
+        d = list(self.dbox)
+        self.nd = (d[1] - d[0] + 1) * (d[3] - d[2] + 1) * (d[5] - d[4] + 1)
+        self.dens = [None] * len(self.stages)
+
+    def denominator(self, stage=0):
+        out = np.empty(self.n)
+        self.lib.odf_denominator(*self.dims, self.stages[stage][0], self.fbox, _d(out))
+        return out
+
+    def density(self, x, R):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.n)
+        self.lib.odf_density_forward(*self.dims, R, self.fbox, _d(x), _d(out))
+        return out
+
+    def density_transpose(self, s, R):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        out = np.empty(self.n)
+        self.lib.odf_density_transpose(*self.dims, R, self.fbox, _d(s), _d(out))
+        return out
+
+    def project(self, x, beta, eta):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.n)
+        self.lib.odf_project(*self.dims, self.fbox, beta, eta, _d(x), _d(out))
+        return out
+
+    def map_to_full(self, design):
+        design = np.ascontiguousarray(design, dtype=np.float64)
+        out = np.empty(self.n)
+        self.lib.odf_map_to_full(*self.dims, self.dbox, _d(design), _d(self.background), _d(out))
+        return out
+
+    def extract(self, full):
+        full = np.ascontiguousarray(full, dtype=np.float64)
+        out = np.empty(self.nd)
+        self.lib.odf_extract(*self.dims, self.dbox, _d(full), _d(out))
+        return out
+
+    def design(self, design):
+        """mapDesignToFull + filterDesign of every stage; caches the density-filtered fields."""
+        x = self.map_to_full(design)
+        for k, (R, proj, beta, eta) in enumerate(self.stages):
+            x = self.density(x, R)
+            self.dens[k] = x
+            if proj:
+                x = self.project(x, beta, eta)
+        return x
+
+    def sensitivities(self, full_sens):
+        """filterSensitivities (stages reversed) + mapFullToDesign."""
+        s = np.array(full_sens, dtype=np.float64)
+        for k in range(len(self.stages) - 1, -1, -1):
+            R, proj, beta, eta = self.stages[k]
+            if proj:
+                self.lib.odf_project_sens(*self.dims, self.fbox, beta, eta, _d(self.dens[k]), _d(s))
+            s = self.density_transpose(s, R)
+        return self.extract(s)
