@@ -154,6 +154,72 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup):
             "batch_device_gb": round(dev / 1e9, 2)}
 
 
+def cone_taps(R, dims):
+    """Lattice offsets d with |d| <= R (the cone's taps per filtered node)."""
+    r = np.arange(-R, R + 1)
+    if dims == 2:
+        return int(np.sum(r[:, None] ** 2 + r[None, :] ** 2 <= R * R))
+    return int(np.sum(r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2 <= R * R))
+
+
+def run_design_pipeline(lbmx, reps=20, cpu=True):
+    """Design pipeline (mapping, density filter, projection, and the sensitivity chain) on the
+    device: the two-stage chain of applications/RobustPump2d/robustPump2d.cc:140-146
+    (radii 8 / 4, beta 2 / 1) on the 1024 x 512 grid, and one projecting stage of radius 3 on
+    the 384 x 192 x 192 grid. Device time per call from HIP events around its kernels."""
+    B = lbmx.make_box
+    cases = [("2d_double_projection_1024x512", (1024, 512, 1), (64, 959, 1, 510, 0, 0),
+              ((8, True, 2.0, 0.5), (4, True, 1.0, 0.5))),
+             ("3d_projection_384x192x192", (384, 192, 192), (32, 351, 1, 190, 1, 190), ((3, True, 2.0, 0.5),))]
+    out = {}
+    rng = np.random.default_rng(0)
+    for name, dims, dbox, stages in cases:
+        f = lbmx.Filter(lbmx.make_filter_desc(*dims, B(*dbox), B(*dbox), stages))
+        d = rng.random(f.n_design)
+        s = rng.standard_normal(f.n_full)
+        f.design(d)
+        f.sensitivities(s)
+        tf, tb = [], []
+        for _ in range(reps):
+            f.design(d)
+            a, _ = f.timing()
+            f.sensitivities(s)
+            _, b = f.timing()
+            tf.append(a)
+            tb.append(b)
+        f.close()
+        n = dims[0] * dims[1] * dims[2]
+        taps = sum(cone_taps(st[0], 3 if dims[2] > 1 else 2) for st in stages)
+        ms_f, ms_b = float(np.median(tf)), float(np.median(tb))
+        # algorithmic HBM bytes per node: forward map 24, per stage cone 24 (in, den, out) and
+        # projection 16; backward per stage scale 32 (sens, x, den, out) and cone 16, extraction 16
+        bf = 24 + sum(24 + (16 if st[1] else 0) for st in stages)
+        bb = sum(32 + 16 for _ in stages) + 16
+        out[name] = {"nodes": n, "stages": [list(st) for st in stages], "cone_taps_per_node": taps,
+                     "design_ms": round(ms_f, 4), "sensitivities_ms": round(ms_b, 4),
+                     "gnodes_per_s": round(n / ((ms_f + ms_b) * 1e-3) / 1e9, 3),
+                     "design_gbs": round(bf * n / (ms_f * 1e-3) / 1e9, 1),
+                     "sensitivities_gbs": round(bb * n / (ms_b * 1e-3) / 1e9, 1),
+                     "cone_gflops": round(2 * 2 * taps * n / ((ms_f + ms_b) * 1e-3) / 1e9, 1)}
+        log(f"design pipeline {name}: design {ms_f:.3f} ms, sensitivities {ms_b:.3f} ms")
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # noqa: E402  (baseline leg only)
+        nx, ny = 256, 128
+        ch = oracle.FilterChain(nx, ny, 1, (16, 239, 1, 126, 0, 0), stages=cases[0][3])
+        d = rng.random(ch.nd)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ch.design(d)
+            ch.sensitivities(np.ones(ch.n))
+        dt = (time.perf_counter() - t0) / 3
+        out["cpu_baseline"] = {"value": round(nx * ny / dt / 1e9, 6), "unit": "Gnodes/s (design + sensitivities)",
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle chain (radii 8 / 4) on {nx}x{ny}, assembled rows, one thread"}
+        log(f"design pipeline cpu baseline: {dt * 1e3:.1f} ms per design + sensitivities on {nx}x{ny}")
+    return out
+
+
 def read_traffic(key):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by the
     copy kernel) for the 384x192x192 workload, from profiles/traffic_d3q19_mrt_brinkman.txt."""
@@ -222,6 +288,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=3000,
                     help="time steps of the batched robust-ensemble measurement (0: skip it)")
     ap.add_argument("--ensemble-slots", type=int, default=64)
+    ap.add_argument("--no-design-pipeline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but "
@@ -284,6 +351,8 @@ def main():
         out["fp32_over_fp64"] = round(res["fp32"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
     if args.ensemble_steps > 0:
         out["ensemble"] = run_ensemble(lbmx, workloads, args.ensemble_steps, args.ensemble_slots, min(args.warmup, 1))
+    if not args.no_design_pipeline:
+        out["design_pipeline"] = run_design_pipeline(lbmx, cpu=not args.no_cpu_baseline)
     out["stream_copy_gbs"] = round(lbmx.stream_copy_gbs(), 1)
     log(f"stream copy ceiling {out['stream_copy_gbs']} GB/s")
     if not args.no_cpu_baseline:

@@ -314,6 +314,12 @@ lbmx_status lbmx_filter_destroy(lbmx_filter* f);
 /* ProjectionFilter2d::setBeta/setEta (projectionFilter2d.hh:121-122),
    DoubleProjectionFilter2d::setFirst/SecondBeta/Eta (doubleProjectionFilter2d.hh:73-76) */
 lbmx_status lbmx_filter_set_projection(lbmx_filter* f, int stage, double beta, double eta);
+/* VariableProjectionFilter2d::setEtaValuesFromVector (src/topOpt/variableProjectionFilter2d.cc:123-128),
+   VariableDoubleProjectionFilter2d::setSecondEtaValuesFromVector
+   (src/topOpt/variableDoubleProjectionFilter2d.cc:28-36): the stage projects with a per-node
+   threshold eta[i] (full grid, n = nx*ny*nz) in place of its constant eta; eta = NULL returns
+   to the constant. */
+lbmx_status lbmx_filter_set_eta_field(lbmx_filter* f, int stage, const double* eta, size_t n);
 /* increaseBeta: every stage's beta *= factor (doubleProjectionFilter2d.hh:77-81) */
 lbmx_status lbmx_filter_increase_beta(lbmx_filter* f, double factor);
 /* Full-grid values outside design_box, n = nx*ny*nz: LBSolver::createDesignVec copies the

@@ -22,6 +22,9 @@ struct FilterStage {
   DeviceBuffer den;      // W 1 (rows outside the box: 1)
   DeviceBuffer dens;     // density-filtered field of the last design call
   DeviceBuffer proj;     // projected field of the last design call (projecting stages)
+  DeviceBuffer eta_dev;  // per-node thresholds (VariableProjectionFilter2d), when set
+  bool var_eta = false;
+  const double* eta_field() const { return var_eta ? eta_dev.as<double>() : nullptr; }
   const double* output() const { return static_cast<const double*>(project ? proj.p : dens.p); }
 };
 
@@ -157,7 +160,7 @@ struct Filter {
         double tb, tsum;
         proj_consts(*st, tb, tsum);
         HIP_OK(launch_project(st->proj.as<double>(), st->dens.as<double>(), d.nx, d.ny, d.nz, fbox, st->beta,
-                              st->eta, tb, tsum, s));
+                              st->eta, tb, tsum, st->eta_field(), s));
       }
       x = st->output();
     }
@@ -178,7 +181,7 @@ struct Filter {
       double tb = 0, tsum = 1;
       if (st.project) proj_consts(st, tb, tsum);
       HIP_OK(launch_sens_scale(tmp.as<double>(), cur, st.dens.as<double>(), st.den.as<double>(), d.nx, d.ny, d.nz,
-                               fbox, st.project, st.beta, st.eta, tsum, s));
+                               fbox, st.project, st.beta, st.eta, tsum, st.eta_field(), s));
       HIP_OK(launch_cone(cone_args(st, tmp.as<double>(), acc.as<double>(), true), s));
       cur = acc.as<double>();
     }
@@ -243,6 +246,23 @@ lbmx_status lbmx_filter_set_projection(lbmx_filter* h, int stage, double beta, d
     if (stage < 0 || stage >= (int)h->f->stages.size()) throw Error{LBMX_ERR_ARG, "no such stage"};
     h->f->stages[stage]->beta = beta;
     h->f->stages[stage]->eta = eta;
+  });
+}
+
+lbmx_status lbmx_filter_set_eta_field(lbmx_filter* h, int stage, const double* eta, size_t n) {
+  CHECK_FILTER(h);
+  return guarded([&] {
+    lbmx::Filter& F = *h->f;
+    if (stage < 0 || stage >= (int)F.stages.size()) throw Error{LBMX_ERR_ARG, "no such stage"};
+    lbmx::FilterStage& st = *F.stages[stage];
+    if (!eta) {
+      st.var_eta = false;
+      return;
+    }
+    if ((long long)n != F.N) throw Error{LBMX_ERR_ARG, "eta field length != grid nodes"};
+    st.eta_dev.ensure(n * 8);
+    HIP_OK(hipMemcpy(st.eta_dev.p, eta, n * 8, hipMemcpyHostToDevice));
+    st.var_eta = true;
   });
 }
 

@@ -153,25 +153,43 @@ __global__ __launch_bounds__(256) void k_extract_box(double* design, const doubl
   design[k] = full[((size_t)(zd + b[4]) * g.ny + (yd + b[2])) * g.nx + (xd + b[0])];
 }
 
+// threshold of node i: the stage constant, or a per-node value with its own tanh terms
+__device__ inline void node_eta(const double* etaf, long long i, double beta, double& eta, double& tb,
+                                double& tsum) {
+#pragma clang fp contract(off)
+  if (etaf) {
+    eta = etaf[i];
+    tb = tanh(beta * eta);
+    tsum = tb + tanh(beta * (1.0 - eta));
+  }
+}
+
 __global__ __launch_bounds__(256) void k_project(double* out, const double* xin, Grid3 g, Box6 bx, double beta,
-                                                  double eta, double tb, double tsum) {
+                                                  double eta, double tb, double tsum, const double* etaf) {
 #pragma clang fp contract(off)
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int x, y, z;
   if (!g.locate(i, x, y, z)) return;
   const double v = xin[i];
-  out[i] = in_box(x, y, z, bx.b) ? (tb + tanh(beta * (v - eta))) / tsum : v;
+  if (in_box(x, y, z, bx.b)) {
+    node_eta(etaf, i, beta, eta, tb, tsum);
+    out[i] = (tb + tanh(beta * (v - eta))) / tsum;
+  } else {
+    out[i] = v;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_sens_scale(double* out, const double* sens, const double* xin,
                                                      const double* den, Grid3 g, Box6 bx, int project, double beta,
-                                                     double eta, double tsum) {
+                                                     double eta, double tsum, const double* etaf) {
 #pragma clang fp contract(off)
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int x, y, z;
   if (!g.locate(i, x, y, z)) return;
   double s = sens[i];
   if (project && in_box(x, y, z, bx.b)) {
+    double tb = 0.0;
+    node_eta(etaf, i, beta, eta, tb, tsum);
     const double t = tanh(beta * (xin[i] - eta));
     s = s * (((1.0 - t * t) * beta) / tsum);
   }
@@ -217,19 +235,19 @@ hipError_t launch_extract_box(double* design, const double* full, int nx, int ny
 }
 
 hipError_t launch_project(double* out, const double* x, int nx, int ny, int nz, const int box[6], double beta,
-                          double eta, double tb, double tsum, hipStream_t s) {
+                          double eta, double tb, double tsum, const double* etaf, hipStream_t s) {
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(k_project, dim3(blocks_for(n)), dim3(256), 0, s, out, x, Grid3{nx, ny, nz}, box6(box), beta,
-                     eta, tb, tsum);
+                     eta, tb, tsum, etaf);
   return hipGetLastError();
 }
 
 hipError_t launch_sens_scale(double* out, const double* sens, const double* x, const double* den, int nx, int ny,
                              int nz, const int box[6], int project, double beta, double eta, double tsum,
-                             hipStream_t s) {
+                             const double* etaf, hipStream_t s) {
   const long long n = (long long)nx * ny * nz;
   hipLaunchKernelGGL(k_sens_scale, dim3(blocks_for(n)), dim3(256), 0, s, out, sens, x, den, Grid3{nx, ny, nz},
-                     box6(box), project, beta, eta, tsum);
+                     box6(box), project, beta, eta, tsum, etaf);
   return hipGetLastError();
 }
 

@@ -44,13 +44,15 @@ hipError_t launch_map_to_full(double* out, const double* design, const double* b
 // design[box-local i] = full[i] over the design box
 hipError_t launch_extract_box(double* design, const double* full, int nx, int ny, int nz, const int box[6],
                               hipStream_t s);
-// out = inside box ? (tb + tanh(beta (x - eta))) / tsum : x
+// out = inside box ? (tb + tanh(beta (x - eta))) / tsum : x, with tb = tanh(beta eta) and
+// tsum = tb + tanh(beta (1 - eta)); with a per-node eta field (etaf != nullptr) eta, tb and
+// tsum are those of etaf[i]
 hipError_t launch_project(double* out, const double* x, int nx, int ny, int nz, const int box[6], double beta,
-                          double eta, double tb, double tsum, hipStream_t s);
+                          double eta, double tb, double tsum, const double* etaf, hipStream_t s);
 // out = (inside box && project ? sens * (1 - tanh(beta (x - eta))^2) beta / tsum : sens) / den
 hipError_t launch_sens_scale(double* out, const double* sens, const double* x, const double* den, int nx, int ny,
                              int nz, const int box[6], int project, double beta, double eta, double tsum,
-                             hipStream_t s);
+                             const double* etaf, hipStream_t s);
 
 // Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
 hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);

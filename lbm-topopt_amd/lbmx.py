@@ -163,6 +163,7 @@ def library():
         "lbmx_filter_destroy": [C.c_void_p],
         "lbmx_filter_set_projection": [C.c_void_p, C.c_int, C.c_double, C.c_double],
         "lbmx_filter_increase_beta": [C.c_void_p, C.c_double],
+        "lbmx_filter_set_eta_field": [C.c_void_p, C.c_int, dp, C.c_size_t],
         "lbmx_filter_set_background": [C.c_void_p, dp, C.c_size_t],
         "lbmx_filter_design": [C.c_void_p, dp, C.c_size_t, dp, C.c_size_t],
         "lbmx_filter_design_to_solver": [C.c_void_p, dp, C.c_size_t, C.c_void_p],
@@ -190,7 +191,7 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_device_count", "lbmx_batch_create", "lbmx_batch_destroy", "lbmx_batch_checkpoint_config",
             "lbmx_batch_set_tile_rows", "lbmx_batch_set_graph", "lbmx_batch_autotune", "lbmx_batch_gradient", "lbmx_batch_get_stats",
             "lbmx_batch_device_bytes", "lbmx_filter_create", "lbmx_filter_destroy", "lbmx_filter_set_projection",
-            "lbmx_filter_increase_beta", "lbmx_filter_set_background", "lbmx_filter_design",
+            "lbmx_filter_set_eta_field", "lbmx_filter_increase_beta", "lbmx_filter_set_background", "lbmx_filter_design",
             "lbmx_filter_design_to_solver", "lbmx_filter_sensitivities", "lbmx_filter_solver_sensitivities",
             "lbmx_filter_get_field", "lbmx_filter_timing"]
 
@@ -463,6 +464,14 @@ class Filter:
 
     def increase_beta(self, factor):
         _check(self.lib.lbmx_filter_increase_beta(self.h, factor))
+
+    def set_eta_field(self, stage, eta):
+        """Per-node projection thresholds of a stage (full grid), or None for its constant eta."""
+        if eta is None:
+            _check(self.lib.lbmx_filter_set_eta_field(self.h, stage, None, 0))
+            return
+        a = np.ascontiguousarray(eta, dtype=np.float64)
+        _check(self.lib.lbmx_filter_set_eta_field(self.h, stage, _dptr(a), a.size))
 
     def set_background(self, full):
         a = np.ascontiguousarray(full, dtype=np.float64)

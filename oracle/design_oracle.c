@@ -152,28 +152,35 @@ void odf_density_transpose(int nx, int ny, int nz, int R, const int* box, const 
   odf_free(&m);
 }
 
-/* ProjectionFilter2d::filterDesign (projectionFilter2d.cc:37-42), inside the box only */
-void odf_project(int nx, int ny, int nz, const int* box, double beta, double eta, const double* x, double* out) {
+/* ProjectionFilter2d::filterDesign (projectionFilter2d.cc:37-42), inside the box only;
+   with a per-node threshold field etaf (may be NULL) as VariableProjectionFilter2d::filterDesign
+   (variableProjectionFilter2d.cc:39-63) */
+void odf_project(int nx, int ny, int nz, const int* box, double beta, double eta, const double* etaf,
+                 const double* x, double* out) {
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
         const long long r = ((long long)k * ny + j) * nx + i;
+        const double e = etaf ? etaf[r] : eta;
         out[r] = in_box(i, j, k, box)
-                     ? (tanh(beta * eta) + tanh(beta * (x[r] - eta))) / (tanh(beta * eta) + tanh(beta * (1. - eta)))
+                     ? (tanh(beta * e) + tanh(beta * (x[r] - e))) / (tanh(beta * e) + tanh(beta * (1. - e)))
                      : x[r];
       }
 }
 
 /* ProjectionFilter2d::filterSensitivities, first part (projectionFilter2d.cc:66-75):
-   s *= projDiff inside the box, projDiff at the density-filtered field x */
-void odf_project_sens(int nx, int ny, int nz, const int* box, double beta, double eta, const double* x, double* s) {
+   s *= projDiff inside the box, projDiff at the density-filtered field x; per-node thresholds
+   as VariableProjectionFilter2d::filterSensitivities (variableProjectionFilter2d.cc:65-104) */
+void odf_project_sens(int nx, int ny, int nz, const int* box, double beta, double eta, const double* etaf,
+                      const double* x, double* s) {
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
         const long long r = ((long long)k * ny + j) * nx + i;
         if (!in_box(i, j, k, box)) continue;
-        const double pd = ((1.0 - pow(tanh(beta * (x[r] - eta)), 2.0)) * beta) /
-                          (tanh(beta * eta) + tanh(beta * (1.0 - eta)));
+        const double e = etaf ? etaf[r] : eta;
+        const double pd = ((1.0 - pow(tanh(beta * (x[r] - e)), 2.0)) * beta) /
+                          (tanh(beta * e) + tanh(beta * (1.0 - e)));
         s[r] *= pd;
       }
 }

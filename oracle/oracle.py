@@ -60,8 +60,8 @@ def library():
         "odf_denominator": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, dp], None),
         "odf_density_forward": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, dp], None),
         "odf_density_transpose": ([C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, dp], None),
-        "odf_project": ([C.c_int, C.c_int, C.c_int, ip, C.c_double, C.c_double, dp, dp], None),
-        "odf_project_sens": ([C.c_int, C.c_int, C.c_int, ip, C.c_double, C.c_double, dp, dp], None),
+        "odf_project": ([C.c_int, C.c_int, C.c_int, ip, C.c_double, C.c_double, dp, dp, dp], None),
+        "odf_project_sens": ([C.c_int, C.c_int, C.c_int, ip, C.c_double, C.c_double, dp, dp, dp], None),
         "odf_map_to_full": ([C.c_int, C.c_int, C.c_int, ip, dp, dp, dp], None),
         "odf_extract": ([C.c_int, C.c_int, C.c_int, ip, dp, dp], None),
     }
@@ -231,6 +231,7 @@ class FilterChain:
         d = list(self.dbox)
         self.nd = (d[1] - d[0] + 1) * (d[3] - d[2] + 1) * (d[5] - d[4] + 1)
         self.dens = [None] * len(self.stages)
+        self.eta_fields = [None] * len(self.stages)   # per-node thresholds (variable projection)
 
     def denominator(self, stage=0):
         out = np.empty(self.n)
@@ -249,11 +250,21 @@ class FilterChain:
         self.lib.odf_density_transpose(*self.dims, R, self.fbox, _d(s), _d(out))
         return out
 
-    def project(self, x, beta, eta):
+    def _etaf(self, etaf):
+        return None if etaf is None else _d(np.ascontiguousarray(etaf, dtype=np.float64))
+
+    def project(self, x, beta, eta, etaf=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
-        out = np.empty(self.n)
-        self.lib.odf_project(*self.dims, self.fbox, beta, eta, _d(x), _d(out))
+        out = np.empty(x.size)
+        self.lib.odf_project(*self.dims, self.fbox, beta, eta, self._etaf(etaf), _d(x), _d(out))
         return out
+
+    def project_sens(self, x, beta, eta, s, etaf=None):
+        """s * projDiff(x) inside the box (a new array)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        s = np.array(s, dtype=np.float64)
+        self.lib.odf_project_sens(*self.dims, self.fbox, beta, eta, self._etaf(etaf), _d(x), _d(s))
+        return s
 
     def map_to_full(self, design):
         design = np.ascontiguousarray(design, dtype=np.float64)
@@ -274,7 +285,7 @@ class FilterChain:
             x = self.density(x, R)
             self.dens[k] = x
             if proj:
-                x = self.project(x, beta, eta)
+                x = self.project(x, beta, eta, self.eta_fields[k])
         return x
 
     def sensitivities(self, full_sens):
@@ -283,6 +294,6 @@ class FilterChain:
         for k in range(len(self.stages) - 1, -1, -1):
             R, proj, beta, eta = self.stages[k]
             if proj:
-                self.lib.odf_project_sens(*self.dims, self.fbox, beta, eta, _d(self.dens[k]), _d(s))
+                s = self.project_sens(self.dens[k], beta, eta, s, self.eta_fields[k])
             s = self.density_transpose(s, R)
         return self.extract(s)

@@ -111,8 +111,7 @@ def test_projection_closed_forms_and_derivative():
         pe = math.tanh(beta * eta) / (math.tanh(beta * eta) + math.tanh(beta * (1 - eta)))
         assert abs(p[2] - pe) < 1e-15
         x = np.linspace(0.01, 0.99, 30)
-        s = np.ones(30)
-        f.lib.odf_project_sens(6, 5, 1, f.fbox, beta, eta, oracle._d(x), oracle._d(s))
+        s = f.project_sens(x, beta, eta, np.ones(30))
         h = 1e-6
         fd = (f.project(x + h, beta, eta) - f.project(x - h, beta, eta)) / (2 * h)
         assert np.max(np.abs(fd - s)) < 1e-7 * max(1.0, np.max(np.abs(s)))
@@ -138,6 +137,29 @@ def test_chain_gradient_matches_central_differences(dims, dbox, fbox):
         fd = (np.dot(s, f.design(d + h * v)) - np.dot(s, f.design(d - h * v))) / (2 * h)
         f.design(d)
         assert abs(fd - np.dot(g, v)) < 1e-7 * max(1.0, abs(fd)), trial
+
+
+def test_variable_threshold_projection():
+    """VariableProjectionFilter2d: a per-node eta field; a constant field reproduces the
+    constant-eta stage bit for bit, and each node follows its own threshold."""
+    f = oracle.FilterChain(8, 6, 1, (0, 7, 0, 5, 0, 0), stages=((2, True, 4.0, 0.5),))
+    rng = np.random.default_rng(6)
+    x = rng.random(f.n)
+    assert np.array_equal(f.project(x, 4.0, 0.5, np.full(f.n, 0.5)), f.project(x, 4.0, 0.5))
+    etaf = rng.uniform(0.2, 0.8, f.n)
+    p = f.project(x, 4.0, 0.5, etaf)
+    for i in (0, 7, 23, 40):
+        assert p[i] == f.project(np.full(f.n, x[i]), 4.0, etaf[i])[0]
+    # the chain's gradient with a variable threshold
+    f.eta_fields[0] = etaf
+    d = rng.random(f.nd)
+    s = rng.standard_normal(f.n)
+    f.design(d)
+    g = f.sensitivities(s)
+    v = rng.standard_normal(f.nd)
+    h = 1e-6
+    fd = (np.dot(s, f.design(d + h * v)) - np.dot(s, f.design(d - h * v))) / (2 * h)
+    assert abs(fd - np.dot(g, v)) < 1e-7 * max(1.0, abs(fd))
 
 
 def test_mapping_round_trip_and_background():

@@ -96,6 +96,28 @@ def test_projection_3d_chain(lbmx_mod):
     dev.close()
 
 
+def test_variable_threshold_double_projection(lbmx_mod):
+    """VariableDoubleProjectionFilter2d as in applications/diffuser/diffuser.cc:224 (radii 4 / 2):
+    a per-node eta field on the second stage."""
+    nx, ny = 120, 64
+    dbox = (30, 90, 1, 62, 0, 0)
+    stages = ((4, True, 2.0, 0.5), (2, True, 3.0, 0.5))
+    rng = np.random.default_rng(15)
+    dev, ref = chain_pair((nx, ny, 1), dbox, dbox, stages)
+    etaf = rng.uniform(0.3, 0.7, nx * ny)
+    dev.set_eta_field(1, etaf)
+    ref.eta_fields[1] = etaf
+    d = rng.random(ref.nd)
+    assert np.max(np.abs(dev.design(d) - ref.design(d))) < 1e-14
+    s = rng.standard_normal(ref.n)
+    gr = ref.sensitivities(s)
+    assert np.max(np.abs(dev.sensitivities(s) - gr)) < 1e-13 * np.max(np.abs(gr))
+    dev.set_eta_field(1, None)      # back to the constant threshold
+    ref.eta_fields[1] = None
+    assert np.max(np.abs(dev.design(d) - ref.design(d))) < 1e-14
+    dev.close()
+
+
 def channel_solver(nx, ny, prec=lbmx.FP64):
     d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN, precision=prec,
                        omega=1.25, interp_q=0.1, alpha_max=0.1)
