@@ -16,26 +16,33 @@ namespace {
 
 constexpr int kConeTX = 64;   // outputs along x per workgroup (one wave wide)
 constexpr int kConeWY = 4;    // waves per workgroup, stacked along y
-constexpr int kConeKY = 4;    // outputs per thread along y
+constexpr int kConeKY = 4;    // outputs per thread along y (four named sums in k_cone)
 constexpr int kConeTY = kConeWY * kConeKY;  // outputs along y per workgroup
 constexpr int kMaxR = 16;
-constexpr int kLdsW = kConeTX + 2 * kMaxR;
-constexpr int kLdsH = kConeTY + 2 * kMaxR;
 
 __device__ inline bool in_box(int x, int y, int z, const int* b) {
   return x >= b[0] && x <= b[1] && y >= b[2] && y <= b[3] && z >= b[4] && z <= b[5];
 }
 
+// LDS bytes of one workgroup: the input region of one plane and that plane's cone weights
+inline size_t cone_lds_bytes(int R) {
+  const size_t W = kConeTX + 2 * R, H = kConeTY + 2 * R, S = 2 * R + 1;
+  return (W * H + S * S) * sizeof(double);
+}
+
 // One workgroup: a 64 x 16 tile of outputs in one z plane. For each in-grid plane z + dz the
-// (64 + 2R) x (16 + 2R) input region is staged in LDS, then every thread adds the cone taps of
-// that plane to its four outputs.
+// (64 + 2R) x (16 + 2R) input region and the plane's (2R + 1)^2 weights are staged in LDS;
+// then for each tap row (dy) and tap (dx) one weight is read and applied to the thread's four
+// outputs (four independent sums, each in ascending (dz, dy, dx) order).
 __global__ __launch_bounds__(256) void k_cone(ConeArgs a) {
 #pragma clang fp contract(off)
-  __shared__ double tile[kLdsW * kLdsH];
+  extern __shared__ double lds[];
   const int R = a.R;
   const int W = kConeTX + 2 * R;
   const int H = kConeTY + 2 * R;
   const int S = 2 * R + 1;
+  double* tile = lds;
+  double* wl = lds + W * H;
   const int tx = threadIdx.x % kConeTX;
   const int ty = threadIdx.x / kConeTX;
   const int x0 = blockIdx.x * kConeTX;
@@ -72,29 +79,68 @@ __global__ __launch_bounds__(256) void k_cone(ConeArgs a) {
       }
       tile[ly * W + lx] = v;
     }
+    const double* wz = a.wt + (size_t)(dz + R) * S * S;
+    for (int idx = threadIdx.x; idx < S * S; idx += blockDim.x) wl[idx] = wz[idx];
     __syncthreads();
     const int* rxz = a.rx + (dz + R) * S;
-    const double* wz = a.wt + (size_t)(dz + R) * S * S;
     for (int dy = -R; dy <= R; ++dy) {
       const int r = rxz[dy + R];
       if (r < 0) continue;
-      const double* w = wz + (dy + R) * S + R;  // w[dx], dx in [-R, R]
-      const bool centre_row = dz == 0 && dy == 0;
-#pragma unroll
-      for (int k = 0; k < kConeKY; ++k) {
-        const double* row = tile + (ty + kConeWY * k + dy + R) * W + tx + R;
-        double s = acc[k];
-        if (!centre_row) {
-          for (int dx = -r; dx <= r; ++dx) s = s + w[dx] * row[dx];
-        } else {
-          for (int dx = -r; dx < 0; ++dx) s = s + w[dx] * row[dx];
-          // the diagonal entry: cone weight for a row in the box, 1 for an identity row
-          if (a.transposed) s = s + (mine[k] ? w[0] : 1.0) * own[k];
-          else s = s + w[0] * row[0];
-          for (int dx = 1; dx <= r; ++dx) s = s + w[dx] * row[dx];
+      const double* w = wl + (dy + R) * S + R;  // w[dx], dx in [-R, R]
+      const double* row = tile + (ty + dy + R) * W + tx + R;
+      constexpr int kStep = kConeWY;           // rows between this thread's outputs
+      const double* r1 = row + kStep * W;
+      const double* r2 = row + 2 * kStep * W;
+      const double* r3 = row + 3 * kStep * W;
+      double s0 = acc[0], s1 = acc[1], s2 = acc[2], s3 = acc[3];
+      // taps [lo, hi) of this row, two at a time with all loads issued first
+      auto taps = [&](int lo, int hi) {
+        int dx = lo;
+        for (; dx + 1 < hi; dx += 2) {
+          const double wa = w[dx], wb = w[dx + 1];
+          const double a0 = row[dx], b0 = row[dx + 1], a1 = r1[dx], b1 = r1[dx + 1];
+          const double a2 = r2[dx], b2 = r2[dx + 1], a3 = r3[dx], b3 = r3[dx + 1];
+          s0 = s0 + wa * a0;
+          s1 = s1 + wa * a1;
+          s2 = s2 + wa * a2;
+          s3 = s3 + wa * a3;
+          s0 = s0 + wb * b0;
+          s1 = s1 + wb * b1;
+          s2 = s2 + wb * b2;
+          s3 = s3 + wb * b3;
         }
-        acc[k] = s;
+        if (dx < hi) {
+          const double wa = w[dx];
+          const double a0 = row[dx], a1 = r1[dx], a2 = r2[dx], a3 = r3[dx];
+          s0 = s0 + wa * a0;
+          s1 = s1 + wa * a1;
+          s2 = s2 + wa * a2;
+          s3 = s3 + wa * a3;
+        }
+      };
+      if (!(dz == 0 && dy == 0)) {
+        taps(-r, r + 1);
+      } else {
+        taps(-r, 0);
+        // the diagonal entry: cone weight for a row in the box, 1 for an identity row
+        const double wv = w[0];
+        if (a.transposed) {
+          s0 = s0 + (mine[0] ? wv : 1.0) * own[0];
+          s1 = s1 + (mine[1] ? wv : 1.0) * own[1];
+          s2 = s2 + (mine[2] ? wv : 1.0) * own[2];
+          s3 = s3 + (mine[3] ? wv : 1.0) * own[3];
+        } else {
+          s0 = s0 + wv * row[0];
+          s1 = s1 + wv * r1[0];
+          s2 = s2 + wv * r2[0];
+          s3 = s3 + wv * r3[0];
+        }
+        taps(1, r + 1);
       }
+      acc[0] = s0;
+      acc[1] = s1;
+      acc[2] = s2;
+      acc[3] = s3;
     }
   }
 
@@ -214,7 +260,7 @@ inline Box6 box6(const int b[6]) {
 hipError_t launch_cone(const ConeArgs& a, hipStream_t s) {
   if (a.R < 1 || a.R > kMaxR || a.nx < 1 || a.ny < 1 || a.nz < 1) return hipErrorInvalidValue;
   dim3 grid((a.nx + kConeTX - 1) / kConeTX, (a.ny + kConeTY - 1) / kConeTY, a.nz);
-  hipLaunchKernelGGL(k_cone, grid, dim3(kConeTX * kConeWY), 0, s, a);
+  hipLaunchKernelGGL(k_cone, grid, dim3(kConeTX * kConeWY), cone_lds_bytes(a.R), s, a);
   return hipGetLastError();
 }
 
