@@ -355,6 +355,11 @@ def main():
         out["design_pipeline"] = run_design_pipeline(lbmx, cpu=not args.no_cpu_baseline)
     out["stream_copy_gbs"] = round(lbmx.stream_copy_gbs(), 1)
     log(f"stream copy ceiling {out['stream_copy_gbs']} GB/s")
+    # the same bytes moved as 19 planes in and 19 planes out (the forward sweep's pattern)
+    pc = lbmx.plane_copy_gbs(19, nx * ny * nz)
+    out["roofline"]["plane_copy_gbs"] = round(pc, 1)
+    out["roofline"]["frac_of_plane_copy"] = round(head["fwd_gbs"] / pc, 4)
+    log(f"19-plane copy {pc:.1f} GB/s; forward sweep at {head['fwd_gbs'] / pc:.3f} of it")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(workloads, lbmx)
     print(json.dumps(out), flush=True)

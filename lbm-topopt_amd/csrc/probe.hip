@@ -16,7 +16,58 @@ __global__ __launch_bounds__(256) void k_stream_copy(const double4* __restrict__
   if (i < n4) out[i] = in[i];
 }
 
+// The access pattern of a Q-direction sweep without its arithmetic: Q planes read at the
+// same node offset, Q planes written (FP64, one node per thread).
+template <int Q>
+__global__ __launch_bounds__(256) void k_plane_copy(const double* __restrict__ in, double* __restrict__ out,
+                                                    long long plane, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = in[q * plane + i];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) out[q * plane + i] = v[q];
+}
+
 }  // namespace lbmx
+
+extern "C" lbmx_status lbmx_stream_copy_planes(int planes, size_t nodes, int iters, double* gbs) {
+  if (!gbs || iters < 1 || nodes < 256 || (planes != 9 && planes != 19)) return LBMX_ERR_ARG;
+  const long long n = (long long)nodes;
+  const long long plane = (n + 31) / 32 * 32;  // 256-byte aligned planes
+  const size_t bytes = (size_t)planes * plane * 8;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return LBMX_ERR_MEM;
+  if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return LBMX_ERR_MEM; }
+  (void)hipMemset(a, 0, bytes);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  auto launch = [&]() {
+    if (planes == 19)
+      hipLaunchKernelGGL(lbmx::k_plane_copy<19>, dim3(blocks), dim3(256), 0, nullptr, (const double*)a, (double*)b,
+                         plane, n);
+    else
+      hipLaunchKernelGGL(lbmx::k_plane_copy<9>, dim3(blocks), dim3(256), 0, nullptr, (const double*)a, (double*)b,
+                         plane, n);
+  };
+  launch();
+  (void)hipEventRecord(e0, nullptr);
+  for (int k = 0; k < iters; ++k) launch();
+  (void)hipEventRecord(e1, nullptr);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) return LBMX_ERR_DEVICE;
+  *gbs = 2.0 * planes * n * 8.0 * iters / (ms * 1e-3) / 1e9;
+  return LBMX_OK;
+}
 
 extern "C" lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs) {
   if (!gbs || iters < 1 || bytes < 1024) return LBMX_ERR_ARG;
