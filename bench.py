@@ -220,6 +220,61 @@ def run_design_pipeline(lbmx, reps=20, cpu=True):
     return out
 
 
+def run_mma(lbmx, cpu=True):
+    """Method of moving asymptotes on the device: the robust form of
+    applications/RobustPump2d/robustPump2d.cc:173-190 (three realisations with a_j = 1 and a
+    volume constraint with a_j = 0) on synthetic smooth responses, at the 1024 x 512 and the
+    384 x 192 x 192 design sizes. Wall milliseconds per update (host round trips of the dual
+    Newton iteration included); the oracle's single-thread update as CPU baseline."""
+    out = {}
+
+    def inputs(x, targets, weights):
+        n = x.size
+        g = [w * np.sum((x - t) ** 2) / n for t, w in zip(targets, weights)] + [x.mean() - 0.45]
+        dg = [2 * w * (x - t) / n for t, w in zip(targets, weights)] + [np.full(n, 1.0 / n)]
+        return np.zeros(n), np.array(g), np.stack(dg)
+
+    rng = np.random.default_rng(0)
+    for name, n in (("1024x512", 524288), ("384x192x192", 14155776)):
+        targets = [0.2 + 0.6 * rng.random(n) for _ in range(3)]
+        w = [1.0, 1.3, 0.8]
+        h = lbmx.MMA(n, 4, a=[1.0, 1.0, 1.0, 0.0])
+        x = np.full(n, 0.5)
+        ms, its = [], []
+        for _ in range(4):
+            x = h.update(x, *inputs(x, targets, w), 0.1)
+            st = h.state()
+            ms.append(st["ms"])
+            its.append(st["newton_iters"])
+        h.close()
+        out[name] = {"n": n, "m": 4, "update_ms": [round(v, 3) for v in ms], "newton_iters": its,
+                     "ms_per_newton_iter": round(float(np.median(np.array(ms) / np.array(its))), 4)}
+        log(f"mma {name}: {', '.join(f'{v:.1f}' for v in ms)} ms per update ({its} dual Newton iterations)")
+        del targets
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # noqa: E402  (baseline leg only)
+        n = 131072
+        targets = [0.2 + 0.6 * np.random.default_rng(0).random(n) for _ in range(3)]
+        ref = oracle.MMA(n, 4, a=[1.0, 1.0, 1.0, 0.0])
+        x = np.full(n, 0.5)
+        t0 = time.perf_counter()
+        its = 0
+        for _ in range(2):
+            x = ref.update(x, *inputs(x, targets, [1.0, 1.3, 0.8]), 0.1)
+            its += ref.state()["newton_iters"]
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(dt * 1e3, 1), "unit": "ms for two updates", "cores": 1, "kind": "port",
+                               "newton_iters": its,
+                               "ns_per_variable_newton_iter": round(dt * 1e9 / (n * its), 4),
+                               "sample": f"oracle MMA, n = {n}, m = 4, first two updates from x = 0.5"}
+        log(f"mma cpu baseline: {dt * 1e3:.1f} ms for two updates ({its} Newton iterations), n = {n}")
+    for name in ("1024x512", "384x192x192"):
+        r = out[name]
+        r["ns_per_variable_newton_iter"] = round(r["ms_per_newton_iter"] * 1e6 / r["n"], 4)
+    return out
+
+
 def read_traffic(key):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated by the
     copy kernel) for the 384x192x192 workload, from profiles/traffic_d3q19_mrt_brinkman.txt."""
@@ -289,6 +344,7 @@ def main():
                     help="time steps of the batched robust-ensemble measurement (0: skip it)")
     ap.add_argument("--ensemble-slots", type=int, default=64)
     ap.add_argument("--no-design-pipeline", action="store_true")
+    ap.add_argument("--no-mma", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but "
@@ -353,6 +409,8 @@ def main():
         out["ensemble"] = run_ensemble(lbmx, workloads, args.ensemble_steps, args.ensemble_slots, min(args.warmup, 1))
     if not args.no_design_pipeline:
         out["design_pipeline"] = run_design_pipeline(lbmx, cpu=not args.no_cpu_baseline)
+    if not args.no_mma:
+        out["mma"] = run_mma(lbmx, cpu=not args.no_cpu_baseline)
     out["stream_copy_gbs"] = round(lbmx.stream_copy_gbs(), 1)
     log(f"stream copy ceiling {out['stream_copy_gbs']} GB/s")
     # the same bytes moved as 19 planes in and 19 planes out (the forward sweep's pattern)

@@ -352,6 +352,36 @@ lbmx_status lbmx_filter_get_field(lbmx_filter* f, int which, double* out, size_t
    (HIP events around its kernels): ms[0] design, ms[1] sensitivities. */
 lbmx_status lbmx_filter_timing(lbmx_filter* f, double ms[2]);
 
+/* ---- method of moving asymptotes (the design update of the optimisation loop) ----
+   MMA of src/topOpt/MMA.cc (Aage's PETSc MMA with the reference's robust constants): each
+   update builds Svanberg's convex separable subproblem around the design and solves it
+   through its dual by an interior-point Newton iteration on the m multipliers. The n design
+   variables live in HBM; design vectors cross the boundary as host arrays in natural order. */
+#define LBMX_MMA_MAX_CONSTRAINTS 8
+typedef struct lbmx_mma lbmx_mma;
+/* MMA::MMA(n, m, x) (src/topOpt/MMA.cc:208-251): a_j = 0, c_j = 1000, asymptote factors
+   0.5 / 0.7 / 1.2; m in 1..LBMX_MMA_MAX_CONSTRAINTS */
+lbmx_status lbmx_mma_create(long long n, int m, lbmx_mma** out);
+lbmx_status lbmx_mma_destroy(lbmx_mma* h);
+/* MMA::SetRobustConstants (MMA.cc:290-296): a[m], the weights of the bound variable z in the
+   constraints (1 for a robust realisation, 0 for the volume constraint) */
+lbmx_status lbmx_mma_set_robust_constants(lbmx_mma* h, const double* a);
+/* MMA::SetAsymptotes (MMA.cc:343-351) */
+lbmx_status lbmx_mma_set_asymptotes(lbmx_mma* h, double init, double decrease, double increase);
+/* MMA::ConstraintModification (MMA.hh:131) and MMA::SetRobustAsymptotesType (MMA.cc:353-367) */
+lbmx_status lbmx_mma_set_options(lbmx_mma* h, int constraint_modification, int robust_asymptotes_type);
+/* MMA::Update (MMA.cc:485-500): x[n] is the design in and the new design out; dfdx[n] the
+   objective sensitivity, gx[m] the constraint values, dgdx[m] pointers to their sensitivities;
+   move limits max(0, x - movlim) .. min(1, x + movlim). */
+lbmx_status lbmx_mma_update(lbmx_mma* h, double* x, const double* dfdx, const double* gx, const double* const* dgdx,
+                            double movlim);
+/* MMA::DesignChange (MMA.cc:389-414): *ch = max |xnew - x before the last update|; xnew = NULL
+   uses the design of the last update */
+lbmx_status lbmx_mma_design_change(lbmx_mma* h, const double* xnew, double* ch);
+/* Diagnostics: asymptotes L[n], U[n], multipliers lam[m], dual Newton iterations and wall
+   milliseconds of the last update (any pointer may be NULL). */
+lbmx_status lbmx_mma_get_state(lbmx_mma* h, double* L, double* U, double* lam, int* newton_iters, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,34 @@ hipError_t launch_sens_scale(double* out, const double* sens, const double* x, c
                              int nz, const int box[6], int project, double beta, double eta, double tsum,
                              const double* etaf, hipStream_t s);
 
+// Method of moving asymptotes: per-variable arrays of one optimiser (n variables, m <= 8
+// constraints; pij, qij, dgdx hold m rows of n) and the multipliers of a dual evaluation.
+constexpr int kMmaMaxM = 8;
+struct MmaArgs {
+  long long n;
+  int m, k, con_mod, asym_type;
+  double movlim, asyminit, asymdec, asyminc;
+  double* x;
+  const double* dfdx;
+  const double* dgdx;
+  const double* xo1;
+  const double* xo2;
+  double *lower, *upper, *L, *U, *alpha, *beta, *p0, *q0, *pij, *qij;
+};
+struct MmaLambda {
+  double lam[kMmaMaxM];
+};
+int mma_blocks(long long n);  // blocks of the reductions (partials per block)
+// subproblem at a.x; out[0..m) = sum_i pij / (U - x) + qij / (x - L)
+hipError_t launch_mma_gensub(const MmaArgs& a, double* part, double* out, hipStream_t s);
+// a.x = x(lam); out[0..m) = the same sums at x(lam); with hess, out[m + j*m + l] = the
+// Gauss-Newton part of the dual Hessian
+hipError_t launch_mma_dual(const MmaArgs& a, const MmaLambda& lm, bool hess, double* part, double* out,
+                           hipStream_t s);
+// out[0] = max |xnew - xo1|
+hipError_t launch_mma_change(const double* xnew, const double* xo1, long long n, double* part, double* out,
+                             hipStream_t s);
+
 // Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
 hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 hipError_t launch_batch_d2q9_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);

@@ -172,6 +172,14 @@ def library():
         "lbmx_filter_solver_sensitivities": [C.c_void_p, C.c_void_p, dp, C.c_size_t],
         "lbmx_filter_get_field": [C.c_void_p, C.c_int, dp, C.c_size_t],
         "lbmx_filter_timing": [C.c_void_p, dp],
+        "lbmx_mma_create": [C.c_longlong, C.c_int, P(C.c_void_p)],
+        "lbmx_mma_destroy": [C.c_void_p],
+        "lbmx_mma_set_robust_constants": [C.c_void_p, dp],
+        "lbmx_mma_set_asymptotes": [C.c_void_p, C.c_double, C.c_double, C.c_double],
+        "lbmx_mma_set_options": [C.c_void_p, C.c_int, C.c_int],
+        "lbmx_mma_update": [C.c_void_p, dp, dp, dp, P(dp), C.c_double],
+        "lbmx_mma_design_change": [C.c_void_p, dp, dp],
+        "lbmx_mma_get_state": [C.c_void_p, dp, dp, dp, P(C.c_int), dp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -194,7 +202,9 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_batch_device_bytes", "lbmx_filter_create", "lbmx_filter_destroy", "lbmx_filter_set_projection",
             "lbmx_filter_set_eta_field", "lbmx_filter_increase_beta", "lbmx_filter_set_background", "lbmx_filter_design",
             "lbmx_filter_design_to_solver", "lbmx_filter_sensitivities", "lbmx_filter_solver_sensitivities",
-            "lbmx_filter_get_field", "lbmx_filter_timing"]
+            "lbmx_filter_get_field", "lbmx_filter_timing", "lbmx_mma_create", "lbmx_mma_destroy",
+            "lbmx_mma_set_robust_constants", "lbmx_mma_set_asymptotes", "lbmx_mma_set_options", "lbmx_mma_update",
+            "lbmx_mma_design_change", "lbmx_mma_get_state"]
 
 
 def checkpoint_simulate(nslots, nsteps):
@@ -516,3 +526,61 @@ class Filter:
         ms = (C.c_double * 2)()
         _check(self.lib.lbmx_filter_timing(self.h, ms))
         return ms[0], ms[1]
+
+
+class MMA:
+    """Method of moving asymptotes on the device (MMA of src/topOpt/MMA.cc): n design
+    variables, m constraints (1..8). update() returns the new design."""
+
+    def __init__(self, n, m, a=None):
+        self.lib = library()
+        self.n, self.m = n, m
+        self.h = C.c_void_p()
+        _check(self.lib.lbmx_mma_create(n, m, C.byref(self.h)))
+        if a is not None:
+            self.set_robust_constants(a)
+
+    def close(self):
+        if self.h:
+            self.lib.lbmx_mma_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_robust_constants(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _check(self.lib.lbmx_mma_set_robust_constants(self.h, _dptr(a)))
+
+    def set_asymptotes(self, init, dec, inc):
+        _check(self.lib.lbmx_mma_set_asymptotes(self.h, init, dec, inc))
+
+    def set_options(self, constraint_modification=False, robust_asymptotes_type=0):
+        _check(self.lib.lbmx_mma_set_options(self.h, int(constraint_modification), int(robust_asymptotes_type)))
+
+    def update(self, x, dfdx, gx, dgdx, movlim):
+        x = np.array(x, dtype=np.float64)
+        dfdx = np.ascontiguousarray(dfdx, dtype=np.float64)
+        gx = np.ascontiguousarray(gx, dtype=np.float64)
+        rows = [np.ascontiguousarray(r, dtype=np.float64) for r in np.asarray(dgdx, dtype=np.float64).reshape(self.m, self.n)]
+        ptrs = (C.POINTER(C.c_double) * self.m)(*[_dptr(r) for r in rows])
+        _check(self.lib.lbmx_mma_update(self.h, _dptr(x), _dptr(dfdx), _dptr(gx), ptrs, movlim))
+        return x
+
+    def design_change(self, xnew=None):
+        ch = C.c_double(0.0)
+        if xnew is None:
+            _check(self.lib.lbmx_mma_design_change(self.h, None, C.byref(ch)))
+        else:
+            a = np.ascontiguousarray(xnew, dtype=np.float64)
+            _check(self.lib.lbmx_mma_design_change(self.h, _dptr(a), C.byref(ch)))
+        return ch.value
+
+    def state(self):
+        L, U, lam = np.empty(self.n), np.empty(self.n), np.empty(self.m)
+        it, ms = C.c_int(0), C.c_double(0.0)
+        _check(self.lib.lbmx_mma_get_state(self.h, _dptr(L), _dptr(U), _dptr(lam), C.byref(it), C.byref(ms)))
+        return {"L": L, "U": U, "lam": lam, "newton_iters": it.value, "ms": ms.value}

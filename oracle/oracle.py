@@ -64,6 +64,14 @@ def library():
         "odf_project_sens": ([C.c_int, C.c_int, C.c_int, ip, C.c_double, C.c_double, dp, dp, dp], None),
         "odf_map_to_full": ([C.c_int, C.c_int, C.c_int, ip, dp, dp, dp], None),
         "odf_extract": ([C.c_int, C.c_int, C.c_int, ip, dp, dp], None),
+        "omma_create": ([C.c_int, C.c_int], vp),
+        "omma_destroy": ([vp], None),
+        "omma_set_a": ([vp, dp], None),
+        "omma_set_asymptotes": ([vp, C.c_double, C.c_double, C.c_double], None),
+        "omma_set_options": ([vp, C.c_int, C.c_int], None),
+        "omma_update": ([vp, dp, dp, dp, dp, C.c_double], None),
+        "omma_design_change": ([vp, dp], C.c_double),
+        "omma_state": ([vp, dp, dp, dp, ip, ip, dp], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -297,3 +305,50 @@ class FilterChain:
                 s = self.project_sens(self.dens[k], beta, eta, s, self.eta_fields[k])
             s = self.density_transpose(s, R)
         return self.extract(s)
+
+
+class MMA:
+    """Method of moving asymptotes restated on the CPU (src/topOpt/MMA.cc): n design
+    variables, m constraints; update() overwrites x with the subproblem's solution."""
+
+    def __init__(self, n, m, a=None):
+        self.lib = library()
+        self.n, self.m = n, m
+        self.h = self.lib.omma_create(n, m)
+        if a is not None:
+            self.set_robust_constants(a)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.omma_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_robust_constants(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.lib.omma_set_a(self.h, _d(a))
+
+    def set_asymptotes(self, init, dec, inc):
+        self.lib.omma_set_asymptotes(self.h, init, dec, inc)
+
+    def set_options(self, constraint_modification=False, robust_asymptotes_type=0):
+        self.lib.omma_set_options(self.h, int(constraint_modification), int(robust_asymptotes_type))
+
+    def update(self, x, dfdx, gx, dgdx, movlim):
+        x = np.array(x, dtype=np.float64)
+        dfdx = np.ascontiguousarray(dfdx, dtype=np.float64)
+        gx = np.ascontiguousarray(gx, dtype=np.float64)
+        dgdx = np.ascontiguousarray(np.asarray(dgdx, dtype=np.float64).reshape(self.m, self.n))
+        self.lib.omma_update(self.h, _d(x), _d(dfdx), _d(gx), _d(dgdx), movlim)
+        return x
+
+    def design_change(self, xnew):
+        xnew = np.ascontiguousarray(xnew, dtype=np.float64)
+        return self.lib.omma_design_change(self.h, _d(xnew))
+
+    def state(self):
+        L, U, lam = np.empty(self.n), np.empty(self.n), np.empty(self.m)
+        k, it, res = C.c_int(), C.c_int(), C.c_double()
+        self.lib.omma_state(self.h, _d(L), _d(U), _d(lam), C.byref(k), C.byref(it), C.byref(res))
+        return {"L": L, "U": U, "lam": lam, "k": k.value, "newton_iters": it.value, "residual": res.value}
