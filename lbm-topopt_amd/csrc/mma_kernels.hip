@@ -146,13 +146,22 @@ __global__ __launch_bounds__(kMmaThreads) void k_mma_dual(MmaArgs a, MmaLambda l
   block_sums<K>(acc, part);
 }
 
-// out[k] = sum over blocks b of part[b * K + k], in block order
+// out[k] = sum over blocks b of part[b * K + k]: thread t adds the partials t, t + 256, ...
+// in order, then a fixed tree over the 256 threads (the same order on every call)
 __global__ __launch_bounds__(kMmaThreads) void k_mma_finish(const double* part, int nblocks, int K, double* out) {
 #pragma clang fp contract(off)
-  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+  __shared__ double red[kMmaThreads];
+  for (int k = 0; k < K; ++k) {
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s = s + part[(size_t)b * K + k];
-    out[k] = s;
+    for (int b = threadIdx.x; b < nblocks; b += kMmaThreads) s = s + part[(size_t)b * K + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kMmaThreads / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
   }
 }
 
