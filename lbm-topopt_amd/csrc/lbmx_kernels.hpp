@@ -35,6 +35,18 @@ template <class S>
 __device__ __forceinline__ void st(char* base, unsigned int off_bytes, S v) {
   *reinterpret_cast<S*>(base + off_bytes) = v;
 }
+// Stores of whole aligned rows that nothing reads before the next sweep bypass the caches'
+// retention (non-temporal): on MI355X this leaves L2 to the lines that shifted pulls share
+// between neighbouring workgroups. Partial-line stores (x-shifted pushes) stay ordinary, so
+// neighbouring waves still merge them in L2.
+template <class S>
+__device__ __forceinline__ void st_nt(char* base, unsigned int off_bytes, S v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<S*>(base + off_bytes));
+}
+template <class S>
+__device__ __forceinline__ S ld_nt(const char* base, unsigned int off_bytes) {
+  return __builtin_nontemporal_load(reinterpret_cast<const S*>(base + off_bytes));
+}
 
 // pull address base of plane q: start + (q*plane - shift_q) elements
 template <class L, int q, class S>
@@ -137,14 +149,14 @@ __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned 
         S e0, e1;
         P::pack2(P::template sr_ready<L, q>(f[q], sr_bits<q>(r0, r1)),
                  P::template sr_ready<L, q + 1>(f[q + 1], sr_bits<q + 1>(r0, r1)), e0, e1);
-        st<S>(own_base_w<q, S>(dst, l), ob, e0);
-        st<S>(own_base_w<q + 1, S>(dst, l), ob, e1);
+        st_nt<S>(own_base_w<q, S>(dst, l), ob, e0);
+        st_nt<S>(own_base_w<q + 1, S>(dst, l), ob, e1);
       } else {
-        st<S>(own_base_w<q, S>(dst, l), ob, enc1(IC<q>{}));
+        st_nt<S>(own_base_w<q, S>(dst, l), ob, enc1(IC<q>{}));
       }
     });
   } else {
-    static_for<0, L::Q>([&](auto qc) { st<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc1(qc)); });
+    static_for<0, L::Q>([&](auto qc) { st_nt<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc1(qc)); });
   }
   if (bnd) {
     static_for<1, L::Q>([&](auto qc) {
@@ -323,7 +335,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   T lam[L::Q];
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    lam[q] = T(ld<LS>(own_base<q, LS>(a.lam_in, a.ll), offl));
+    lam[q] = T(ld_nt<LS>(own_base<q, LS>(a.lam_in, a.ll), offl));
   });
   if (bnd) {
     // nu_t(x)[r] for x + c_r outside was pushed by this node into the halo cell x + c_r,
@@ -349,12 +361,14 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
   T dg = 0;
   reverse_collide<L, COLL, DES, T>(f, gam, a.m, lam, dg, srcA, srcB, srcAT);
-  a.sens[n] += double(dg);
+  __builtin_nontemporal_store(__builtin_nontemporal_load(a.sens + n) + double(dg), a.sens + n);
   if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, a.nbc, x, y, z);
-  // transposed stream: nu_{t-1}(x - c_q)[q] = lam[q], i.e. a store to the pull address
+  // transposed stream: nu_{t-1}(x - c_q)[q] = lam[q], i.e. a store to the pull address (whole
+  // rows when c_q has no x component)
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    st<LS>(pull_base_w<L, q, LS>(a.lam_out, a.ll), offl, LS(lam[q]));
+    if constexpr (L::ex[q] == 0) st_nt<LS>(pull_base_w<L, q, LS>(a.lam_out, a.ll), offl, LS(lam[q]));
+    else st<LS>(pull_base_w<L, q, LS>(a.lam_out, a.ll), offl, LS(lam[q]));
   });
 }
 

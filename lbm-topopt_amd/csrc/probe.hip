@@ -17,7 +17,8 @@ __global__ __launch_bounds__(256) void k_stream_copy(const double4* __restrict__
 }
 
 // The access pattern of a Q-direction sweep without its arithmetic: Q planes read at the
-// same node offset, Q planes written (FP64, one node per thread).
+// same node offset, Q planes written with non-temporal stores as the sweeps write their
+// aligned rows (FP64, one node per thread).
 template <int Q>
 __global__ __launch_bounds__(256) void k_plane_copy(const double* __restrict__ in, double* __restrict__ out,
                                                     long long plane, long long n) {
@@ -27,7 +28,7 @@ __global__ __launch_bounds__(256) void k_plane_copy(const double* __restrict__ i
 #pragma unroll
   for (int q = 0; q < Q; ++q) v[q] = in[q * plane + i];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) out[q * plane + i] = v[q];
+  for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(v[q], out + q * plane + i);
 }
 
 }  // namespace lbmx
