@@ -96,6 +96,9 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
         "objective": J,
         "_sens": sens,
         "recompute_sweeps_per_gradient": recompute[-1],
+        # algorithmic bytes of the whole pass (recorded and recomputed forward sweeps, adjoint
+        # sweeps) over its wall time, host hand-off and initialisation included
+        "pass_gbs": ((horizon + recompute[-1]) * bf + horizon * ba) * N / t_step / 1e9,
         "device_gb": dev_bytes / 1e9,
     }
 
@@ -381,7 +384,8 @@ def main():
             "kernel": "k_fwd<D3Q19,MRT,%s> (forward sweep)" % head_form,
             "bytes_per_lu": head["fwd_bytes_per_lu"],
             "adjoint_kernel": {"achieved": round(head["adj_gbs"], 1), "frac": round(head["adj_gbs"] / PEAK_GBS, 4),
-                               "bytes_per_lu": head["adj_bytes_per_lu"]}}
+                               "bytes_per_lu": head["adj_bytes_per_lu"]},
+            "gradient_pass": {"achieved": round(head["pass_gbs"], 1), "frac": round(head["pass_gbs"] / PEAK_GBS, 4)}}
     out = {
         "metric": METRIC,
         "value": round(head["mlups_fwd_adj"], 2),
