@@ -56,6 +56,29 @@ def build(verbose=True):
     return LIB
 
 
+HOST = os.path.join(HERE, "host")
+EXAMPLES = [os.path.join(ROOT, "tests", "cpp", "robust_pump_miniature.cpp")]
+
+
+def build_host_examples(verbose=True):
+    """Compile the C++ programs written against the host mirror (host/lbm_topopt.hpp) with the
+    system g++, linked to the in-tree liblbmx.so (found at run time through a relative RUNPATH)."""
+    out = []
+    for src in EXAMPLES:
+        exe = os.path.splitext(src)[0]
+        rel = os.path.relpath(HERE, os.path.dirname(exe))
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + HOST, src,
+               "-L" + HERE, "-llbmx", "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath-link,/opt/rocm/lib", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host example build failed: {' '.join(cmd)}\n{r.stderr[-4000:]}")
+        out.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return out
+
+
 if __name__ == "__main__":
     build()
+    build_host_examples()
     sys.exit(0)

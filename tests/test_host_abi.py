@@ -38,6 +38,16 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_host_mirror_header_compiles():
+    """The C++ mirror of the reference's classes (lbm-topopt_amd/host/lbm_topopt.hpp) and the
+    program written against it compile with the system C++ compiler (no device needed)."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "robust_pump_miniature.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(ROOT, "lbm-topopt_amd", "host"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_error_paths_without_device():
     lib = lbmx.library()
     import ctypes as C
