@@ -352,6 +352,17 @@ lbmx_status lbmx_filter_get_field(lbmx_filter* f, int which, double* out, size_t
    (HIP events around its kernels): ms[0] design, ms[1] sensitivities. */
 lbmx_status lbmx_filter_timing(lbmx_filter* f, double ms[2]);
 
+/* ---- file formats either side of the engine (host only) ---- */
+/* writeVectorToBinary / loadVectorFromBinary (src/core/binaryIO.cc:3-34): PETSc's binary Vec
+   format, big-endian int32 class id 1211214, int32 length, then the FP64 values in natural
+   order; the write also leaves PETSc's `<path>.info` options file. The load fails unless the
+   file holds exactly n values. */
+lbmx_status lbmx_vec_write_binary(const char* path, const double* v, size_t n);
+lbmx_status lbmx_vec_load_binary(const char* path, double* v, size_t n);
+/* LBSolver::outputFields (src/latticeBoltzmann/LBSolver.cc:251-286): the current state's macros
+   (rho, ux, uy[, uz][, T]) and the design field as a VTK XML StructuredGrid (.vts, ASCII). */
+lbmx_status lbmx_write_vts(lbmx_solver* s, const char* path);
+
 /* ---- method of moving asymptotes (the design update of the optimisation loop) ----
    MMA of src/topOpt/MMA.cc (Aage's PETSc MMA with the reference's robust constants): each
    update builds Svanberg's convex separable subproblem around the design and solves it

@@ -1223,6 +1223,8 @@ static Engine& engine_of(lbmx_solver* s) {
 
 hipStream_t solver_stream(lbmx_solver* s) { return engine_of(s).stream; }
 
+int solver_lattice(lbmx_solver* s) { return engine_of(s).d.lattice; }
+
 long long solver_nodes(lbmx_solver* s, int dims[3]) {
   Engine& e = engine_of(s);
   dims[0] = e.d.nx;

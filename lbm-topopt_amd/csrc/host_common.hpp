@@ -73,6 +73,7 @@ inline lbmx_status guarded(F&& f) {
 // Access to a solver handle's device fields for the design pipeline (implemented with the
 // engine). All of them throw Error.
 hipStream_t solver_stream(lbmx_solver* s);
+int solver_lattice(lbmx_solver* s);
 long long solver_nodes(lbmx_solver* s, int dims[3]);
 // Sets the solver's design from a device FP64 array in natural order, on the solver's stream.
 void solver_set_design_device(lbmx_solver* s, const double* dev);

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "lbmx.h"
@@ -310,6 +311,16 @@ inline LBMacroFunctional ThermalOutflowEast2d(const Box2d& out) {  // temperatur
   return LBMacroFunctional(LBMX_OBJ_HEAT_FLUX, out.raw());
 }
 
+// ---- binary design files (src/core/binaryIO.cc:3-34): PETSc's binary Vec format ----
+inline PetscErrorCode writeVectorToBinary(const std::string& folder, const std::string& name,
+                                          const std::vector<double>& v) {
+  return lbmx_vec_write_binary((folder + name).c_str(), v.data(), v.size());
+}
+inline PetscErrorCode loadVectorFromBinary(const std::string& folder, const std::string& name,
+                                           std::vector<double>& v) {
+  return lbmx_vec_load_binary((folder + name).c_str(), v.data(), v.size());
+}
+
 class AdjointLBSolver;
 
 // ---- solvers (src/latticeBoltzmann/LBSolver.hh:14-73, NewLBSolver2d.hh, isothermalLBSolver2d.hh) ----
@@ -361,6 +372,10 @@ class LBSolver {
   }
   PetscErrorCode setDistributions(const std::vector<double>& f, PetscInt timestep) {
     return lbmx_set_state(h_, f.data(), f.size(), timestep);
+  }
+  // LBSolver::outputFields (src/latticeBoltzmann/LBSolver.cc:251-286): macros and design, .vts
+  PetscErrorCode outputFields(const std::string& folder, const std::string& name) const {
+    return lbmx_write_vts(h_, (folder + name).c_str());
   }
   // LBSolver::makeSourceAdjoint: the adjoint solver shares this engine's adjoint fields
   PetscErrorCode makeSourceAdjoint(AdjointLBSolver* adj);

@@ -172,6 +172,9 @@ def library():
         "lbmx_filter_solver_sensitivities": [C.c_void_p, C.c_void_p, dp, C.c_size_t],
         "lbmx_filter_get_field": [C.c_void_p, C.c_int, dp, C.c_size_t],
         "lbmx_filter_timing": [C.c_void_p, dp],
+        "lbmx_vec_write_binary": [C.c_char_p, dp, C.c_size_t],
+        "lbmx_vec_load_binary": [C.c_char_p, dp, C.c_size_t],
+        "lbmx_write_vts": [C.c_void_p, C.c_char_p],
         "lbmx_mma_create": [C.c_longlong, C.c_int, P(C.c_void_p)],
         "lbmx_mma_destroy": [C.c_void_p],
         "lbmx_mma_set_robust_constants": [C.c_void_p, dp],
@@ -202,7 +205,8 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_batch_device_bytes", "lbmx_filter_create", "lbmx_filter_destroy", "lbmx_filter_set_projection",
             "lbmx_filter_set_eta_field", "lbmx_filter_increase_beta", "lbmx_filter_set_background", "lbmx_filter_design",
             "lbmx_filter_design_to_solver", "lbmx_filter_sensitivities", "lbmx_filter_solver_sensitivities",
-            "lbmx_filter_get_field", "lbmx_filter_timing", "lbmx_mma_create", "lbmx_mma_destroy",
+            "lbmx_filter_get_field", "lbmx_filter_timing", "lbmx_vec_write_binary", "lbmx_vec_load_binary",
+            "lbmx_write_vts", "lbmx_mma_create", "lbmx_mma_destroy",
             "lbmx_mma_set_robust_constants", "lbmx_mma_set_asymptotes", "lbmx_mma_set_options", "lbmx_mma_update",
             "lbmx_mma_design_change", "lbmx_mma_get_state"]
 
@@ -227,6 +231,19 @@ def stream_copy_gbs(nbytes=1 << 31, iters=20):
     g = C.c_double(0.0)
     _check(library().lbmx_stream_copy(nbytes, iters, C.byref(g)))
     return g.value
+
+
+def write_petsc_vec(path, v):
+    """Write v in PETSc's binary Vec format (writeVectorToBinary)."""
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    _check(library().lbmx_vec_write_binary(os.fsencode(path), _dptr(a), a.size))
+
+
+def load_petsc_vec(path, n):
+    """Read n values from a PETSc binary Vec file (loadVectorFromBinary)."""
+    a = np.empty(n)
+    _check(library().lbmx_vec_load_binary(os.fsencode(path), _dptr(a), n))
+    return a
 
 
 def plane_copy_gbs(planes, nodes, iters=20):
@@ -353,6 +370,10 @@ class Solver:
         st = Stats()
         _check(self.lib.lbmx_get_stats(self.h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def write_vts(self, path):
+        """Macros and design of the current state as a VTK StructuredGrid (outputFields)."""
+        _check(self.lib.lbmx_write_vts(self.h, os.fsencode(path)))
 
     def time_sweeps(self, kind, nsweeps):
         """Device time (ms, HIP events) of nsweeps forward (kind 0) or adjoint (kind 1) sweeps."""

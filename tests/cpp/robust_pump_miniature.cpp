@@ -98,6 +98,12 @@ int main() {
     std::printf("iteration %d J %.12e volume %.6f change %.6f\n", iter, J, vol + 0.5, ch);
   }
 
+  // final design written and read back as the application does for restarts, fields as .vts
+  std::vector<double> reread(designDomain.size(), 0.0);
+  CHK(writeVectorToBinary("", "robust_pump_miniature_design.bin", designDomain));
+  CHK(loadVectorFromBinary("", "robust_pump_miniature_design.bin", reread));
+  const bool restart_ok = reread == designDomain;
+
   // per-step objective (collideAndStream + computeMacroFunctional) against the fused pass
   CHK(topOpt.computeObjective());
   const double Jfused = topOpt.getObjective();
@@ -110,9 +116,11 @@ int main() {
     Jsteps += v;
   }
   const double rel = std::fabs(Jsteps - Jfused) / std::fabs(Jfused);
+  CHK(solver.outputFields("", "robust_pump_miniature_fields.vts"));  // the state after the last step
   std::printf("per-step objective %.15e fused %.15e rel %.3e\n", Jsteps, Jfused, rel);
   std::printf("pressure drop %.6e -> %.6e monotone %d volume %.6f\n", J0, Jlast, monotone ? 1 : 0, volLast + 0.5);
-  const bool ok = rel < 1e-12 && Jlast < 0.95 * J0 && monotone && volLast <= 0.02;
+  std::printf("design restart file round trip %d\n", restart_ok ? 1 : 0);
+  const bool ok = rel < 1e-12 && Jlast < 0.95 * J0 && monotone && volLast <= 0.02 && restart_ok;
   std::printf(ok ? "RESULT OK\n" : "RESULT FAIL\n");
   return ok ? 0 : 2;
 }

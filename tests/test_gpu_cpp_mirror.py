@@ -14,12 +14,14 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "robust_pump_miniature")
 
 
-def test_robust_pump_miniature_through_the_mirror(lbmx_mod):
+def test_robust_pump_miniature_through_the_mirror(lbmx_mod, tmp_path):
     if not os.path.exists(EXE):
         raise RuntimeError(f"{EXE} not built (run __graft_entry__.build())")
-    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
     assert "RESULT OK" in r.stdout
     lines = [ln for ln in r.stdout.splitlines() if ln.startswith("iteration")]
     assert len(lines) == 10
+    assert (tmp_path / "robust_pump_miniature_design.bin").exists()
+    assert (tmp_path / "robust_pump_miniature_fields.vts").exists()

@@ -48,6 +48,28 @@ def test_host_mirror_header_compiles():
     assert r.returncode == 0, r.stderr
 
 
+def test_petsc_binary_vec_format(tmp_path):
+    """writeVectorToBinary / loadVectorFromBinary: PETSc's binary Vec layout, big-endian int32
+    class id 1211214, int32 length, big-endian FP64 values (the byte string is built here
+    from that description, not by the library)."""
+    import struct
+    v = np.random.default_rng(8).standard_normal(37)
+    path = str(tmp_path / "Design_variables_iter_0.bin")
+    lbmx.write_petsc_vec(path, v)
+    raw = open(path, "rb").read()
+    assert raw == struct.pack(">ii", 1211214, v.size) + v.astype(">f8").tobytes()
+    assert open(path + ".info").read() == "-vecload_block_size 1\n"
+    assert np.array_equal(lbmx.load_petsc_vec(path, v.size), v)
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.load_petsc_vec(path, v.size + 1)                     # length mismatch
+    open(path, "wb").write(struct.pack(">ii", 1211215, 2) + b"\0" * 16)
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.load_petsc_vec(path, 2)                              # wrong class id
+    open(path, "wb").write(struct.pack(">ii", 1211214, 4) + b"\0" * 16)
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.load_petsc_vec(path, 4)                              # truncated data
+
+
 def test_error_paths_without_device():
     lib = lbmx.library()
     import ctypes as C
