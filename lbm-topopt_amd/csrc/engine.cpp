@@ -755,10 +755,11 @@ struct Engine {
             const int xo = x + ex[q], yo = y + ey[q], zo = z + ez[q];
             const bool out_dom = (ex[q] || ey[q] || ez[q]) &&
                                  (xo < 0 || xo >= d.nx || yo < 0 || yo >= d.ny || zo < 0 || zo >= d.nz);
-            // pushed into the halo cell x + c_q, plane opp(q), when x + c_q is outside
-            const long long e = out_dom ? (long long)op[q] * ll.plane + off +
-                                              (ex[q] + (long long)ey[q] * ll.px + (long long)ez[q] * ll.px * ll.py)
-                                        : (long long)q * ll.plane + off;
+            // the buffer holds m_1 at own nodes: nu_0(x)[q] = m_1(x + c_q)[q], or the mid-way
+            // reflection m_1(x)[opp q] when x + c_q is outside
+            const long long e = out_dom ? (long long)op[q] * ll.plane + off
+                                        : (long long)q * ll.plane + off +
+                                              (ex[q] + (long long)ey[q] * ll.px + (long long)ez[q] * ll.px * ll.py);
             out[node * Q + q] = val(e);
           }
         }

@@ -10,10 +10,10 @@
 //
 //  k_fwd : pull p_{t-1}, boundary pieces (insertion order), functional of s_t, collision,
 //          store p_t (aligned) + halo reflections at boundary nodes.
-//  k_adj : recompute s_t = B_t S p_{t-1}, read nu_t (own node; halo at boundary nodes),
-//          reverse collision, functional source, transposed boundaries (reverse order),
-//          push nu_{t-1} to the pull addresses (the transposed stream, halo included);
-//          dJ/dgamma accumulated in FP64.
+//  k_adj : recompute s_t = B_t S p_{t-1}, gather nu_t = S^T m_t from the previous adjoint
+//          sweep's m_t (at x + c_q; the own node's opposite plane where x + c_q is outside),
+//          reverse collision, functional source, transposed boundaries (reverse order), store
+//          m_{t-1} at the own node; dJ/dgamma accumulated in FP64.
 // Workgroups are 64 (x) x 4 (y) threads; the linear workgroup id is remapped so that each
 // XCD (workgroups are dealt round-robin to the 8 XCDs) receives a contiguous run of tiles and
 // x-neighbouring tiles share that XCD's L2 for the lines straddling their edge.
@@ -37,15 +37,11 @@ __device__ __forceinline__ void st(char* base, unsigned int off_bytes, S v) {
 }
 // Stores of whole aligned rows that nothing reads before the next sweep bypass the caches'
 // retention (non-temporal): on MI355X this leaves L2 to the lines that shifted pulls share
-// between neighbouring workgroups. Partial-line stores (x-shifted pushes) stay ordinary, so
-// neighbouring waves still merge them in L2.
+// between neighbouring workgroups. Every sweep writes only its own nodes, so all its stores
+// are whole rows; the shifts are all on the read side.
 template <class S>
 __device__ __forceinline__ void st_nt(char* base, unsigned int off_bytes, S v) {
   __builtin_nontemporal_store(v, reinterpret_cast<S*>(base + off_bytes));
-}
-template <class S>
-__device__ __forceinline__ S ld_nt(const char* base, unsigned int off_bytes) {
-  return __builtin_nontemporal_load(reinterpret_cast<const S*>(base + off_bytes));
 }
 
 // pull address base of plane q: start + (q*plane - shift_q) elements
@@ -56,6 +52,11 @@ __device__ __forceinline__ const char* pull_base(const void* start, const Layout
 template <class L, int q, class S>
 __device__ __forceinline__ char* pull_base_w(void* start, const Layout& l) {
   return static_cast<char*>(start) + ((long long)q * l.plane - shift_of<L, q>(l)) * (long long)sizeof(S);
+}
+// gather address base of plane q: start + (q*plane + shift_q) elements (node x + c_q)
+template <class L, int q, class S>
+__device__ __forceinline__ const char* gather_base(const void* start, const Layout& l) {
+  return static_cast<const char*>(start) + ((long long)q * l.plane + shift_of<L, q>(l)) * (long long)sizeof(S);
 }
 template <int q, class S>
 __device__ __forceinline__ char* own_base_w(void* start, const Layout& l) {
@@ -332,18 +333,20 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   pull_node<L, P>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
   if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, tfac, x, y, z);
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
+  // The adjoint field between sweeps is m_t, the boundary-corrected adjoint of s_t before the
+  // transposed stream, stored at its own node. The transposed stream is gathered here:
+  // nu_t(x)[q] = m_t(x + c_q)[q], or the mid-way reflection m_t(x)[opp q] when x + c_q lies
+  // outside the domain (then the gathered halo value is replaced).
   T lam[L::Q];
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    lam[q] = T(ld_nt<LS>(own_base<q, LS>(a.lam_in, a.ll), offl));
+    lam[q] = T(ld<LS>(gather_base<L, q, LS>(a.lam_in, a.ll), offl));
   });
   if (bnd) {
-    // nu_t(x)[r] for x + c_r outside was pushed by this node into the halo cell x + c_r,
-    // plane opp(r): the pull address of plane opp(r)
     static_for<1, L::Q>([&](auto qc) {
       constexpr int r = decltype(qc)::value;
       if constexpr (L::ex[r] != 0 || L::ey[r] != 0 || L::ez[r] != 0) {
-        if (leaves<L, r>(x, y, z, nx, ny, nz)) lam[r] = T(ld<LS>(pull_base<L, L::opp[r], LS>(a.lam_in, a.ll), offl));
+        if (leaves<L, r>(x, y, z, nx, ny, nz)) lam[r] = T(ld<LS>(own_base<L::opp[r], LS>(a.lam_in, a.ll), offl));
       }
     });
   }
@@ -363,12 +366,10 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   reverse_collide<L, COLL, DES, T>(f, gam, a.m, lam, dg, srcA, srcB, srcAT);
   __builtin_nontemporal_store(__builtin_nontemporal_load(a.sens + n) + double(dg), a.sens + n);
   if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, a.nbc, x, y, z);
-  // transposed stream: nu_{t-1}(x - c_q)[q] = lam[q], i.e. a store to the pull address (whole
-  // rows when c_q has no x component)
+  // m_{t-1} at the own node (whole rows); the next adjoint sweep gathers its transposed stream
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    if constexpr (L::ex[q] == 0) st_nt<LS>(pull_base_w<L, q, LS>(a.lam_out, a.ll), offl, LS(lam[q]));
-    else st<LS>(pull_base_w<L, q, LS>(a.lam_out, a.ll), offl, LS(lam[q]));
+    st_nt<LS>(own_base_w<q, LS>(a.lam_out, a.ll), offl, LS(lam[q]));
   });
 }
 
