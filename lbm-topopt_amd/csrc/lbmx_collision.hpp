@@ -94,15 +94,24 @@ __device__ __forceinline__ T pair_moment(const Pairs<L, T>& p) {
   static_assert(LBMX_BAS.par[k] != 0, "basis row without parity");
   T m = 0;
   if constexpr (LBMX_BAS.par[k] == 1) {
-    if constexpr (LBMX_BAS.M[k][0] != 0) m += T(LBMX_BAS.M[k][0]) * p.f0;
+    if constexpr (LBMX_BAS.M[k][0] != 0) {
+      constexpr T kc = T(LBMX_BAS.M[k][0]);
+      m += kc * p.f0;
+    }
     static_for<1, L::HALF + 1>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      if constexpr (LBMX_BAS.M[k][d] != 0) m += T(LBMX_BAS.M[k][d]) * p.s[d];
+      if constexpr (LBMX_BAS.M[k][d] != 0) {
+        constexpr T kc = T(LBMX_BAS.M[k][d]);
+        m += kc * p.s[d];
+      }
     });
   } else {
     static_for<1, L::HALF + 1>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      if constexpr (LBMX_BAS.M[k][d] != 0) m += T(LBMX_BAS.M[k][d]) * p.a[d];
+      if constexpr (LBMX_BAS.M[k][d] != 0) {
+        constexpr T kc = T(LBMX_BAS.M[k][d]);
+        m += kc * p.a[d];
+      }
     });
   }
   return m;
@@ -112,15 +121,24 @@ __device__ __forceinline__ T pair_moment(const Pairs<L, T>& p) {
 template <class L, int k, class T>
 __device__ __forceinline__ void pair_row_update(Pairs<L, T>& x, T c) {
   if constexpr (LBMX_BAS.par[k] == 1) {
-    if constexpr (LBMX_BAS.M[k][0] != 0) x.f0 -= T(LBMX_BAS.M[k][0]) * c;
+    if constexpr (LBMX_BAS.M[k][0] != 0) {
+      constexpr T kc = T(LBMX_BAS.M[k][0]);
+      x.f0 -= kc * c;
+    }
     static_for<1, L::HALF + 1>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      if constexpr (LBMX_BAS.M[k][d] != 0) x.s[d] -= T(2.0 * LBMX_BAS.M[k][d]) * c;
+      if constexpr (LBMX_BAS.M[k][d] != 0) {
+        constexpr T kc = T(2.0 * LBMX_BAS.M[k][d]);
+        x.s[d] -= kc * c;
+      }
     });
   } else {
     static_for<1, L::HALF + 1>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
-      if constexpr (LBMX_BAS.M[k][d] != 0) x.a[d] -= T(2.0 * LBMX_BAS.M[k][d]) * c;
+      if constexpr (LBMX_BAS.M[k][d] != 0) {
+        constexpr T kc = T(2.0 * LBMX_BAS.M[k][d]);
+        x.a[d] -= kc * c;
+      }
     });
   }
 }
@@ -129,17 +147,26 @@ __device__ __forceinline__ void pair_row_update(Pairs<L, T>& x, T c) {
 template <class L, int k, class T>
 __device__ __forceinline__ T meq_row(T rho, const T (&v)[3], T inv_rho_s) {
   T m = 0;
-  if constexpr (LBMX_BAS.a[k] != 0) m += T(LBMX_BAS.a[k]) * rho;
+  if constexpr (LBMX_BAS.a[k] != 0) {
+    constexpr T kc = T(LBMX_BAS.a[k]);
+    m += kc * rho;
+  }
   static_for<0, 3>([&](auto ac) {
     constexpr int a = decltype(ac)::value;
-    if constexpr (LBMX_BAS.b[k][a] != 0) m += T(LBMX_BAS.b[k][a]) * v[a];
+    if constexpr (LBMX_BAS.b[k][a] != 0) {
+      constexpr T kc = T(LBMX_BAS.b[k][a]);
+      m += kc * v[a];
+    }
   });
   T quad = 0;
   static_for<0, 3>([&](auto ac) {
     constexpr int a = decltype(ac)::value;
     static_for<0, 3>([&](auto bc) {
       constexpr int b = decltype(bc)::value;
-      if constexpr (LBMX_BAS.C[k][a][b] != 0) quad += T(LBMX_BAS.C[k][a][b]) * v[a] * v[b];
+      if constexpr (LBMX_BAS.C[k][a][b] != 0) {
+        constexpr T kc = T(LBMX_BAS.C[k][a][b]);
+        quad += kc * v[a] * v[b];
+      }
     });
   });
   return m + quad * inv_rho_s;
@@ -149,13 +176,22 @@ __device__ __forceinline__ T meq_row(T rho, const T (&v)[3], T inv_rho_s) {
 template <class L, int k, class T>
 __device__ __forceinline__ T src_row(const T (&v)[3], T u2) {
   T p = 0;
-  if constexpr (LBMX_BAS.G0[k] != 0) p -= T(3.0 * LBMX_BAS.G0[k]) * u2;
+  if constexpr (LBMX_BAS.G0[k] != 0) {
+    constexpr T kc = T(3.0 * LBMX_BAS.G0[k]);
+    p -= kc * u2;
+  }
   static_for<0, 3>([&](auto ac) {
     constexpr int a = decltype(ac)::value;
-    if constexpr (LBMX_BAS.G1[k][a] != 0) p += T(3.0 * LBMX_BAS.G1[k][a]) * v[a];
+    if constexpr (LBMX_BAS.G1[k][a] != 0) {
+      constexpr T kc = T(3.0 * LBMX_BAS.G1[k][a]);
+      p += kc * v[a];
+    }
     static_for<0, 3>([&](auto bc) {
       constexpr int b = decltype(bc)::value;
-      if constexpr (LBMX_BAS.G2[k][a][b] != 0) p += T(9.0 * LBMX_BAS.G2[k][a][b]) * v[a] * v[b];
+      if constexpr (LBMX_BAS.G2[k][a][b] != 0) {
+        constexpr T kc = T(9.0 * LBMX_BAS.G2[k][a][b]);
+        p += kc * v[a] * v[b];
+      }
     });
   });
   return p;
@@ -211,13 +247,16 @@ __device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3
       constexpr int k = decltype(kc)::value;
       if constexpr (LBMX_BAS.rate[k] != 0) {
         T mk = pair_moment<L, k>(p);
-        T sk = LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]);
+        constexpr T s_k = T(LBMX_BAS.s[k]), inv_n2 = T(LBMX_BAS.invn2[k]);
+        constexpr bool own_rate = LBMX_BAS.rate[k] == 1;
+        T sk = own_rate ? omega : s_k;
         T val = sk * (mk - meq_row<L, k>(rho, vm, inv_rho_s));
         if constexpr (DES == D_BRINK && row_has_src<L, k>())
           val += (T(1) - T(0.5) * sk) * alpha * src_row<L, k>(v, u2);
-        pair_row_update<L, k>(p, val * T(LBMX_BAS.invn2[k]));
+        pair_row_update<L, k>(p, val * inv_n2);
       } else if constexpr (DES == D_BRINK && row_has_src<L, k>()) {
-        pair_row_update<L, k>(p, alpha * src_row<L, k>(v, u2) * T(LBMX_BAS.invn2[k]));
+        constexpr T inv_n2 = T(LBMX_BAS.invn2[k]);
+        pair_row_update<L, k>(p, alpha * src_row<L, k>(v, u2) * inv_n2);
       }
     });
   }
@@ -554,8 +593,10 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
       static_for<0, Q>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         if constexpr (LBMX_BAS.rate[k] != 0 && LBMX_BAS.par[k] == -1) {
-          T sk = LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]);
-          T dm = sk * (pair_moment<L, k>(g) - meq_row<L, k>(rho, v, T(1))) * T(LBMX_BAS.invn2[k]);
+          constexpr T s_k = T(LBMX_BAS.s[k]), inv_n2 = T(LBMX_BAS.invn2[k]);
+          constexpr bool own_rate = LBMX_BAS.rate[k] == 1;
+        T sk = own_rate ? omega : s_k;
+          T dm = sk * (pair_moment<L, k>(g) - meq_row<L, k>(rho, v, T(1))) * inv_n2;
           pair_row_update<L, k>(g, dm);
         }
       });
@@ -615,15 +656,23 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
     static_for<0, Q>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
       if constexpr (row_active<L, DES, k>()) {
-        const T ak = pair_moment<L, k>(pl) * T(LBMX_BAS.invn2[k]);
-        const T sk = LBMX_BAS.rate[k] == 0 ? T(0) : (LBMX_BAS.rate[k] == 1 ? omega : T(LBMX_BAS.s[k]));
+        constexpr T s_k = T(LBMX_BAS.s[k]), inv_n2 = T(LBMX_BAS.invn2[k]);
+        const T ak = pair_moment<L, k>(pl) * inv_n2;
+        constexpr int rate = LBMX_BAS.rate[k];
+        const T sk = rate == 0 ? T(0) : (rate == 1 ? omega : s_k);
         if constexpr (LBMX_BAS.rate[k] != 0) {
           const T b = sk * ak;
           pair_row_update<L, k>(pl, b);
-          if constexpr (LBMX_BAS.a[k] != 0) rho_bar += b * T(LBMX_BAS.a[k]);
+          if constexpr (LBMX_BAS.a[k] != 0) {
+            constexpr T kc = T(LBMX_BAS.a[k]);
+            rho_bar += b * kc;
+          }
           static_for<0, 3>([&](auto ac) {
             constexpr int a = decltype(ac)::value;
-            if constexpr (LBMX_BAS.b[k][a] != 0) vb[a] += b * T(LBMX_BAS.b[k][a]);
+            if constexpr (LBMX_BAS.b[k][a] != 0) {
+              constexpr T kc = T(LBMX_BAS.b[k][a]);
+              vb[a] += b * kc;
+            }
             static_for<0, 3>([&](auto cc) {
               constexpr int c = decltype(cc)::value;
               constexpr double csum = LBMX_BAS.C[k][a][c] + LBMX_BAS.C[k][c][a];
@@ -637,13 +686,17 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
           alpha_bar += pbar * src_row<L, k>(v, u2);
           const T pa = pbar * alpha;
           if constexpr (LBMX_BAS.G0[k] != 0) {
-            vb[0] -= pa * T(6.0 * LBMX_BAS.G0[k]) * v[0];
-            vb[1] -= pa * T(6.0 * LBMX_BAS.G0[k]) * v[1];
-            vb[2] -= pa * T(6.0 * LBMX_BAS.G0[k]) * v[2];
+            constexpr T kc = T(6.0 * LBMX_BAS.G0[k]);
+            vb[0] -= pa * kc * v[0];
+            vb[1] -= pa * kc * v[1];
+            vb[2] -= pa * kc * v[2];
           }
           static_for<0, 3>([&](auto ac) {
             constexpr int a = decltype(ac)::value;
-            if constexpr (LBMX_BAS.G1[k][a] != 0) vb[a] += pa * T(3.0 * LBMX_BAS.G1[k][a]);
+            if constexpr (LBMX_BAS.G1[k][a] != 0) {
+              constexpr T kc = T(3.0 * LBMX_BAS.G1[k][a]);
+              vb[a] += pa * kc;
+            }
             static_for<0, 3>([&](auto cc) {
               constexpr int c = decltype(cc)::value;
               constexpr double gs = LBMX_BAS.G2[k][a][c] + LBMX_BAS.G2[k][c][a];

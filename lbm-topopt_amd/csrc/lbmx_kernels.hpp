@@ -198,6 +198,15 @@ __device__ __forceinline__ bool tile_meets(const int* bb, int x0, int y0, int z)
   return !(x1 < bb[0] || x0 > bb[1] || y1 < bb[2] || y0 > bb[3] || z < bb[4] || z > bb[5]);
 }
 
+// the workgroup tile holds nodes of some functional term (uniform over the workgroup)
+template <int TY>
+__device__ __forceinline__ bool tile_meets_terms(const ObjArgs& o, int x0, int y0, int z) {
+  if (!tile_meets<TY>(o.bbox, x0, y0, z)) return false;
+  for (int m = 0; m < o.nterm; ++m)
+    if (tile_meets<TY>(o.terms[m].box, x0, y0, z)) return true;
+  return false;
+}
+
 template <int TY>
 __device__ __forceinline__ double block_sum(double v) {
   __shared__ double part[TY];
@@ -229,7 +238,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool active = x < nx && y < ny;
   const bool objblk = a.obj.acc != nullptr && a.mode == 0 &&
-                      tile_meets<TY>(a.obj.bbox, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
+                      tile_meets_terms<TY>(a.obj, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
   double objv = 0.0;
   if (active) {
     const unsigned int off = layout_off(a.ls, x, y, z);
