@@ -228,7 +228,8 @@ def run_mma(lbmx, cpu=True):
     applications/RobustPump2d/robustPump2d.cc:173-190 (three realisations with a_j = 1 and a
     volume constraint with a_j = 0) on synthetic smooth responses, at the 1024 x 512 and the
     384 x 192 x 192 design sizes. Wall milliseconds per update (host round trips of the dual
-    Newton iteration included); the oracle's single-thread update as CPU baseline."""
+    Newton iteration included), with host arrays and with device arrays (lbmx_mma_update_device);
+    the oracle's single-thread update as CPU baseline."""
     out = {}
 
     def inputs(x, targets, weights):
@@ -237,23 +238,36 @@ def run_mma(lbmx, cpu=True):
         dg = [2 * w * (x - t) / n for t, w in zip(targets, weights)] + [np.full(n, 1.0 / n)]
         return np.zeros(n), np.array(g), np.stack(dg)
 
+    import torch
     rng = np.random.default_rng(0)
     for name, n in (("1024x512", 524288), ("384x192x192", 14155776)):
         targets = [0.2 + 0.6 * rng.random(n) for _ in range(3)]
         w = [1.0, 1.3, 0.8]
         h = lbmx.MMA(n, 4, a=[1.0, 1.0, 1.0, 0.0])
+        hd = lbmx.MMA(n, 4, a=[1.0, 1.0, 1.0, 0.0])   # the same updates on device arrays
         x = np.full(n, 0.5)
-        ms, its = [], []
+        xd = torch.from_numpy(x).to("cuda")
+        ms, its, msd, same = [], [], [], True
         for _ in range(4):
-            x = h.update(x, *inputs(x, targets, w), 0.1)
+            dfdx, g, dg = inputs(x, targets, w)
+            dfdx_d = torch.from_numpy(dfdx).to("cuda")
+            dg_d = [torch.from_numpy(r).to("cuda") for r in dg]
+            x = h.update(x, dfdx, g, dg, 0.1)
             st = h.state()
             ms.append(st["ms"])
             its.append(st["newton_iters"])
+            hd.update_device(xd, dfdx_d, g, dg_d, 0.1)
+            msd.append(hd.state()["ms"])
+            same = same and bool(np.array_equal(xd.cpu().numpy(), x))
+            del dfdx_d, dg_d
         h.close()
+        hd.close()
         out[name] = {"n": n, "m": 4, "update_ms": [round(v, 3) for v in ms], "newton_iters": its,
-                     "ms_per_newton_iter": round(float(np.median(np.array(ms) / np.array(its))), 4)}
-        log(f"mma {name}: {', '.join(f'{v:.1f}' for v in ms)} ms per update ({its} dual Newton iterations)")
-        del targets
+                     "ms_per_newton_iter": round(float(np.median(np.array(ms) / np.array(its))), 4),
+                     "device_arrays_update_ms": [round(v, 3) for v in msd], "device_arrays_bitwise_equal": same}
+        log(f"mma {name}: {', '.join(f'{v:.1f}' for v in ms)} ms per update ({its} dual Newton iterations); "
+            f"device arrays {', '.join(f'{v:.1f}' for v in msd)} ms, bitwise equal {same}")
+        del targets, xd
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # noqa: E402  (baseline leg only)

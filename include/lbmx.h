@@ -210,10 +210,35 @@ lbmx_status lbmx_checkpoint_config(lbmx_solver* s, int nslots);
    recomputation. J (may be NULL) = objective; sens (may be NULL, n = nodes) = dJ/dgamma. */
 lbmx_status lbmx_gradient(lbmx_solver* s, int nsteps, double rho0, const double u0[3], double T0,
                           double* J, double* sens, size_t n);
-/* AdjointLBSolver::getSensitivities after the last lbmx_gradient (the sensitivityGlobal Vec). */
+/* AdjointLBSolver::getSensitivities after the last lbmx_gradient or lbmx_adjoint_step (the
+   sensitivityGlobal Vec). */
 lbmx_status lbmx_get_sensitivity(lbmx_solver* s, double* sens, size_t n);
-/* The adjoint distributions after the last lbmx_gradient (adjointGlobal), AoS [node][q]. */
+/* The adjoint distributions after the last lbmx_gradient or lbmx_adjoint_step (adjointGlobal),
+   AoS [node][q]. */
 lbmx_status lbmx_get_adjoint(lbmx_solver* s, double* adj, size_t n);
+
+/* The reverse loop step by step, for callers that drive the forward states themselves (their
+   own checkpointing, a CheckpointingBase of src/topOpt/checkpointingBase.hh:7-18).
+   AdjointLBSolver::resetAdjoints + setCurrentTimestep (src/adjointLatticeBoltzmann/adjointLBSolver.cc:138-145):
+   zeroes the adjoint distributions and dJ/dgamma; the next reverse step is `timestep`. */
+lbmx_status lbmx_adjoint_reset(lbmx_solver* s, int timestep);
+/* AdjointLBSolver::setCurrentTimestep (src/adjointLatticeBoltzmann/adjointLBSolver.cc:147-151):
+   the next reverse step, after lbmx_adjoint_reset. */
+lbmx_status lbmx_adjoint_set_timestep(lbmx_solver* s, int timestep);
+/* One reverse step t: adjointCollide (src/adjointLatticeBoltzmann/adjointLBSolver.cc:28-48),
+   computeCollideSource of the functional set by lbmx_set_objective when with_source != 0
+   (:83-102) and adjointStream (:50-81), in the order of applications/RobustPump2d/robustPump2d.cc:268-276;
+   dJ/dgamma accumulates; then t := t - 1. The forward state must be the state of step t:
+   after lbmx_forward reached t, or lbmx_set_state(..., t) (getForwardSolution); else
+   LBMX_ERR_STATE. Step 0 (the initial state) is never reversed. */
+lbmx_status lbmx_adjoint_step(lbmx_solver* s, int with_source);
+
+/* Device-resident hand-off (HIP device pointers, FP64, natural node order). Both calls return
+   with the device work done. setFieldFromDesignVec from a device array: */
+lbmx_status lbmx_set_design_device(lbmx_solver* s, const double* gamma_dev, size_t n);
+/* getSensitivities as a borrowed handle (src/adjointLatticeBoltzmann/adjointLBSolver.cc:153-157):
+   the solver's own dJ/dgamma array, valid until the next gradient, reset or destroy. */
+lbmx_status lbmx_get_sensitivity_device(lbmx_solver* s, const double** sens_dev, size_t* n);
 
 /* ---- instrumentation ---- */
 typedef struct {
@@ -344,6 +369,15 @@ lbmx_status lbmx_filter_sensitivities(lbmx_filter* f, const double* full_sens, s
                                       size_t nd);
 /* The same on the solver's device-resident dJ/dgamma of its last lbmx_gradient. */
 lbmx_status lbmx_filter_solver_sensitivities(lbmx_filter* f, lbmx_solver* s, double* design_sens, size_t nd);
+/* The four calls above on device arrays: every double* argument is a HIP device pointer
+   (physical may still be NULL). The design and its sensitivities then stay in HBM through
+   the whole optimisation iteration. Inputs must be ready when the call is made; the calls
+   return with the device work done. */
+lbmx_status lbmx_filter_design_device(lbmx_filter* f, const double* design, size_t nd, double* physical, size_t nf);
+lbmx_status lbmx_filter_design_to_solver_device(lbmx_filter* f, const double* design, size_t nd, lbmx_solver* s);
+lbmx_status lbmx_filter_sensitivities_device(lbmx_filter* f, const double* full_sens, size_t nf, double* design_sens,
+                                             size_t nd);
+lbmx_status lbmx_filter_solver_sensitivities_device(lbmx_filter* f, lbmx_solver* s, double* design_sens, size_t nd);
 /* Intermediate fields of the last lbmx_filter_design (outputIntermediateFields,
    projectionFilter2d.cc:89-103): which = 2*stage for the stage's density-filtered field,
    2*stage + 1 for its output; n = full nodes. */
@@ -389,6 +423,12 @@ lbmx_status lbmx_mma_update(lbmx_mma* h, double* x, const double* dfdx, const do
 /* MMA::DesignChange (MMA.cc:389-414): *ch = max |xnew - x before the last update|; xnew = NULL
    uses the design of the last update */
 lbmx_status lbmx_mma_design_change(lbmx_mma* h, const double* xnew, double* ch);
+/* The same two calls on device arrays (x, dfdx and every dgdx[j] are HIP device pointers; gx
+   stays a host array of m values): the optimisation loop keeps the design and its
+   sensitivities in HBM. They return with the device work done. */
+lbmx_status lbmx_mma_update_device(lbmx_mma* h, double* x, const double* dfdx, const double* gx,
+                                   const double* const* dgdx, double movlim);
+lbmx_status lbmx_mma_design_change_device(lbmx_mma* h, const double* xnew, double* ch);
 /* Diagnostics: asymptotes L[n], U[n], multipliers lam[m], dual Newton iterations and wall
    milliseconds of the last update (any pointer may be NULL). */
 lbmx_status lbmx_mma_get_state(lbmx_mma* h, double* L, double* U, double* lam, int* newton_iters, double* ms);

@@ -57,7 +57,7 @@ def build(verbose=True):
 
 
 HOST = os.path.join(HERE, "host")
-EXAMPLES = [os.path.join(ROOT, "tests", "cpp", "robust_pump_miniature.cpp")]
+EXAMPLES = [os.path.join(ROOT, "tests", "cpp", name) for name in ("robust_pump_miniature.cpp", "unfused_reverse_loop.cpp")]
 
 
 def build_host_examples(verbose=True):

@@ -147,12 +147,16 @@ struct Filter {
     tsum = tb + std::tanh(st.beta * (1.0 - st.eta));
   }
 
-  // mapDesignToFull + filterDesign of every stage (the last stage's output is the physical field)
-  const double* run_design(const double* design_host, hipStream_t s) {
-    HIP_OK(hipMemcpyAsync(design_in.p, design_host, (size_t)ND * 8, hipMemcpyHostToDevice, s));
+  // mapDesignToFull + filterDesign of every stage (the last stage's output is the physical
+  // field); the design is a host array, or a device array read in place when dev is set
+  const double* run_design(const double* design, bool dev, hipStream_t s) {
+    const double* din = design;
+    if (!dev) {
+      HIP_OK(hipMemcpyAsync(design_in.p, design, (size_t)ND * 8, hipMemcpyHostToDevice, s));
+      din = design_in.as<double>();
+    }
     HIP_OK(hipEventRecord(ev[0], s));
-    HIP_OK(launch_map_to_full(full0.as<double>(), design_in.as<double>(), background.as<double>(), d.nx, d.ny, d.nz,
-                              dbox, s));
+    HIP_OK(launch_map_to_full(full0.as<double>(), din, background.as<double>(), d.nx, d.ny, d.nz, dbox, s));
     const double* x = full0.as<double>();
     for (auto& st : stages) {
       HIP_OK(launch_cone(cone_args(*st, x, st->dens.as<double>(), false), s));
@@ -169,8 +173,9 @@ struct Filter {
     return x;
   }
 
-  // filterSensitivities (stages in reverse) + mapFullToDesign; sens_dev is only read
-  void run_sensitivities(const double* sens_dev, double* out_host, hipStream_t s) {
+  // filterSensitivities (stages in reverse) + mapFullToDesign; sens_dev is only read. The
+  // result goes to a host array, or straight into a device array when out_dev is set.
+  void run_sensitivities(const double* sens_dev, double* out, bool out_dev, hipStream_t s) {
     for (auto& st : stages)
       if (st->project && !have_design)
         throw Error{LBMX_ERR_STATE, "a projecting filter needs lbmx_filter_design before its sensitivities"};
@@ -185,9 +190,9 @@ struct Filter {
       HIP_OK(launch_cone(cone_args(st, tmp.as<double>(), acc.as<double>(), true), s));
       cur = acc.as<double>();
     }
-    HIP_OK(launch_extract_box(design_out.as<double>(), cur, d.nx, d.ny, d.nz, dbox, s));
+    HIP_OK(launch_extract_box(out_dev ? out : design_out.as<double>(), cur, d.nx, d.ny, d.nz, dbox, s));
     HIP_OK(hipEventRecord(ev[1], s));
-    HIP_OK(hipMemcpyAsync(out_host, design_out.p, (size_t)ND * 8, hipMemcpyDeviceToHost, s));
+    if (!out_dev) HIP_OK(hipMemcpyAsync(out, design_out.p, (size_t)ND * 8, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     float t = 0;
     HIP_OK(hipEventElapsedTime(&t, ev[0], ev[1]));
@@ -281,64 +286,106 @@ lbmx_status lbmx_filter_set_background(lbmx_filter* h, const double* full, size_
   });
 }
 
-lbmx_status lbmx_filter_design(lbmx_filter* h, const double* design, size_t nd, double* physical, size_t nf) {
+}  // extern "C"
+
+namespace {
+
+lbmx_status filter_design(lbmx_filter* h, const double* design, size_t nd, double* physical, size_t nf, bool dev) {
   CHECK_FILTER(h);
   if (!design) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   return guarded([&] {
     lbmx::Filter& F = *h->f;
     F.check_sizes(nd, nf, physical != nullptr);
-    const double* x = F.run_design(design, F.own);
-    if (physical) HIP_OK(hipMemcpyAsync(physical, x, (size_t)F.N * 8, hipMemcpyDeviceToHost, F.own));
+    const double* x = F.run_design(design, dev, F.own);
+    if (physical)
+      HIP_OK(hipMemcpyAsync(physical, x, (size_t)F.N * 8, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                            F.own));
     HIP_OK(hipStreamSynchronize(F.own));
     F.finish_design_timing();
   });
 }
 
-lbmx_status lbmx_filter_design_to_solver(lbmx_filter* h, const double* design, size_t nd, lbmx_solver* s) {
+void check_solver_grid(const lbmx::Filter& F, lbmx_solver* s) {
+  int dims[3];
+  lbmx::solver_nodes(s, dims);
+  if (dims[0] != F.d.nx || dims[1] != F.d.ny || dims[2] != F.d.nz)
+    throw Error{LBMX_ERR_ARG, "solver grid differs from the filter grid"};
+}
+
+lbmx_status filter_design_to_solver(lbmx_filter* h, const double* design, size_t nd, lbmx_solver* s, bool dev) {
   CHECK_FILTER(h);
   if (!design || !s) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   return guarded([&] {
     lbmx::Filter& F = *h->f;
     F.check_sizes(nd, 0, false);
-    int dims[3];
-    lbmx::solver_nodes(s, dims);
-    if (dims[0] != F.d.nx || dims[1] != F.d.ny || dims[2] != F.d.nz)
-      throw Error{LBMX_ERR_ARG, "solver grid differs from the filter grid"};
+    check_solver_grid(F, s);
     hipStream_t st = lbmx::solver_stream(s);
-    const double* x = F.run_design(design, st);
+    const double* x = F.run_design(design, dev, st);
     lbmx::solver_set_design_device(s, x);
     HIP_OK(hipStreamSynchronize(st));
     F.finish_design_timing();
   });
 }
 
-lbmx_status lbmx_filter_sensitivities(lbmx_filter* h, const double* full_sens, size_t nf, double* design_sens,
-                                      size_t nd) {
+lbmx_status filter_sensitivities(lbmx_filter* h, const double* full_sens, size_t nf, double* design_sens, size_t nd,
+                                 bool dev) {
   CHECK_FILTER(h);
   if (!full_sens || !design_sens) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   return guarded([&] {
     lbmx::Filter& F = *h->f;
     F.check_sizes(nd, nf, true);
-    // the input is staged in the accumulation buffer, which the first stage only reads
-    // before the transposed filter overwrites it
-    HIP_OK(hipMemcpyAsync(F.acc.p, full_sens, nf * 8, hipMemcpyHostToDevice, F.own));
-    F.run_sensitivities(F.acc.as<double>(), design_sens, F.own);
+    const double* in = full_sens;
+    if (!dev) {
+      // the host input is staged in the accumulation buffer, which the first stage only
+      // reads before the transposed filter overwrites it
+      HIP_OK(hipMemcpyAsync(F.acc.p, full_sens, nf * 8, hipMemcpyHostToDevice, F.own));
+      in = F.acc.as<double>();
+    }
+    F.run_sensitivities(in, design_sens, dev, F.own);
   });
 }
 
-lbmx_status lbmx_filter_solver_sensitivities(lbmx_filter* h, lbmx_solver* s, double* design_sens, size_t nd) {
+lbmx_status filter_solver_sensitivities(lbmx_filter* h, lbmx_solver* s, double* design_sens, size_t nd, bool dev) {
   CHECK_FILTER(h);
   if (!s || !design_sens) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   return guarded([&] {
     lbmx::Filter& F = *h->f;
     F.check_sizes(nd, 0, false);
-    int dims[3];
-    lbmx::solver_nodes(s, dims);
-    if (dims[0] != F.d.nx || dims[1] != F.d.ny || dims[2] != F.d.nz)
-      throw Error{LBMX_ERR_ARG, "solver grid differs from the filter grid"};
+    check_solver_grid(F, s);
     const double* sd = lbmx::solver_sensitivity_device(s);
-    F.run_sensitivities(sd, design_sens, lbmx::solver_stream(s));
+    F.run_sensitivities(sd, design_sens, dev, lbmx::solver_stream(s));
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+lbmx_status lbmx_filter_design(lbmx_filter* h, const double* design, size_t nd, double* physical, size_t nf) {
+  return filter_design(h, design, nd, physical, nf, false);
+}
+lbmx_status lbmx_filter_design_device(lbmx_filter* h, const double* design, size_t nd, double* physical, size_t nf) {
+  return filter_design(h, design, nd, physical, nf, true);
+}
+lbmx_status lbmx_filter_design_to_solver(lbmx_filter* h, const double* design, size_t nd, lbmx_solver* s) {
+  return filter_design_to_solver(h, design, nd, s, false);
+}
+lbmx_status lbmx_filter_design_to_solver_device(lbmx_filter* h, const double* design, size_t nd, lbmx_solver* s) {
+  return filter_design_to_solver(h, design, nd, s, true);
+}
+lbmx_status lbmx_filter_sensitivities(lbmx_filter* h, const double* full_sens, size_t nf, double* design_sens,
+                                      size_t nd) {
+  return filter_sensitivities(h, full_sens, nf, design_sens, nd, false);
+}
+lbmx_status lbmx_filter_sensitivities_device(lbmx_filter* h, const double* full_sens, size_t nf, double* design_sens,
+                                             size_t nd) {
+  return filter_sensitivities(h, full_sens, nf, design_sens, nd, true);
+}
+lbmx_status lbmx_filter_solver_sensitivities(lbmx_filter* h, lbmx_solver* s, double* design_sens, size_t nd) {
+  return filter_solver_sensitivities(h, s, design_sens, nd, false);
+}
+lbmx_status lbmx_filter_solver_sensitivities_device(lbmx_filter* h, lbmx_solver* s, double* design_sens, size_t nd) {
+  return filter_solver_sensitivities(h, s, design_sens, nd, true);
 }
 
 lbmx_status lbmx_filter_get_field(lbmx_filter* h, int which, double* out, size_t n) {

@@ -143,6 +143,7 @@ struct Engine {
   double init_rho = 1, init_u[3] = {0, 0, 0}, init_T = 0;
   int nslots = 64;
   bool grad_valid = false;
+  int adj_t = -1;        // step of the next per-step reverse sweep (-1: lbmx_adjoint_reset not called)
   lbmx_stats stats{};
 
   explicit Engine(const lbmx_desc& desc) : d(desc) {
@@ -722,9 +723,60 @@ struct Engine {
     stats.reverse_phase_ms = std::chrono::duration<double, std::milli>(tr1 - ops.t_fwd_end).count();
     if (lin != lam_a.p) std::swap(lam_a.p, lam_b.p);  // final adjoint in lam_a
     grad_valid = true;
+    adj_t = -1;
     state_kind = -1;  // the forward buffers were reused
     if (J) *J = Jv;
     if (sens) HIP_OK(hipMemcpy(sens, sens_dev.p, (size_t)N * 8, hipMemcpyDeviceToHost));
+  }
+
+  // ---- per-step reverse loop on a caller-driven forward state ----
+  // resetAdjoints + setCurrentTimestep (src/adjointLatticeBoltzmann/adjointLBSolver.cc:138-145)
+  void adjoint_reset(int timestep) {
+    if (!inc) throw Error{LBMX_ERR_SUP, "no adjoint for the standard (compressible) operators"};
+    if (d.design == LBMX_DESIGN_DARCY)
+      throw Error{LBMX_ERR_SUP, "DarcyMRT2d has no reverse operator (ReverseDarcyMRT2d is empty)"};
+    if (timestep < 0) throw Error{LBMX_ERR_ARG, "timestep must be >= 0"};
+    prepare();
+    lam_a.ensure(lam_bytes());
+    lam_b.ensure(lam_bytes());
+    sens_dev.ensure((size_t)N * 8);
+    zero_async(lam_a.p, lam_bytes(), stream);
+    zero_async(sens_dev.p, (size_t)N * 8, stream);
+    HIP_OK(hipStreamSynchronize(stream));
+    adj_t = timestep;
+    grad_valid = true;
+  }
+
+  // AdjointLBSolver::setCurrentTimestep (src/adjointLatticeBoltzmann/adjointLBSolver.cc:147-151)
+  void adjoint_set_timestep(int timestep) {
+    if (adj_t < 0) throw Error{LBMX_ERR_STATE, "adjoint not reset (call lbmx_adjoint_reset)"};
+    if (timestep < 0) throw Error{LBMX_ERR_ARG, "timestep must be >= 0"};
+    adj_t = timestep;
+  }
+
+  // One reverse step t = adj_t (adjointCollide, computeCollideSource, adjointStream:
+  // src/adjointLatticeBoltzmann/adjointLBSolver.cc:28-102, applications/RobustPump2d/robustPump2d.cc:268-276)
+  // on the current forward state s_t: recomputed from p_{t-1} after lbmx_forward, or read as
+  // the AoS state given to lbmx_set_state.
+  void adjoint_step(int with_source) {
+    if (adj_t < 0) throw Error{LBMX_ERR_STATE, "adjoint not reset (call lbmx_adjoint_reset)"};
+    if (adj_t < 1) throw Error{LBMX_ERR_STATE, "the reverse loop ends at step 1 (step 0 is the initial state)"};
+    require_state();
+    if (t != adj_t)
+      throw Error{LBMX_ERR_STATE, "forward state is at step " + std::to_string(t) + ", the reverse step is " +
+                                      std::to_string(adj_t)};
+    if (state_kind != 0 && state_kind != 2) throw Error{LBMX_ERR_STATE, "no forward state at this step"};
+    prepare();
+    ObjArgs oa = obj;
+    oa.acc = nullptr;
+    if (!with_source || !obj_set) oa.nterm = 0;
+    AdjArgs a = adj_args(state_kind == 0 ? prev : nullptr, lam_a.p, lam_b.p, adj_t, oa);
+    if (state_kind == 2) a.fwd_aos = static_cast<const double*>(raw_aos.p);
+    HIP_OK(run_adj(a));
+    std::swap(lam_a.p, lam_b.p);
+    HIP_OK(hipStreamSynchronize(stream));
+    stats.adj_sweeps += 1;
+    adj_t -= 1;
   }
 
   void get_sensitivity(double* s, size_t n) {
@@ -1383,6 +1435,41 @@ lbmx_status lbmx_gradient(lbmx_solver* s, int nsteps, double rho0, const double 
                           double* sens, size_t n) {
   CHECK_HANDLE(s);
   return guarded([&] { s->e->gradient(nsteps, rho0, u0, T0, J, sens, n); });
+}
+
+lbmx_status lbmx_adjoint_reset(lbmx_solver* s, int timestep) {
+  CHECK_HANDLE(s);
+  return guarded([&] { s->e->adjoint_reset(timestep); });
+}
+
+lbmx_status lbmx_adjoint_set_timestep(lbmx_solver* s, int timestep) {
+  CHECK_HANDLE(s);
+  return guarded([&] { s->e->adjoint_set_timestep(timestep); });
+}
+
+lbmx_status lbmx_adjoint_step(lbmx_solver* s, int with_source) {
+  CHECK_HANDLE(s);
+  return guarded([&] { s->e->adjoint_step(with_source); });
+}
+
+lbmx_status lbmx_set_design_device(lbmx_solver* s, const double* gamma_dev, size_t n) {
+  CHECK_HANDLE(s);
+  if (!gamma_dev) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] {
+    lbmx::Engine& e = *s->e;
+    if ((long long)n != e.N) throw lbmx::Error{LBMX_ERR_ARG, "design vector length != number of nodes"};
+    e.set_design_device(gamma_dev);
+    HIP_OK(hipStreamSynchronize(e.stream));
+  });
+}
+
+lbmx_status lbmx_get_sensitivity_device(lbmx_solver* s, const double** sens_dev, size_t* n) {
+  CHECK_HANDLE(s);
+  if (!sens_dev) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] {
+    *sens_dev = lbmx::solver_sensitivity_device(s);
+    if (n) *n = (size_t)s->e->N;
+  });
 }
 
 lbmx_status lbmx_get_sensitivity(lbmx_solver* s, double* sens, size_t n) {

@@ -92,8 +92,11 @@ struct FwdArgs {
 };
 
 // Adjoint sweep at reverse step t: reads p_{t-1} (fwd_src), nu_t (lam_in), writes nu_{t-1}.
+// With fwd_aos set, the pre-collision state s_t is read from that AoS FP64 array instead of
+// being recomputed from p_{t-1} (a caller-held forward state).
 struct AdjArgs {
   const void* fwd_src;
+  const double* fwd_aos;
   const void* lam_in;
   void* lam_out;
   const void* gamma;

@@ -339,8 +339,12 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   const int n = (z * ny + y) * nx + x;
   const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
   T f[L::Q];
-  pull_node<L, P>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
-  if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, tfac, x, y, z);
+  if (a.fwd_aos) {
+    static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(a.fwd_aos[(long long)n * L::Q + decltype(qc)::value]); });
+  } else {
+    pull_node<L, P>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
+    if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, tfac, x, y, z);
+  }
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
   // The adjoint field between sweeps is m_t, the boundary-corrected adjoint of s_t before the
   // transposed stream, stored at its own node. The transposed stream is gathered here:
@@ -443,6 +447,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   const BatchInst& I = batch_locate<TY>(st, i, x, y, z, tile);
   AdjArgs a;
   a.fwd_src = static_cast<const char*>(st.src) + I.state_off;
+  a.fwd_aos = nullptr;
   a.lam_in = static_cast<const char*>(st.lam_in) + I.lam_off;
   a.lam_out = static_cast<char*>(st.lam_out) + I.lam_off;
   a.gamma = I.gamma;

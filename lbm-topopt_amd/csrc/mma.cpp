@@ -148,14 +148,17 @@ struct Mma {
     }
   }
 
-  // Update (MMA.cc:485-524): GenSub, history shift, dual interior-point solve
-  void update(double* xh, const double* dfdxh, const double* gx, const double* const* dgdxh, double movlim) {
+  // Update (MMA.cc:485-524): GenSub, history shift, dual interior-point solve. The caller's
+  // arrays are host arrays, or device arrays when dev is set.
+  void update(double* xh, const double* dfdxh, const double* gx, const double* const* dgdxh, double movlim,
+              bool dev) {
     auto t0 = std::chrono::steady_clock::now();
     const size_t nb = (size_t)n * 8;
-    HIP_OK(hipMemcpyAsync(x.p, xh, nb, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(dfdx.p, dfdxh, nb, hipMemcpyHostToDevice, st));
+    const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIP_OK(hipMemcpyAsync(x.p, xh, nb, in, st));
+    HIP_OK(hipMemcpyAsync(dfdx.p, dfdxh, nb, in, st));
     for (int j = 0; j < m; ++j)
-      HIP_OK(hipMemcpyAsync(dgdx.as<double>() + (size_t)j * n, dgdxh[j], nb, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(dgdx.as<double>() + (size_t)j * n, dgdxh[j], nb, in, st));
     k += 1;
     MmaArgs ar = args(movlim);
     HIP_OK(launch_mma_gensub(ar, part.as<double>(), out.as<double>(), st));
@@ -167,7 +170,7 @@ struct Mma {
     HIP_OK(hipMemcpyAsync(xo1.p, x.p, nb, hipMemcpyDeviceToDevice, st));
     ar = args(movlim);
     solve_dual(ar, sums);
-    HIP_OK(hipMemcpyAsync(xh, x.p, nb, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(xh, x.p, nb, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -219,10 +222,12 @@ struct Mma {
     }
   }
 
-  double design_change(const double* xnew_host) {
+  double design_change(const double* xnew, bool dev) {
     const double* xn = x.as<double>();
-    if (xnew_host) {
-      HIP_OK(hipMemcpyAsync(tmp.p, xnew_host, (size_t)n * 8, hipMemcpyHostToDevice, st));
+    if (xnew && dev) {
+      xn = xnew;
+    } else if (xnew) {
+      HIP_OK(hipMemcpyAsync(tmp.p, xnew, (size_t)n * 8, hipMemcpyHostToDevice, st));
       xn = tmp.as<double>();
     }
     HIP_OK(launch_mma_change(xn, xo1.as<double>(), n, part.as<double>(), out.as<double>(), st));
@@ -298,13 +303,28 @@ lbmx_status lbmx_mma_update(lbmx_mma* h, double* x, const double* dfdx, const do
   if (!x || !dfdx || !gx || !dgdx) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   for (int j = 0; j < h->o->m; ++j)
     if (!dgdx[j]) { lbmx::g_last_error = "null constraint sensitivity"; return LBMX_ERR_ARG; }
-  return guarded([&] { h->o->update(x, dfdx, gx, dgdx, movlim); });
+  return guarded([&] { h->o->update(x, dfdx, gx, dgdx, movlim, false); });
+}
+
+lbmx_status lbmx_mma_update_device(lbmx_mma* h, double* x, const double* dfdx, const double* gx,
+                                   const double* const* dgdx, double movlim) {
+  CHECK_MMA(h);
+  if (!x || !dfdx || !gx || !dgdx) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  for (int j = 0; j < h->o->m; ++j)
+    if (!dgdx[j]) { lbmx::g_last_error = "null constraint sensitivity"; return LBMX_ERR_ARG; }
+  return guarded([&] { h->o->update(x, dfdx, gx, dgdx, movlim, true); });
 }
 
 lbmx_status lbmx_mma_design_change(lbmx_mma* h, const double* xnew, double* ch) {
   CHECK_MMA(h);
   if (!ch) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
-  return guarded([&] { *ch = h->o->design_change(xnew); });
+  return guarded([&] { *ch = h->o->design_change(xnew, false); });
+}
+
+lbmx_status lbmx_mma_design_change_device(lbmx_mma* h, const double* xnew, double* ch) {
+  CHECK_MMA(h);
+  if (!ch) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { *ch = h->o->design_change(xnew, true); });
 }
 
 lbmx_status lbmx_mma_get_state(lbmx_mma* h, double* L, double* U, double* lam, int* newton_iters, double* ms) {

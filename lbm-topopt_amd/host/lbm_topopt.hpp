@@ -441,9 +441,100 @@ class AdjointLBSolver {
     return lbmx_get_adjoint(solver_->handle(), adj->data(), adj->size());
   }
 
+  // The reverse loop step by step (applications/RobustPump2d/robustPump2d.cc:263-277), for a
+  // caller that restores the forward states itself (a CheckpointingBase below).
+  // resetAdjoints (adjointLBSolver.cc:138-145) and setCurrentTimestep (:147-151)
+  PetscErrorCode resetAdjoints() {
+    if (!solver_) return LBMX_ERR_STATE;
+    source_ = false;
+    return lbmx_adjoint_reset(solver_->handle(), (int)t_);
+  }
+  PetscErrorCode setCurrentTimestep(PetscInt t) {
+    if (!solver_) return LBMX_ERR_STATE;
+    t_ = t;
+    return lbmx_adjoint_set_timestep(solver_->handle(), (int)t);
+  }
+  // adjointCollide(f, material) (adjointLBSolver.cc:28-48): f is the forward state of the
+  // current step (getForwardSolution); the material is the solver's design field. The engine
+  // fuses the reverse collision, the source and the transposed stream into one sweep, which
+  // runs at adjointStream.
+  PetscErrorCode adjointCollide(const std::vector<double>& f) {
+    if (!solver_) return LBMX_ERR_STATE;
+    source_ = false;
+    return lbmx_set_state(solver_->handle(), f.data(), f.size(), (int)t_);
+  }
+  // computeCollideSource(objective, timeConst, f, material) (adjointLBSolver.cc:83-102)
+  PetscErrorCode computeCollideSource(const LBMacroFunctional& obj, PetscScalar timeConst) {
+    if (!solver_) return LBMX_ERR_STATE;
+    const lbmx_objective o = obj.objective(timeConst);
+    PetscErrorCode e = lbmx_set_objective(solver_->handle(), &o);
+    if (!e) source_ = true;
+    return e;
+  }
+  // adjointStream(f) (adjointLBSolver.cc:50-81), then the step counter moves to t - 1
+  PetscErrorCode adjointStream() {
+    if (!solver_) return LBMX_ERR_STATE;
+    PetscErrorCode e = lbmx_adjoint_step(solver_->handle(), source_ ? 1 : 0);
+    source_ = false;
+    if (!e) --t_;
+    return e;
+  }
+  PetscInt currentTimestep() const { return t_; }
+
  private:
   friend class LBSolver;
   LBSolver* solver_ = nullptr;
+  PetscInt t_ = 0;
+  bool source_ = false;
+};
+
+// ---- checkpointing plug-in interface (src/topOpt/checkpointingBase.hh:7-18) on host AoS
+//      state vectors (getDistributions / setDistributions) ----
+class CheckpointingBase {
+ public:
+  virtual ~CheckpointingBase() = default;
+  virtual PetscErrorCode makeCheckpoint(PetscInt step, const std::vector<double>& f, void* userContext) = 0;
+  virtual PetscErrorCode getForwardSolution(PetscInt step, std::vector<double>& f, void* userContext) = 0;
+  virtual PetscErrorCode reset(void* userContext) = 0;
+
+ protected:
+  virtual PetscErrorCode advance(void* userContext) = 0;
+  virtual PetscErrorCode setSolverState(PetscInt step, const std::vector<double>& f, void* userContext) = 0;
+  virtual PetscErrorCode copyFromSolverState(std::vector<double>& f, void* userContext) = 0;
+};
+
+// Every state kept in host memory (the reference's ForwardSolutionStorage pattern): the
+// simplest CheckpointingBase; the device engine's own Wang checkpointing is
+// UnsteadyLBTopOpt::allocateMemoryForForwardSolution.
+class LBNaiveCheckpointing : public CheckpointingBase {
+ public:
+  explicit LBNaiveCheckpointing(LBSolver& s) : solver_(s) {}
+  PetscErrorCode makeCheckpoint(PetscInt step, const std::vector<double>& f, void*) override {
+    if (step < 0) return LBMX_ERR_ARG;
+    if ((size_t)step >= states_.size()) states_.resize((size_t)step + 1);
+    states_[(size_t)step] = f;
+    return 0;
+  }
+  PetscErrorCode getForwardSolution(PetscInt step, std::vector<double>& f, void*) override {
+    if (step < 0 || (size_t)step >= states_.size() || states_[(size_t)step].empty()) return LBMX_ERR_STATE;
+    f = states_[(size_t)step];
+    return 0;
+  }
+  PetscErrorCode reset(void*) override {
+    states_.clear();
+    return 0;
+  }
+
+ protected:
+  PetscErrorCode advance(void*) override { return solver_.collideAndStream(); }
+  PetscErrorCode setSolverState(PetscInt step, const std::vector<double>& f, void*) override {
+    return solver_.setDistributions(f, step);
+  }
+  PetscErrorCode copyFromSolverState(std::vector<double>& f, void*) override { return solver_.getDistributions(&f); }
+  LBSolver& solver_;
+
+ private:
+  std::vector<std::vector<double>> states_;
 };
 
 inline PetscErrorCode LBSolver::makeSourceAdjoint(AdjointLBSolver* adj) {

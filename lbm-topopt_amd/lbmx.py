@@ -121,6 +121,15 @@ def library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"lbmx engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+    # One HIP runtime per process. The torch wheel ships its own HIP runtime and loads it by
+    # path; loaded after the engine, it would be a second runtime beside the system one, fail
+    # to open the GPU, and device pointers could not pass between them. Importing torch first
+    # makes the engine bind to torch's runtime (same soname), so torch tensors and the engine
+    # share device memory.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     P = C.POINTER
     dp = P(C.c_double)
@@ -143,6 +152,11 @@ def library():
         "lbmx_gradient": [C.c_void_p, C.c_int, C.c_double, dp, C.c_double, dp, dp, C.c_size_t],
         "lbmx_get_sensitivity": [C.c_void_p, dp, C.c_size_t],
         "lbmx_get_adjoint": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_adjoint_reset": [C.c_void_p, C.c_int],
+        "lbmx_adjoint_step": [C.c_void_p, C.c_int],
+        "lbmx_adjoint_set_timestep": [C.c_void_p, C.c_int],
+        "lbmx_set_design_device": [C.c_void_p, dp, C.c_size_t],
+        "lbmx_get_sensitivity_device": [C.c_void_p, P(dp), P(C.c_size_t)],
         "lbmx_get_stats": [C.c_void_p, P(Stats)],
         "lbmx_device_bytes": [C.c_void_p, P(C.c_size_t)],
         "lbmx_time_sweeps": [C.c_void_p, C.c_int, C.c_int, dp],
@@ -170,6 +184,10 @@ def library():
         "lbmx_filter_design_to_solver": [C.c_void_p, dp, C.c_size_t, C.c_void_p],
         "lbmx_filter_sensitivities": [C.c_void_p, dp, C.c_size_t, dp, C.c_size_t],
         "lbmx_filter_solver_sensitivities": [C.c_void_p, C.c_void_p, dp, C.c_size_t],
+        "lbmx_filter_design_device": [C.c_void_p, dp, C.c_size_t, dp, C.c_size_t],
+        "lbmx_filter_design_to_solver_device": [C.c_void_p, dp, C.c_size_t, C.c_void_p],
+        "lbmx_filter_sensitivities_device": [C.c_void_p, dp, C.c_size_t, dp, C.c_size_t],
+        "lbmx_filter_solver_sensitivities_device": [C.c_void_p, C.c_void_p, dp, C.c_size_t],
         "lbmx_filter_get_field": [C.c_void_p, C.c_int, dp, C.c_size_t],
         "lbmx_filter_timing": [C.c_void_p, dp],
         "lbmx_vec_write_binary": [C.c_char_p, dp, C.c_size_t],
@@ -182,6 +200,8 @@ def library():
         "lbmx_mma_set_options": [C.c_void_p, C.c_int, C.c_int],
         "lbmx_mma_update": [C.c_void_p, dp, dp, dp, P(dp), C.c_double],
         "lbmx_mma_design_change": [C.c_void_p, dp, dp],
+        "lbmx_mma_update_device": [C.c_void_p, dp, dp, dp, P(dp), C.c_double],
+        "lbmx_mma_design_change_device": [C.c_void_p, dp, dp],
         "lbmx_mma_get_state": [C.c_void_p, dp, dp, dp, P(C.c_int), dp],
     }
     for name, args in sig.items():
@@ -208,7 +228,10 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_filter_get_field", "lbmx_filter_timing", "lbmx_vec_write_binary", "lbmx_vec_load_binary",
             "lbmx_write_vts", "lbmx_mma_create", "lbmx_mma_destroy",
             "lbmx_mma_set_robust_constants", "lbmx_mma_set_asymptotes", "lbmx_mma_set_options", "lbmx_mma_update",
-            "lbmx_mma_design_change", "lbmx_mma_get_state"]
+            "lbmx_mma_design_change", "lbmx_mma_get_state", "lbmx_adjoint_reset", "lbmx_adjoint_step", "lbmx_adjoint_set_timestep",
+            "lbmx_set_design_device", "lbmx_get_sensitivity_device", "lbmx_filter_design_device",
+            "lbmx_filter_design_to_solver_device", "lbmx_filter_sensitivities_device",
+            "lbmx_filter_solver_sensitivities_device", "lbmx_mma_update_device", "lbmx_mma_design_change_device"]
 
 
 def checkpoint_simulate(nslots, nsteps):
@@ -267,6 +290,26 @@ def _check(code):
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _tptr(t, n=None):
+    """Device pointer of a contiguous float64 GPU tensor (the *_device entry points). Work
+    queued on torch's current stream is finished first: the engine runs on its own streams."""
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise TypeError("expected a contiguous float64 tensor on the GPU")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"tensor holds {t.numel()} values, expected {n}")
+    torch.cuda.current_stream(t.device).synchronize()
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double))
+
+
+class _DeviceView:
+    """A borrowed device array, exposed to torch through __cuda_array_interface__ (torch takes
+    no read-only views: writing through it changes the engine's own array)."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f8", "data": (ptr, False), "version": 2}
 
 
 class Solver:
@@ -365,6 +408,31 @@ class Solver:
         a = np.empty(self.nodes * self.Q)
         _check(self.lib.lbmx_get_adjoint(self.h, _dptr(a), a.size))
         return a
+
+    def adjoint_reset(self, timestep):
+        """resetAdjoints + setCurrentTimestep: the next adjoint_step reverses step `timestep`."""
+        _check(self.lib.lbmx_adjoint_reset(self.h, int(timestep)))
+
+    def adjoint_set_timestep(self, timestep):
+        _check(self.lib.lbmx_adjoint_set_timestep(self.h, int(timestep)))
+
+    def adjoint_step(self, with_source=True):
+        """One reverse step (adjointCollide, computeCollideSource, adjointStream) on the current
+        forward state, which must be at the adjoint's step."""
+        _check(self.lib.lbmx_adjoint_step(self.h, int(bool(with_source))))
+
+    def set_design_device(self, gamma):
+        """setFieldFromDesignVec from a float64 GPU tensor (natural node order)."""
+        _check(self.lib.lbmx_set_design_device(self.h, _tptr(gamma, self.nodes), self.nodes))
+
+    def sensitivity_device(self):
+        """dJ/dgamma as a torch view of the solver's own device array (borrowed: valid until the
+        next gradient, adjoint reset or close)."""
+        import torch
+        p = C.POINTER(C.c_double)()
+        n = C.c_size_t(0)
+        _check(self.lib.lbmx_get_sensitivity_device(self.h, C.byref(p), C.byref(n)))
+        return torch.as_tensor(_DeviceView(C.cast(p, C.c_void_p).value, n.value), device="cuda")
 
     def stats(self):
         st = Stats()
@@ -538,6 +606,35 @@ class Filter:
         _check(self.lib.lbmx_filter_solver_sensitivities(self.h, solver.h, _dptr(out), out.size))
         return out
 
+    def design_device(self, design, out=None):
+        """design() on float64 GPU tensors; returns the physical field (out, or a new tensor)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.n_full, dtype=torch.float64, device=design.device)
+        _check(self.lib.lbmx_filter_design_device(self.h, _tptr(design, self.n_design), self.n_design,
+                                                  _tptr(out, self.n_full), self.n_full))
+        return out
+
+    def design_to_solver_device(self, design, solver):
+        _check(self.lib.lbmx_filter_design_to_solver_device(self.h, _tptr(design, self.n_design), self.n_design,
+                                                            solver.h))
+
+    def sensitivities_device(self, full_sens, out=None):
+        import torch
+        if out is None:
+            out = torch.empty(self.n_design, dtype=torch.float64, device=full_sens.device)
+        _check(self.lib.lbmx_filter_sensitivities_device(self.h, _tptr(full_sens, self.n_full), self.n_full,
+                                                         _tptr(out, self.n_design), self.n_design))
+        return out
+
+    def solver_sensitivities_device(self, solver, out=None):
+        import torch
+        if out is None:
+            out = torch.empty(self.n_design, dtype=torch.float64, device="cuda")
+        _check(self.lib.lbmx_filter_solver_sensitivities_device(self.h, solver.h, _tptr(out, self.n_design),
+                                                                self.n_design))
+        return out
+
     def field(self, which):
         out = np.empty(self.n_full)
         _check(self.lib.lbmx_filter_get_field(self.h, which, _dptr(out), out.size))
@@ -598,6 +695,23 @@ class MMA:
         else:
             a = np.ascontiguousarray(xnew, dtype=np.float64)
             _check(self.lib.lbmx_mma_design_change(self.h, _dptr(a), C.byref(ch)))
+        return ch.value
+
+    def update_device(self, x, dfdx, gx, dgdx, movlim):
+        """update() on float64 GPU tensors: x (updated in place), dfdx and the m rows dgdx; gx
+        is a host sequence of m constraint values."""
+        gx = np.ascontiguousarray(gx, dtype=np.float64)
+        if len(dgdx) != self.m:
+            raise ValueError(f"expected {self.m} constraint sensitivity rows")
+        ptrs = (C.POINTER(C.c_double) * self.m)(*[_tptr(r, self.n) for r in dgdx])
+        _check(self.lib.lbmx_mma_update_device(self.h, _tptr(x, self.n), _tptr(dfdx, self.n), _dptr(gx), ptrs,
+                                               movlim))
+        return x
+
+    def design_change_device(self, xnew=None):
+        ch = C.c_double(0.0)
+        _check(self.lib.lbmx_mma_design_change_device(self.h, None if xnew is None else _tptr(xnew, self.n),
+                                                      C.byref(ch)))
         return ch.value
 
     def state(self):

@@ -2,7 +2,9 @@
 
 Forward runs lbmx_gradient (forward sweeps with checkpoints, reverse sweeps with
 recomputation) and keeps dJ/dgamma; backward scales it by the incoming gradient. PyTorch is
-only the tensor front end here: the computation is the HIP engine behind the C ABI.
+only the tensor front end here: the computation is the HIP engine behind the C ABI. A design
+tensor on the GPU stays there: it is handed to the engine by device pointer and dJ/dgamma is
+copied device to device from the engine's own array.
 """
 import numpy as np
 import torch
@@ -16,10 +18,16 @@ class DesignObjective(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, gamma, solver, nsteps, rho0=1.0):
-        g = gamma.detach().to("cpu", torch.float64).contiguous().numpy()
-        solver.set_design(g)
-        J, sens = solver.gradient(int(nsteps), rho0=rho0)
-        ctx.save_for_backward(torch.from_numpy(sens).to(gamma.device))
+        if gamma.is_cuda:
+            solver.set_design_device(gamma.detach().to(torch.float64).contiguous())
+            J, _ = solver.gradient(int(nsteps), rho0=rho0, want_sens=False)
+            sens = solver.sensitivity_device().clone().to(gamma.dtype)
+        else:
+            g = gamma.detach().to(torch.float64).contiguous().numpy()
+            solver.set_design(g)
+            J, sens = solver.gradient(int(nsteps), rho0=rho0)
+            sens = torch.from_numpy(sens).to(gamma.dtype)
+        ctx.save_for_backward(sens)
         return gamma.new_tensor(J, dtype=torch.float64)
 
     @staticmethod

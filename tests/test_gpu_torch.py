@@ -30,3 +30,23 @@ def test_autograd_matches_central_differences(lbmx_mod, coll, design):
     scale = np.max(np.abs(grad))
     assert np.max(np.abs(fd - grad[cells])) <= 1e-6 * scale, (fd, grad[cells])
     s.close()
+
+
+def test_gpu_design_tensor_stays_on_device(lbmx_mod):
+    """A design tensor on the GPU: same J and bitwise the same gradient as the host path."""
+    nx, ny, steps = 40, 18, 40
+    d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN, omega=1.25,
+                       interp_q=0.1, alpha_max=0.1)
+    s = lbmx.Solver(d)
+    workloads.channel2d(s, nx, ny, time_const=1.0 / steps)
+    g0 = workloads.design_field(nx, ny, 1, seed=2)
+    gh = torch.tensor(g0, dtype=torch.float64, requires_grad=True)
+    Jh = lbmx_torch.objective(gh, s, steps)
+    Jh.backward()
+    gd = torch.tensor(g0, dtype=torch.float64, device="cuda", requires_grad=True)
+    Jd = lbmx_torch.objective(gd, s, steps)
+    Jd.backward()
+    assert gd.grad.is_cuda
+    assert Jd.item() == Jh.item()
+    assert np.array_equal(gd.grad.cpu().numpy(), gh.grad.numpy())
+    s.close()
