@@ -97,15 +97,21 @@ typedef enum {
                                      (src/boundaryConditions/ZouHe2d.hh:23-195)                  */
   LBMX_BC_ZOUHE_PRESSURE = 2,     /* {Incompressible,Standard}ZouHePressure{East,West}2d
                                      (src/boundaryConditions/ZouHe2d.hh:240-390)                 */
-  LBMX_BC_ZOUHE_NEUMANN = 3,      /* IncompressibleZouHeNeumann{East,West}2d (forward only)      */
+  LBMX_BC_ZOUHE_NEUMANN = 3,      /* IncompressibleZouHeNeumann{East,West}2d<D2Q9>
+                                     (src/boundaryConditions/ZouHe2d.hh:440-527; adjoint
+                                     adjointZouHe2d.cc:114-172): u_x extrapolated from the two
+                                     nodes inwards, u_y = 0; east/west, grid nx >= 4             */
   LBMX_BC_HECHT_VELOCITY = 4,     /* {Incompressible,Standard}HechtVelocityWest3d
                                      (src/boundaryConditions/Hecht3d.hh:26-110)                  */
   LBMX_BC_HECHT_PRESSURE = 5,     /* {Incompressible,Standard}HechtPressureEast3d
                                      (src/boundaryConditions/Hecht3d.hh:120-238)                 */
   LBMX_BC_THERMAL_BOUNCEBACK = 6, /* ThermalBounceback2d<D2Q9,D2Q5> (adiabatic),
                                      (src/boundaryConditions/thermalBounceback2d.hh:260-300)     */
-  LBMX_BC_THERMAL_DIRICHLET = 7   /* ThermalZouHeWest2d<D2Q9,D2Q5> (thermalZouHe2d.hh:62-93); the
+  LBMX_BC_THERMAL_DIRICHLET = 7,  /* ThermalZouHeWest2d<D2Q9,D2Q5> (thermalZouHe2d.hh:62-93); the
                                      N/S/E orientations are the same construction, defined here  */
+  LBMX_BC_THERMAL_NEUMANN = 8     /* ThermalZouHeNeumann{East,West}2d<D2Q9,D2Q5>
+                                     (thermalZouHe2d.hh:229-300; adjoint adjointThermalZouHe2d.hh:
+                                     161-227): T extrapolated from the two nodes inwards          */
 } lbmx_bc_type;
 
 typedef enum {

@@ -134,6 +134,8 @@ struct Engine {
   // device
   DeviceBuffer gamma64_dev;  // FP64 copy of a design set on the device, for the FP32-design forms
   DeviceBuffer gamma_dev, bc_dev, term_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
+  DeviceBuffer nside_dev;  // Neumann pieces: adjoint transfer coefficients of the last adjoint sweep
+  int nside_count = 0;     // doubles in nside_dev
   std::vector<std::unique_ptr<DeviceBuffer>> pool;  // state buffers: [0],[1] working; rest slots
   bool bc_dirty = true;
   void* cur = nullptr;   // p_t
@@ -272,7 +274,9 @@ struct Engine {
         if (D != 2 || nn != 1) return false;
         return inc ? (n[0] == -1 || n[1] == 1 || n[0] == 1) : n[0] == -1;
       case LBMX_BC_ZOUHE_PRESSURE: return D == 2 && nn == 1 && n[1] == 0;
-      case LBMX_BC_ZOUHE_NEUMANN: return false;  // needs neighbour data: not on the device path
+      // the two nodes inwards must be interior nodes
+      case LBMX_BC_ZOUHE_NEUMANN: return D == 2 && nn == 1 && n[1] == 0 && inc && d.nx >= 4;
+      case LBMX_BC_THERMAL_NEUMANN: return d.lattice == LBMX_D2Q9_D2Q5 && nn == 1 && n[1] == 0 && d.nx >= 4;
       case LBMX_BC_HECHT_VELOCITY: return D == 3 && nn == 1 && n[0] == -1;
       case LBMX_BC_HECHT_PRESSURE: return D == 3 && nn == 1 && n[0] == 1;
       case LBMX_BC_THERMAL_BOUNCEBACK: return d.lattice == LBMX_D2Q9_D2Q5;
@@ -291,7 +295,7 @@ struct Engine {
   }
 
   void add_boundary(const lbmx_box& box, const lbmx_bc_desc& bc) {
-    if (bc.type < LBMX_BC_BOUNCEBACK || bc.type > LBMX_BC_THERMAL_DIRICHLET)
+    if (bc.type < LBMX_BC_BOUNCEBACK || bc.type > LBMX_BC_THERMAL_NEUMANN)
       throw Error{LBMX_ERR_ARG, "unknown boundary type"};
     int dom[6] = {0, d.nx - 1, 0, d.ny - 1, 0, d.nz - 1};
     int ub[6] = {box.x0, box.x1, box.y0, box.y1, box.z0, box.z1};
@@ -331,6 +335,10 @@ struct Engine {
       e.center[0] = bc.center[0]; e.center[1] = bc.center[1];
       e.radius[0] = bc.radius[0]; e.radius[1] = bc.radius[1];
       e.t_period = bc.t_period; e.t_amp = bc.t_amp; e.t_offset = bc.t_offset; e.t_phase = bc.t_phase;
+      if (is_neumann(e.type)) {  // two adjoint transfer coefficients per node of the piece
+        e.aux = nside_count;
+        nside_count += 2 * (o[3] - o[2] + 1);
+      }
       bcs.push_back(e);
     }
     bc_dirty = true;
@@ -439,6 +447,7 @@ struct Engine {
       terms_dirty = false;
     }
     if (bc_dirty) {
+      nside_dev.ensure(std::max<size_t>(1, (size_t)nside_count) * sizeof(double));
       bc_dev.ensure(std::max<size_t>(1, bcs.size()) * sizeof(DevBC));
       if (!bcs.empty())
         HIP_OK(hipMemcpyAsync(bc_dev.p, bcs.data(), bcs.size() * sizeof(DevBC), hipMemcpyHostToDevice, stream));
@@ -509,7 +518,30 @@ struct Engine {
     }
 #undef CALL
   }
+  static bool is_neumann(int type) { return type == LBMX_BC_ZOUHE_NEUMANN || type == LBMX_BC_THERMAL_NEUMANN; }
+
+  // an adjoint sweep, then the Neumann pieces' transfers into their inward neighbours
   hipError_t run_adj(const AdjArgs& a) {
+    hipError_t e = run_adj_sweep(a);
+    if (e != hipSuccess || nside_count == 0) return e;
+    for (const DevBC& b : bcs) {
+      if (!is_neumann(b.type)) continue;
+      e = run_neumann_fix(b, a.lam_out);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  hipError_t run_neumann_fix(const DevBC& b, void* lam) {
+    const double* ns = static_cast<const double*>(nside_dev.p);
+#define CALL(S) launch_neumann_fix_##S(b, lam, ll, ns, stream)
+    switch (d.lattice) {
+      case LBMX_D2Q9: LBMX_PREC_SWITCH(d2q9, CALL)
+      case LBMX_D3Q19: LBMX_PREC_SWITCH(d3q19, CALL)
+      default: LBMX_PREC_SWITCH(thermal, CALL)
+    }
+#undef CALL
+  }
+  hipError_t run_adj_sweep(const AdjArgs& a) {
 #define CALL(S) launch_adj_##S(d.collision, d.design, a, stream)
     switch (d.lattice) {
       case LBMX_D2Q9: LBMX_PREC_SWITCH(d2q9, CALL)
@@ -527,6 +559,7 @@ struct Engine {
     a.lam_out = lout;
     a.gamma = gamma_dev.p;
     a.sens = static_cast<double*>(sens_dev.p);
+    a.nside = static_cast<double*>(nside_dev.p);
     a.bcs = static_cast<const DevBC*>(bc_dev.p);
     a.nbc = (int)bcs.size();
     a.t = tstep;
@@ -893,7 +926,7 @@ struct Engine {
 
   size_t device_bytes() const {
     size_t b = gamma64_dev.bytes + gamma_dev.bytes + bc_dev.bytes + term_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
-               lam_b.bytes + raw_aos.bytes;
+               lam_b.bytes + raw_aos.bytes + nside_dev.bytes;
     for (auto& p : pool) b += p->bytes;
     return b;
   }
@@ -931,6 +964,7 @@ struct Batch {
     for (size_t i = 0; i < m.size(); ++i) {
       const lbmx_desc& d = m[i]->d;
       if (d.lattice != LBMX_D2Q9) throw Error{LBMX_ERR_SUP, "batched sweeps are built for D2Q9"};
+      if (m[i]->nside_count > 0) throw Error{LBMX_ERR_SUP, "batched sweeps do not take Zou-He Neumann pieces"};
       if (d.collision != d0.collision || d.design != d0.design || d.precision != d0.precision)
         throw Error{LBMX_ERR_ARG, "batch members must share operator, design coupling and precision"};
       for (size_t k = 0; k < i; ++k)

@@ -19,7 +19,7 @@ struct DevBC {
   int profile;
   int time_mode;
   int inc;     // 1: incompressible variant of the condition
-  int pad;
+  int aux;     // Neumann pieces: first slot of the piece in the adjoint side buffer (2 per node)
   double value;
   double dir[3];
   double center[2];
@@ -101,6 +101,7 @@ struct AdjArgs {
   void* lam_out;
   const void* gamma;
   double* sens;
+  double* nside;      // Neumann pieces: the two coefficients each node sends to its inward neighbours
   const DevBC* bcs;
   int nbc;
   int t;

@@ -641,6 +641,7 @@ __device__ __forceinline__ void bc_node_adj(T (&lio)[L::Q], const DevBC& b) {
   switch (b.type) {
     case LBMX_BC_BOUNCEBACK: bc_bounceback_adj<L>(l, b.nrm); break;
     case LBMX_BC_ZOUHE_VELOCITY:
+    case LBMX_BC_ZOUHE_NEUMANN:  // own block (the neighbours' part is sent separately)
       if constexpr (L::D == 2) zh_velocity_adj<T>(l, b.nrm);
       break;
     case LBMX_BC_ZOUHE_PRESSURE:
@@ -663,6 +664,7 @@ __device__ __forceinline__ void bc_node_adj(T (&lio)[L::Q], const DevBC& b) {
       }
       break;
     case LBMX_BC_THERMAL_DIRICHLET:
+    case LBMX_BC_THERMAL_NEUMANN:  // own block (the neighbours' part is sent separately)
       if constexpr (L::QT == 5) {
         constexpr int o = L::QF;
         int m = d2q5_incoming_mask(b.nrm);

@@ -173,21 +173,135 @@ __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned 
   }
 }
 
-template <class L, class T>
-__device__ __forceinline__ void apply_bcs(T (&f)[L::Q], const DevBC* __restrict__ bcs, int nbc,
-                                          const double* tfac, int x, int y, int z) {
+// Zou-He Neumann conditions (2D, east/west): the node's unknowns follow from the moment of
+// the two nodes inwards extrapolated to the boundary, (4 m_1 - m_2) / 3, with m = u_x for the
+// flow (IncompressibleZouHeNeumann{East,West}2d<D2Q9>, src/boundaryConditions/ZouHe2d.hh:440-527;
+// the unknowns are those of the Zou-He velocity condition with u_y = 0) and m = T for the
+// D2Q5 temperature (ThermalZouHeNeumann{East,West}2d<.,D2Q5>, thermalZouHe2d.hh:229-300). The
+// grid has nx >= 4, so both neighbours are interior nodes and their post-stream values are
+// plain pulls of the source field.
+template <class L, class P>
+__device__ __forceinline__ void neumann_node(typename P::T (&fio)[L::Q], const DevBC& e, const void* src,
+                                             const Layout& ls, int x, int y, int z) {
+  using T = typename P::T;
+  using S = typename P::S;
+  const int dir = -e.nrm[0];
+  const unsigned int o1 = layout_off(ls, x + dir, y, z) * (unsigned int)sizeof(S);
+  const unsigned int o2 = layout_off(ls, x + 2 * dir, y, z) * (unsigned int)sizeof(S);
+  auto g = [&](auto qc, unsigned int ob) -> T {
+    constexpr int q = decltype(qc)::value;
+    return P::template dec<L, q>(ld<S>(pull_base<L, q, S>(src, ls), ob));
+  };
+  T f[L::Q];
+  static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = fio[decltype(qc)::value]; });
+  if (e.type == LBMX_BC_ZOUHE_NEUMANN) {
+    const T u1 = g(IC<1>{}, o1) + g(IC<7>{}, o1) + g(IC<8>{}, o1) - (g(IC<3>{}, o1) + g(IC<4>{}, o1) + g(IC<5>{}, o1));
+    const T u2 = g(IC<1>{}, o2) + g(IC<7>{}, o2) + g(IC<8>{}, o2) - (g(IC<3>{}, o2) + g(IC<4>{}, o2) + g(IC<5>{}, o2));
+    zh_velocity<T>(f, e.nrm, true, (T(4) * u1 - u2) / T(3), T(0));
+  } else if constexpr (L::QT == 5) {
+    constexpr int o = L::QF;
+    const T T1 = g(IC<o>{}, o1) + g(IC<o + 1>{}, o1) + g(IC<o + 2>{}, o1) + g(IC<o + 3>{}, o1) + g(IC<o + 4>{}, o1);
+    const T T2 = g(IC<o>{}, o2) + g(IC<o + 1>{}, o2) + g(IC<o + 2>{}, o2) + g(IC<o + 3>{}, o2) + g(IC<o + 4>{}, o2);
+    const T Tv = (T(4) * T1 - T2) / T(3);
+    const bool east = e.nrm[0] == 1;
+    const T g2 = Tv - f[o] - f[o + 1] - f[o + 3] - f[o + 4];
+    const T g4 = Tv - f[o] - f[o + 1] - f[o + 2] - f[o + 3];
+    f[o + 2] = bsel(east, g2, f[o + 2]);
+    f[o + 4] = bsel(east, f[o + 4], g4);
+  }
+  static_for<0, L::Q>([&](auto qc) { fio[decltype(qc)::value] = f[decltype(qc)::value]; });
+}
+
+template <class L>
+__device__ __forceinline__ bool is_neumann(const DevBC& e) {
+  if constexpr (L::D == 2) return e.type == LBMX_BC_ZOUHE_NEUMANN || e.type == LBMX_BC_THERMAL_NEUMANN;
+  else return false;
+}
+
+// Boundary pieces in insertion order at node (x,y,z) of the post-stream state; src/ls: the
+// field the state was pulled from (the Neumann conditions read two inward neighbours).
+template <class L, class P>
+__device__ __forceinline__ void apply_bcs(typename P::T (&f)[L::Q], const DevBC* __restrict__ bcs, int nbc,
+                                          const double* tfac, int x, int y, int z, const void* src, const Layout& ls) {
+  using T = typename P::T;
   for (int b = 0; b < nbc; ++b) {
     const DevBC& e = bcs[b];
-    if (in_box(e.box, x, y, z)) bc_node<L, T>(f, e, T(tfac[b]), x, y, z);
+    if (!in_box(e.box, x, y, z)) continue;
+    if (is_neumann<L>(e)) {
+      if constexpr (L::D == 2) neumann_node<L, P>(f, e, src, ls, x, y, z);
+    } else {
+      bc_node<L, T>(f, e, T(tfac[b]), x, y, z);
+    }
   }
 }
 
+// Transposes in reverse order. A Neumann piece also feeds its inward neighbours: the node
+// writes the two coefficients of that transfer to nside (slot e.aux + 2 (y - y0)), which
+// k_neumann_fix adds into the neighbours' stored adjoints after the sweep
+// (AdjointIncompressibleZouHeNeumann{East,West}2d, src/adjointBoundaryConditions/adjointZouHe2d.cc:114-172;
+// AdjointThermalZouHeNeumann{East,West}2d, adjointThermalZouHe2d.hh:161-227).
 template <class L, class T>
 __device__ __forceinline__ void apply_bcs_adj(T (&l)[L::Q], const DevBC* __restrict__ bcs, int nbc,
-                                              int x, int y, int z) {
+                                              int x, int y, int z, double* nside) {
   for (int b = nbc - 1; b >= 0; --b) {
     const DevBC& e = bcs[b];
-    if (in_box(e.box, x, y, z)) bc_node_adj<L, T>(l, e);
+    if (!in_box(e.box, x, y, z)) continue;
+    if constexpr (L::D == 2) {
+      if (is_neumann<L>(e)) {
+        const bool east = e.nrm[0] == 1;
+        T c1, c2;
+        if (e.type == LBMX_BC_ZOUHE_NEUMANN) {
+          // u_x of neighbour k enters the unknowns with weights -(1/6, 2/3, 1/6) (east) times
+          // d u_x / d u_k = 4/3, -1/3; neighbour k's u_x is sum_q c_qx f_q
+          const T a = east ? l[3] + l[5] : l[1] + l[7];
+          const T m = east ? l[4] : l[8];
+          const T s1 = T(2.0 / 9.0) * a + T(8.0 / 9.0) * m;
+          const T s2 = T(1.0 / 18.0) * a + T(2.0 / 9.0) * m;
+          c1 = east ? -s1 : s1;
+          c2 = east ? s2 : -s2;
+        } else {
+          if constexpr (L::QT == 5) {
+            constexpr int o = L::QF;
+            const T lu = east ? l[o + 2] : l[o + 4];
+            c1 = T(4.0 / 3.0) * lu;
+            c2 = -(T(1.0 / 3.0) * lu);
+          }
+        }
+        const int slot = e.aux + 2 * (y - e.box[2]);
+        nside[slot] = double(c1);
+        nside[slot + 1] = double(c2);
+        bc_node_adj<L, T>(l, e);  // the node's own block: that of the Zou-He velocity / temperature condition
+        continue;
+      }
+    }
+    bc_node_adj<L, T>(l, e);
+  }
+}
+
+// After an adjoint sweep: the Neumann transfers of one piece added into the stored adjoints
+// m of the two nodes inwards of each of its nodes (flow: c_qx times the coefficient on every
+// direction; temperature: the coefficient on every D2Q5 direction). One thread per node of
+// the piece; the neighbours of different nodes are different, and pieces run one after another.
+template <class L, class P>
+__global__ __launch_bounds__(64) void k_neumann_fix(void* lam, Layout ll, DevBC e, const double* nside) {
+  using LS = typename P::LS;
+  const int y = e.box[2] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (y > e.box[3]) return;
+  const int slot = e.aux + 2 * (y - e.box[2]);
+  const int dir = -e.nrm[0];
+  const bool flow = e.type == LBMX_BC_ZOUHE_NEUMANN;
+  for (int k = 1; k <= 2; ++k) {
+    const LS c = LS(nside[slot + k - 1]);
+    const unsigned int ob = layout_off(ll, e.box[0] + k * dir, y, e.box[4]) * (unsigned int)sizeof(LS);
+    static_for<0, L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      char* base = own_base_w<q, LS>(lam, ll);
+      if constexpr (q < L::QF && L::ex[q] != 0) {
+        if (flow) st<LS>(base, ob, ld<LS>(base, ob) + LS(L::ex[q]) * c);
+      } else if constexpr (q >= L::QF) {
+        if (!flow) st<LS>(base, ob, ld<LS>(base, ob) + c);
+      }
+    });
   }
 }
 
@@ -250,7 +364,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
       static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
     } else {
       pull_node<L, P>(a.src, a.ls, off, f);
-      if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, tfac, x, y, z);
+      if (bnd) apply_bcs<L, P>(f, a.bcs, a.nbc, tfac, x, y, z, a.src, a.ls);
     }
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
@@ -343,7 +457,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
     static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(a.fwd_aos[(long long)n * L::Q + decltype(qc)::value]); });
   } else {
     pull_node<L, P>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
-    if (bnd) apply_bcs<L, T>(f, a.bcs, a.nbc, tfac, x, y, z);
+    if (bnd) apply_bcs<L, P>(f, a.bcs, a.nbc, tfac, x, y, z, a.fwd_src, a.ls);
   }
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
   // The adjoint field between sweeps is m_t, the boundary-corrected adjoint of s_t before the
@@ -378,7 +492,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   T dg = 0;
   reverse_collide<L, COLL, DES, T>(f, gam, a.m, lam, dg, srcA, srcB, srcAT);
   __builtin_nontemporal_store(__builtin_nontemporal_load(a.sens + n) + double(dg), a.sens + n);
-  if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, a.nbc, x, y, z);
+  if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, a.nbc, x, y, z, a.nside);
   // m_{t-1} at the own node (whole rows); the next adjoint sweep gathers its transposed stream
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -448,6 +562,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   AdjArgs a;
   a.fwd_src = static_cast<const char*>(st.src) + I.state_off;
   a.fwd_aos = nullptr;
+  a.nside = nullptr;
   a.lam_in = static_cast<const char*>(st.lam_in) + I.lam_off;
   a.lam_out = static_cast<char*>(st.lam_out) + I.lam_off;
   a.gamma = I.gamma;
@@ -484,6 +599,18 @@ template <class L, int COLL, int DES, class P>
 hipError_t launch_adj_t(const AdjArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_adj<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), 0, s, a);
   return hipGetLastError();
+}
+
+template <class L, class P>
+hipError_t launch_neumann_fix_t(const DevBC& e, void* lam, const Layout& ll, const double* nside, hipStream_t s) {
+  if constexpr (L::D != 2) {
+    return hipErrorInvalidValue;
+  } else {
+    const int n = e.box[3] - e.box[2] + 1;
+    if (n < 1) return hipSuccess;
+    hipLaunchKernelGGL((k_neumann_fix<L, P>), dim3((n + 63) / 64), dim3(64, 1, 1), 0, s, lam, ll, e, nside);
+    return hipGetLastError();
+  }
 }
 
 template <class L, int COLL, int DES, class P, int TY>

@@ -11,7 +11,9 @@ namespace lbmx {
   hipError_t launch_fwd_##SUFFIX(int coll, int des, const FwdArgs& a, hipStream_t s);                    \
   hipError_t launch_init_##SUFFIX(int coll, int des, const FwdArgs& a, const double* macros,             \
                                   const double* u, double rho0, double T0, hipStream_t s);               \
-  hipError_t launch_adj_##SUFFIX(int coll, int des, const AdjArgs& a, hipStream_t s);
+  hipError_t launch_adj_##SUFFIX(int coll, int des, const AdjArgs& a, hipStream_t s);                    \
+  hipError_t launch_neumann_fix_##SUFFIX(const DevBC& e, void* lam, const Layout& ll, const double* nside,  \
+                                         hipStream_t s);
 #define LBMX_DECLARE_LATTICE(LAT) \
   LBMX_DECLARE_LAUNCHERS(LAT##_f64) LBMX_DECLARE_LAUNCHERS(LAT##_f32) LBMX_DECLARE_LAUNCHERS(LAT##_f16) LBMX_DECLARE_LAUNCHERS(LAT##_bf16)
 LBMX_DECLARE_LATTICE(d2q9)

@@ -252,6 +252,18 @@ inline NewBoundaryDescriptor2d ThermalBouncebackDescriptor() {  // thermalBounce
   b.desc.type = LBMX_BC_THERMAL_BOUNCEBACK;
   return b;
 }
+// ZouHeNeumann2d (ZouHe2d.hh:528-566): east/west outflow, u_x from the two nodes inwards
+inline NewBoundaryDescriptor2d ZouHeNeumannDescriptor() {
+  NewBoundaryDescriptor2d b;
+  b.desc.type = LBMX_BC_ZOUHE_NEUMANN;
+  return b;
+}
+// ThermalZouHeNeumann2d (src/descriptors/thermalZouHeDescriptor2d.hh:55-116): T from the two nodes inwards
+inline NewBoundaryDescriptor2d ThermalZouHeNeumannDescriptor() {
+  NewBoundaryDescriptor2d b;
+  b.desc.type = LBMX_BC_THERMAL_NEUMANN;
+  return b;
+}
 
 // ---- macroscopic functionals (src/functionals/LBMacroFunctional.hh:6-14) ----
 class LBMacroFunctional {

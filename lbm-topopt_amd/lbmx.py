@@ -20,6 +20,7 @@ DESIGN_NONE, DESIGN_PBB, DESIGN_BRINKMAN, DESIGN_DARCY = 0, 1, 2, 3
 FP64, FP32, FP16_SHIFTED, BF16_SHIFTED = 0, 1, 2, 3
 BC_BOUNCEBACK, BC_ZOUHE_VELOCITY, BC_ZOUHE_PRESSURE, BC_ZOUHE_NEUMANN = 0, 1, 2, 3
 BC_HECHT_VELOCITY, BC_HECHT_PRESSURE, BC_THERMAL_BOUNCEBACK, BC_THERMAL_DIRICHLET = 4, 5, 6, 7
+BC_THERMAL_NEUMANN = 8
 PROFILE_UNIFORM, PROFILE_PARABOLIC, PROFILE_PARABOLIC_DISK, PROFILE_PARABOLIC_DUCT = 0, 1, 2, 3
 TIME_CONSTANT, TIME_SINE = 0, 1
 OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX = 0, 1, 2, 3, 4

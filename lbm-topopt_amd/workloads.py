@@ -58,9 +58,11 @@ def design_field(nx, ny, nz=1, seed=0, smooth=True):
 
 
 def channel2d(obj, nx, ny, u_max=0.05, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_DROP, time_const=1.0,
-              modulation=None):
+              modulation=None, outlet="pressure"):
     """Config-0/1 channel: Zou-He parabolic velocity inlet (x=0, 1<=y<=ny-2), Zou-He pressure
-    outlet (x=nx-1), on-grid bounce-back walls at y=0 and y=ny-1 (corners included)."""
+    outlet (x=nx-1) or, with outlet="neumann", the Zou-He Neumann outflow of
+    applications/pump2d/pump2d.cc:82 (u_x extrapolated from the two nodes inwards), on-grid
+    bounce-back walls at y=0 and y=ny-1 (corners included)."""
     B = lbmx.make_box
     mid = (ny - 1) / 2.0
     kw = {}
@@ -70,7 +72,10 @@ def channel2d(obj, nx, ny, u_max=0.05, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_
     obj.add_boundary(B(0, nx - 1, ny - 1, ny - 1), lbmx.make_bc(lbmx.BC_BOUNCEBACK))
     obj.add_boundary(B(0, 0, 1, ny - 2), lbmx.make_bc(lbmx.BC_ZOUHE_VELOCITY, u_max, (1.0, 0.0, 0.0),
                                                       lbmx.PROFILE_PARABOLIC, (mid, 0.0), (mid, 1.0), **kw))
-    obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_ZOUHE_PRESSURE, rho_out))
+    if outlet == "neumann":
+        obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_ZOUHE_NEUMANN))
+    else:
+        obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_ZOUHE_PRESSURE, rho_out))
     if objective != lbmx.OBJ_NONE:
         inlet = B(0, 0, 1, ny - 2)
         outlet = B(nx - 1, nx - 1, 1, ny - 2)
@@ -95,16 +100,22 @@ def duct3d(obj, nx, ny, nz, u_in=0.04, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_
         obj.set_objective(lbmx.make_objective(objective, inlet, outlet, time_const))
 
 
-def thermal_channel2d(obj, nx, ny, u_max=0.05, T_in=1.0, T_wall=0.0, time_const=1.0):
+def thermal_channel2d(obj, nx, ny, u_max=0.05, T_in=1.0, T_wall=0.0, time_const=1.0, thermal_outlet="adiabatic",
+                      outlet="pressure"):
     """Config-2 thermal channel: flow as channel2d; hot inlet T=1 (Dirichlet, west), cold
-    walls T=0 (Dirichlet, north/south), adiabatic outlet (bounce-back of the D2Q5 part);
+    walls T=0 (Dirichlet, north/south), adiabatic outlet (bounce-back of the D2Q5 part) or,
+    with thermal_outlet="neumann", the temperature extrapolated from the two nodes inwards
+    (ThermalZouHeNeumann2d, as applications/Regenerator2d/regenerator2d.cc:429 uses it);
     objective: outlet heat flux (ThermalOutflowEast2d)."""
     B = lbmx.make_box
-    channel2d(obj, nx, ny, u_max, objective=lbmx.OBJ_NONE)
+    channel2d(obj, nx, ny, u_max, objective=lbmx.OBJ_NONE, outlet=outlet)
     obj.add_boundary(B(1, nx - 2, 0, 0), lbmx.make_bc(lbmx.BC_THERMAL_DIRICHLET, T_wall))
     obj.add_boundary(B(1, nx - 2, ny - 1, ny - 1), lbmx.make_bc(lbmx.BC_THERMAL_DIRICHLET, T_wall))
     obj.add_boundary(B(0, 0, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_DIRICHLET, T_in))
-    obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_BOUNCEBACK))
+    if thermal_outlet == "neumann":
+        obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_NEUMANN))
+    else:
+        obj.add_boundary(B(nx - 1, nx - 1, 1, ny - 2), lbmx.make_bc(lbmx.BC_THERMAL_BOUNCEBACK))
     obj.set_objective(lbmx.make_objective(lbmx.OBJ_HEAT_FLUX, B(nx - 1, nx - 1, 1, ny - 2), None, time_const))
 
 

@@ -352,6 +352,7 @@ static int bc_supported(const orc_problem* p, int type, const int* n) {
     case LBMX_BC_HECHT_PRESSURE: return p->D == 3 && nn == 1 && n[0] == 1;
     case LBMX_BC_THERMAL_BOUNCEBACK: return p->Qt > 0;
     case LBMX_BC_THERMAL_DIRICHLET: return p->Qt > 0 && nn == 1;
+    case LBMX_BC_THERMAL_NEUMANN: return p->Qt > 0 && p->D == 2 && nn == 1 && n[1] == 0;
   }
   return 0;
 }
@@ -1082,6 +1083,16 @@ static void bc_node(const orc_problem* p, const orc_bc* e, int t, int i, int j, 
         if (cn < 0) g[q] = g[D2Q5_OPP[q]];
       }
     } break;
+    case LBMX_BC_THERMAL_NEUMANN: { /* ThermalZouHeNeumann{East,West}2d<.,D2Q5> (thermalZouHe2d.hh:229-300) */
+      double* g = f + p->Qf;
+      const double* g1 = nb1 + p->Qf;
+      const double* g2 = nb2 + p->Qf;
+      double T1 = g1[0] + g1[1] + g1[2] + g1[3] + g1[4];
+      double T2 = g2[0] + g2[1] + g2[2] + g2[3] + g2[4];
+      double T = (4. * T1 - T2) / 3.;
+      if (n[0] == 1) g[2] = T - g[0] - g[1] - g[3] - g[4];
+      else g[4] = T - g[0] - g[1] - g[2] - g[3];
+    } break;
     case LBMX_BC_THERMAL_DIRICHLET: { /* ThermalZouHeWest2d<.,D2Q5> (thermalZouHe2d.hh:62-93) */
       double* g = f + p->Qf;
       double T = bc_value(p, e, t, i, j, k);
@@ -1107,7 +1118,7 @@ void orc_apply_bc(const orc_problem* p, int piece, int t, double* f) {
     for (int j = e->box.y0; j <= e->box.y1; ++j)
       for (int i = e->box.x0; i <= e->box.x1; ++i) {
         const double *nb1 = NULL, *nb2 = NULL;
-        if (e->type == LBMX_BC_ZOUHE_NEUMANN) {
+        if (e->type == LBMX_BC_ZOUHE_NEUMANN || e->type == LBMX_BC_THERMAL_NEUMANN) {
           int dir = -e->nrm[0];
           nb1 = f + nidx(p, i + dir, j, k) * Q;
           nb2 = f + nidx(p, i + 2 * dir, j, k) * Q;
@@ -1123,7 +1134,7 @@ void orc_apply_bc(const orc_problem* p, int piece, int t, double* f) {
 void orc_apply_bc_adjoint(const orc_problem* p, int piece, int t, double* lam) {
   const orc_bc* e = &p->bcs[piece];
   const int Q = p->Q;
-  const int nblk = e->type == LBMX_BC_ZOUHE_NEUMANN ? 3 : 1;
+  const int nblk = (e->type == LBMX_BC_ZOUHE_NEUMANN || e->type == LBMX_BC_THERMAL_NEUMANN) ? 3 : 1;
   for (int k = e->box.z0; k <= e->box.z1; ++k)
     for (int j = e->box.y0; j <= e->box.y1; ++j)
       for (int i = e->box.x0; i <= e->box.x1; ++i) {

@@ -24,6 +24,11 @@ def build_pair(lattice, coll, design, nx, ny, nz=1, precision=lbmx.FP64, omega=1
             workloads.duct3d(obj, nx, ny, nz, time_const=time_const)
         elif setup == "thermal":
             workloads.thermal_channel2d(obj, nx, ny, time_const=time_const)
+        elif setup == "thermal-neumann":
+            workloads.thermal_channel2d(obj, nx, ny, time_const=time_const, thermal_outlet="neumann",
+                                        outlet="neumann")
+        elif setup == "neumann":
+            workloads.channel2d(obj, nx, ny, time_const=time_const, outlet="neumann")
         else:
             workloads.channel2d(obj, nx, ny, time_const=time_const, modulation=modulation)
         obj.set_design(g)

@@ -28,6 +28,8 @@ CASES = [
                            setup="duct", omega=1.6), 16),
     ("thermal", dict(lattice=lbmx.D2Q9_D2Q5, coll=lbmx.MRT, design=lbmx.DESIGN_PBB, nx=30, ny=14,
                      setup="thermal"), 20),
+    ("d2q9 mrt neumann outlet", dict(lattice=lbmx.D2Q9, coll=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN, nx=40, ny=18,
+                                     setup="neumann"), 30),
 ]
 
 

@@ -126,3 +126,42 @@ def test_forward_darcy_mrt(lbmx_mod):
     with pytest.raises(lbmx.LbmxError):
         s.gradient(5)
 
+
+
+@pytest.mark.parametrize("coll,design", [(lbmx.MRT, lbmx.DESIGN_BRINKMAN), (lbmx.BGK, lbmx.DESIGN_PBB),
+                                         (lbmx.CASCADED, lbmx.DESIGN_PBB)])
+def test_zouhe_neumann_outlet_forward_and_gradient(lbmx_mod, coll, design):
+    """IncompressibleZouHeNeumannEast2d outlet (u_x extrapolated from the two nodes inwards):
+    the forward reads the neighbours' post-stream values inside the sweep, the adjoint sends
+    the neighbours' share through the side buffer and the fix-up launch."""
+    s, o, _ = build_pair(lbmx.D2Q9, coll, design, 46, 22, setup="neumann", time_const=1.0 / 70)
+    s.init_equilibrium(); o.init_equilibrium()
+    s.forward(70); o.forward(70)
+    assert relmax(s.get_state(), o.f) < POP_TOL
+    s.checkpoint_config(0)
+    Js, gs = s.gradient(70)
+    Jo, go, ao = o.gradient(70)
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+    assert relmax(gs, go) < GRAD_TOL
+    assert relmax(s.get_adjoint(), ao) < GRAD_TOL
+    # checkpointed with recomputation: bitwise the stored-trajectory result
+    s.checkpoint_config(5)
+    J2, g2 = s.gradient(70)
+    assert J2 == Js and np.array_equal(g2, gs)
+
+
+def test_thermal_neumann_outlets_forward_and_gradient(lbmx_mod):
+    """ThermalZouHeNeumannEast2d<.,D2Q5> with a flow Neumann outlet on the thermal lattice."""
+    u0 = (0.03, 0.0, 0.0)
+    s, o, _ = build_pair(lbmx.D2Q9_D2Q5, lbmx.MRT, lbmx.DESIGN_PBB, 30, 14, setup="thermal-neumann",
+                         time_const=1.0 / 60)
+    s.init_equilibrium(u0=u0, T0=0.5); o.init_equilibrium(u0=u0, T0=0.5)
+    s.forward(60); o.forward(60)
+    assert relmax(s.get_state(), o.f) < POP_TOL
+    s.checkpoint_config(0)
+    Js, gs = s.gradient(60, u0=u0, T0=0.5)
+    Jo, go, ao = o.gradient(60, u0=u0, T0=0.5)
+    assert abs(Jo) > 1e-4
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+    assert relmax(gs, go) < GRAD_TOL
+    assert relmax(s.get_adjoint(), ao) < GRAD_TOL

@@ -507,6 +507,85 @@ def test_adjoint_boundary_conditions_are_transposes(lattice):
         assert abs(lhs - rhs) < 1e-11 * max(1, abs(lhs)), piece
 
 
+def _neumann_channel(thermal, nx=9, ny=6):
+    import workloads
+    if thermal:
+        o = mk(lbmx.D2Q9_D2Q5, lbmx.BGK, lbmx.DESIGN_NONE, nx=nx, ny=ny)
+        workloads.thermal_channel2d(o, nx, ny, thermal_outlet="neumann", outlet="neumann")
+    else:
+        o = mk(lbmx.D2Q9, lbmx.BGK, lbmx.DESIGN_NONE, nx=nx, ny=ny)
+        workloads.channel2d(o, nx, ny, outlet="neumann")
+    return o
+
+
+@pytest.mark.parametrize("thermal", [False, True], ids=["flow", "temperature"])
+def test_neumann_conditions_extrapolate(thermal):
+    """Zou-He Neumann outlets (ZouHe2d.hh:440-527, thermalZouHe2d.hh:229-300): after the
+    condition, the node's u_x (flow) or T (temperature) equals (4 m_1 - m_2)/3 of the two nodes
+    inwards, the flow's u_y is 0, and only the unknown populations change."""
+    o = _neumann_channel(thermal)
+    c, op, w = lat(o)
+    Q, Qf = len(w), 9
+    nx, ny = o.desc.nx, o.desc.ny
+    pieces = [(k, p) for k, p in enumerate(o.boundaries())
+              if p[0] in (lbmx.BC_ZOUHE_NEUMANN, lbmx.BC_THERMAL_NEUMANN)]
+    assert {p[0] for _, p in pieces} == ({lbmx.BC_ZOUHE_NEUMANN, lbmx.BC_THERMAL_NEUMANN} if thermal
+                                         else {lbmx.BC_ZOUHE_NEUMANN})
+    for piece, (typ, nrm, box) in pieces:
+        field = np.empty((ny, nx, Q))
+        for idx in np.ndindex(ny, nx):
+            ff = near_eq(c[:Qf], w[:Qf], 0.02)
+            if Q > Qf:
+                ff = np.concatenate([ff, 0.5 * w[Qf:] * (1 + 0.2 * RNG.standard_normal(Q - Qf))])
+            field[idx] = ff
+        out = o.apply_bc(piece, 3, field.ravel()).reshape(field.shape)
+        x0, x1, y0, y1 = box[:4]
+        assert x0 == x1 == nx - 1 and (y0, y1) == (1, ny - 2)
+        changed = np.zeros(Q, bool)
+        for j in range(y0, y1 + 1):
+            fi, fo = field[j, x0], out[j, x0]
+            n1, n2 = field[j, x0 - 1], field[j, x0 - 2]
+            if typ == lbmx.BC_ZOUHE_NEUMANN:
+                m1, m2 = c[:Qf, 0] @ n1[:Qf], c[:Qf, 0] @ n2[:Qf]
+                u = c[:Qf].T @ fo[:Qf]
+                assert abs(u[0] - (4 * m1 - m2) / 3) < 1e-15
+                assert abs(u[1]) < 1e-15
+                known = (c[:Qf] @ np.array(nrm, float)) >= 0
+                assert np.array_equal(fo[:Qf][known], fi[:Qf][known])
+            else:
+                m1, m2 = n1[Qf:].sum(), n2[Qf:].sum()
+                assert abs(fo[Qf:].sum() - (4 * m1 - m2) / 3) < 1e-15
+                assert np.array_equal(fo[:Qf], fi[:Qf])
+            changed |= fo != fi
+        # the unknowns: the three directions entering from the east (flow) or the one D2Q5 direction
+        expect = np.zeros(Q, bool)
+        if typ == lbmx.BC_ZOUHE_NEUMANN:
+            expect[:Qf] = c[:Qf, 0] < 0
+        else:
+            expect[Qf:] = c[Qf:, 0] < 0
+        assert np.array_equal(changed, expect)
+        # nothing else in the field moves
+        rest = np.ones((ny, nx), bool)
+        rest[y0:y1 + 1, x0] = False
+        assert np.array_equal(out[rest], field[rest])
+
+
+@pytest.mark.parametrize("thermal", [False, True], ids=["flow", "temperature"])
+def test_neumann_adjoints_are_transposes(thermal):
+    """The dot test of the whole-field map, which includes the neighbour blocks."""
+    o = _neumann_channel(thermal)
+    Q = len(lat(o)[2])
+    N = o.desc.nx * o.desc.ny * Q
+    for piece, (typ, _, _) in enumerate(o.boundaries()):
+        if typ not in (lbmx.BC_ZOUHE_NEUMANN, lbmx.BC_THERMAL_NEUMANN):
+            continue
+        f, d, lam = RNG.standard_normal(N), RNG.standard_normal(N), RNG.standard_normal(N)
+        Ld = o.apply_bc(piece, 2, f + d) - o.apply_bc(piece, 2, f)
+        lhs = lam @ Ld
+        rhs = o.apply_bc_adjoint(piece, 2, lam) @ d
+        assert abs(lhs - rhs) < 1e-11 * max(1, abs(lhs)), piece
+
+
 def test_boundary_piece_split_2d():
     """Edges exclude corners; corners get their own diagonal orientation (geometry2d.cc:41-49)."""
     o = _channel(lbmx.D2Q9, nx=7, ny=6)
@@ -560,6 +639,8 @@ FD_CASES = [
     (lbmx.D2Q9, lbmx.CASCADED, lbmx.DESIGN_PBB, (12, 8, 1), "channel"),
     (lbmx.D3Q19, lbmx.MRT, lbmx.DESIGN_BRINKMAN, (8, 6, 6), "duct"),
     (lbmx.D2Q9_D2Q5, lbmx.MRT, lbmx.DESIGN_BRINKMAN, (10, 7, 1), "thermal"),
+    (lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_BRINKMAN, (12, 8, 1), "neumann"),
+    (lbmx.D2Q9_D2Q5, lbmx.MRT, lbmx.DESIGN_PBB, (10, 7, 1), "thermal-neumann"),
 ]
 
 
@@ -574,11 +655,16 @@ def test_gradient_matches_central_differences(lattice, coll, design, shape, setu
         workloads.duct3d(o, nx, ny, nz, time_const=1.0 / steps)
     elif setup == "thermal":
         workloads.thermal_channel2d(o, nx, ny, time_const=1.0 / steps)
+    elif setup == "thermal-neumann":
+        workloads.thermal_channel2d(o, nx, ny, time_const=1.0 / steps, thermal_outlet="neumann", outlet="neumann")
+    elif setup == "neumann":
+        workloads.channel2d(o, nx, ny, time_const=1.0 / steps, outlet="neumann")
     else:
         workloads.channel2d(o, nx, ny, time_const=1.0 / steps)
     g = workloads.design_field(nx, ny, nz, seed=5)
     o.set_design(g)
-    kw = dict(u0=(0.03, 0, 0), T0=0.5) if setup == "thermal" else {}
+    thermal = setup.startswith("thermal")
+    kw = dict(u0=(0.03, 0, 0), T0=0.5) if thermal else {}
     J, sens, lam0 = o.gradient(steps, **kw)
     # The reference's reverse loop runs while step > 0, so the first collision (of the initial
     # equilibrium) is never reverse-collided (applications/RobustPump2d/robustPump2d.cc:268-276).
@@ -588,7 +674,7 @@ def test_gradient_matches_central_differences(lattice, coll, design, shape, setu
     Q = o.Q
     step0 = np.array([o.reverse_collide_node(o.f[n * Q:(n + 1) * Q], g[n], lam0[n * Q:(n + 1) * Q])[1]
                       for n in range(nx * ny * nz)])
-    if setup != "thermal":
+    if not thermal:
         assert np.max(np.abs(step0)) <= 1e-13 * np.max(np.abs(sens))
     full = sens + step0
 
