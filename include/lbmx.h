@@ -402,6 +402,17 @@ lbmx_status lbmx_vec_load_binary(const char* path, double* v, size_t n);
 /* LBSolver::outputFields (src/latticeBoltzmann/LBSolver.cc:251-286): the current state's macros
    (rho, ux, uy[, uz][, T]) and the design field as a VTK XML StructuredGrid (.vts, ASCII). */
 lbmx_status lbmx_write_vts(lbmx_solver* s, const char* path);
+/* PETSc DMDA orderings (the reference's distributed Vecs, DMDACreate2d/3d in
+   src/latticeBoltzmann/NewLBSolver2d.cc and LBSolver3d.cc): a global Vec of a DMDA split over
+   procs[0] x procs[1] x procs[2] ranks holds the ranks' blocks one after another (rank
+   i + procs[0] (j + procs[1] k)), each block in natural order over its own nodes, `dof` values
+   per node. lx[procs[0]], ly[procs[1]], lz[procs[2]] are the nodes each rank owns along an axis
+   (DMDAGetOwnershipRanges; lz may be NULL when procs[2] == 1). These convert such a Vec's
+   array to the natural order the engine uses (DMDAGlobalToNatural) and back. */
+lbmx_status lbmx_dmda_global_to_natural(const int dims[3], const int procs[3], const int* lx, const int* ly,
+                                        const int* lz, int dof, const double* global, double* natural);
+lbmx_status lbmx_dmda_natural_to_global(const int dims[3], const int procs[3], const int* lx, const int* ly,
+                                        const int* lz, int dof, const double* natural, double* global);
 
 /* ---- method of moving asymptotes (the design update of the optimisation loop) ----
    MMA of src/topOpt/MMA.cc (Aage's PETSc MMA with the reference's robust constants): each

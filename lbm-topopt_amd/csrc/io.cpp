@@ -1,7 +1,8 @@
 // Host-side file formats either side of the engine: the PETSc binary Vec format the reference
 // uses for design restarts (writeVectorToBinary / loadVectorFromBinary,
-// src/core/binaryIO.cc:3-34), and a VTK XML StructuredGrid (.vts) of the macroscopic fields
-// and the design (LBSolver::outputFields, src/latticeBoltzmann/LBSolver.cc:251-286).
+// src/core/binaryIO.cc:3-34), a VTK XML StructuredGrid (.vts) of the macroscopic fields and
+// the design (LBSolver::outputFields, src/latticeBoltzmann/LBSolver.cc:251-286), and the
+// DMDA global <-> natural orderings of the reference's distributed Vecs.
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -41,6 +42,50 @@ struct File {
     if (f) std::fclose(f);
   }
 };
+
+// Walks a DMDA's ranks in PETSc order and each rank's block in natural order, calling
+// visit(natural node index, global node index) for every node.
+template <class F>
+void dmda_walk(const int dims[3], const int procs[3], const int* lx, const int* ly, const int* lz, F&& visit) {
+  const int* own[3] = {lx, ly, lz};
+  std::vector<long long> start[3];
+  for (int a = 0; a < 3; ++a) {
+    if (dims[a] < 1 || procs[a] < 1) throw Error{LBMX_ERR_ARG, "grid dimensions and rank counts must be positive"};
+    if (!own[a] && procs[a] != 1) throw Error{LBMX_ERR_ARG, "ownership ranges missing for a split axis"};
+    start[a].assign((size_t)procs[a] + 1, 0);
+    for (int r = 0; r < procs[a]; ++r) {
+      const int w = own[a] ? own[a][r] : dims[a];
+      if (w < 0) throw Error{LBMX_ERR_ARG, "negative ownership range"};
+      start[a][r + 1] = start[a][r] + w;
+    }
+    if (start[a][procs[a]] != dims[a]) throw Error{LBMX_ERR_ARG, "ownership ranges do not add up to the grid"};
+  }
+  long long g = 0;
+  for (int rk = 0; rk < procs[2]; ++rk)
+    for (int rj = 0; rj < procs[1]; ++rj)
+      for (int ri = 0; ri < procs[0]; ++ri)
+        for (long long k = start[2][rk]; k < start[2][rk + 1]; ++k)
+          for (long long j = start[1][rj]; j < start[1][rj + 1]; ++j)
+            for (long long i = start[0][ri]; i < start[0][ri + 1]; ++i)
+              visit((k * dims[1] + j) * dims[0] + i, g++);
+}
+
+lbmx_status dmda_convert(const int dims[3], const int procs[3], const int* lx, const int* ly, const int* lz, int dof,
+                         const double* in, double* out, bool to_natural) {
+  if (!dims || !procs || !in || !out) {
+    g_last_error = "null argument";
+    return LBMX_ERR_ARG;
+  }
+  return guarded([&] {
+    if (dof < 1) throw Error{LBMX_ERR_ARG, "dof must be >= 1"};
+    if (in == out) throw Error{LBMX_ERR_ARG, "the conversion is not in place"};
+    dmda_walk(dims, procs, lx, ly, lz, [&](long long nat, long long glob) {
+      const double* src = in + (to_natural ? glob : nat) * dof;
+      double* dst = out + (to_natural ? nat : glob) * dof;
+      for (int c = 0; c < dof; ++c) dst[c] = src[c];
+    });
+  });
+}
 
 }  // namespace
 }  // namespace lbmx
@@ -94,6 +139,16 @@ lbmx_status lbmx_vec_load_binary(const char* path, double* v, size_t n) {
       std::memcpy(&v[i], &bits, 8);
     }
   });
+}
+
+lbmx_status lbmx_dmda_global_to_natural(const int dims[3], const int procs[3], const int* lx, const int* ly,
+                                        const int* lz, int dof, const double* global, double* natural) {
+  return lbmx::dmda_convert(dims, procs, lx, ly, lz, dof, global, natural, true);
+}
+
+lbmx_status lbmx_dmda_natural_to_global(const int dims[3], const int procs[3], const int* lx, const int* ly,
+                                        const int* lz, int dof, const double* natural, double* global) {
+  return lbmx::dmda_convert(dims, procs, lx, ly, lz, dof, natural, global, false);
 }
 
 lbmx_status lbmx_write_vts(lbmx_solver* s, const char* path) {

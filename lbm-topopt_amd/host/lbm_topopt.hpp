@@ -333,6 +333,25 @@ inline PetscErrorCode loadVectorFromBinary(const std::string& folder, const std:
   return lbmx_vec_load_binary((folder + name).c_str(), v.data(), v.size());
 }
 
+// DMDAGlobalToNatural / DMDANaturalToGlobal for a DMDA split over lx.size() x ly.size() x
+// lz.size() ranks (ownership ranges as DMDAGetOwnershipRanges gives them), dof values per node
+inline PetscErrorCode DMDAGlobalToNatural(const int dims[3], const std::vector<int>& lx, const std::vector<int>& ly,
+                                          const std::vector<int>& lz, int dof, const std::vector<double>& global,
+                                          std::vector<double>& natural) {
+  const int procs[3] = {(int)lx.size(), (int)ly.size(), lz.empty() ? 1 : (int)lz.size()};
+  natural.resize(global.size());
+  return lbmx_dmda_global_to_natural(dims, procs, lx.data(), ly.data(), lz.empty() ? nullptr : lz.data(), dof,
+                                     global.data(), natural.data());
+}
+inline PetscErrorCode DMDANaturalToGlobal(const int dims[3], const std::vector<int>& lx, const std::vector<int>& ly,
+                                          const std::vector<int>& lz, int dof, const std::vector<double>& natural,
+                                          std::vector<double>& global) {
+  const int procs[3] = {(int)lx.size(), (int)ly.size(), lz.empty() ? 1 : (int)lz.size()};
+  global.resize(natural.size());
+  return lbmx_dmda_natural_to_global(dims, procs, lx.data(), ly.data(), lz.empty() ? nullptr : lz.data(), dof,
+                                     natural.data(), global.data());
+}
+
 class AdjointLBSolver;
 
 // ---- solvers (src/latticeBoltzmann/LBSolver.hh:14-73, NewLBSolver2d.hh, isothermalLBSolver2d.hh) ----

@@ -194,6 +194,8 @@ def library():
         "lbmx_vec_write_binary": [C.c_char_p, dp, C.c_size_t],
         "lbmx_vec_load_binary": [C.c_char_p, dp, C.c_size_t],
         "lbmx_write_vts": [C.c_void_p, C.c_char_p],
+        "lbmx_dmda_global_to_natural": [P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_int, dp, dp],
+        "lbmx_dmda_natural_to_global": [P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_int, dp, dp],
         "lbmx_mma_create": [C.c_longlong, C.c_int, P(C.c_void_p)],
         "lbmx_mma_destroy": [C.c_void_p],
         "lbmx_mma_set_robust_constants": [C.c_void_p, dp],
@@ -232,7 +234,8 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_mma_design_change", "lbmx_mma_get_state", "lbmx_adjoint_reset", "lbmx_adjoint_step", "lbmx_adjoint_set_timestep",
             "lbmx_set_design_device", "lbmx_get_sensitivity_device", "lbmx_filter_design_device",
             "lbmx_filter_design_to_solver_device", "lbmx_filter_sensitivities_device",
-            "lbmx_filter_solver_sensitivities_device", "lbmx_mma_update_device", "lbmx_mma_design_change_device"]
+            "lbmx_filter_solver_sensitivities_device", "lbmx_mma_update_device", "lbmx_mma_design_change_device",
+            "lbmx_dmda_global_to_natural", "lbmx_dmda_natural_to_global"]
 
 
 def checkpoint_simulate(nslots, nsteps):
@@ -268,6 +271,29 @@ def load_petsc_vec(path, n):
     a = np.empty(n)
     _check(library().lbmx_vec_load_binary(os.fsencode(path), _dptr(a), n))
     return a
+
+
+def _dmda_call(name, dims, lx, ly, lz, dof, arr):
+    dims3 = list(dims) + [1] * (3 - len(dims))
+    own = [np.asarray(v, dtype=np.int32) if v is not None else None for v in (lx, ly, lz)]
+    procs = [len(v) if v is not None else 1 for v in own]
+    a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+    out = np.empty_like(a)
+    ip = C.POINTER(C.c_int)
+    cptr = [v.ctypes.data_as(ip) if v is not None else None for v in own]
+    _check(getattr(library(), name)((C.c_int * 3)(*dims3), (C.c_int * 3)(*procs), cptr[0], cptr[1], cptr[2], dof,
+                                    _dptr(a), _dptr(out)))
+    return out
+
+
+def dmda_global_to_natural(arr, dims, lx, ly, lz=None, dof=1):
+    """A DMDA global Vec's array (ranks' blocks in PETSc order) in natural order; lx, ly, lz:
+    nodes owned per rank along each axis (DMDAGetOwnershipRanges)."""
+    return _dmda_call("lbmx_dmda_global_to_natural", dims, lx, ly, lz, dof, arr)
+
+
+def dmda_natural_to_global(arr, dims, lx, ly, lz=None, dof=1):
+    return _dmda_call("lbmx_dmda_natural_to_global", dims, lx, ly, lz, dof, arr)
 
 
 def plane_copy_gbs(planes, nodes, iters=20):

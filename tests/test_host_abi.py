@@ -70,6 +70,41 @@ def test_petsc_binary_vec_format(tmp_path):
         lbmx.load_petsc_vec(path, 4)                              # truncated data
 
 
+def test_dmda_orderings():
+    """DMDA global (PETSc) order <-> natural order. Worked example from PETSc's ordering rule:
+    a 3 x 2 grid over 2 x 1 ranks owning 2 and 1 columns holds rank 0's nodes (0,0), (1,0),
+    (0,1), (1,1), then rank 1's (2,0), (2,1), i.e. natural indices 0, 1, 3, 4, 2, 5."""
+    natural = np.arange(6, dtype=float) * 10
+    glob = lbmx.dmda_natural_to_global(natural, (3, 2), [2, 1], [2])
+    assert np.array_equal(glob, [0, 10, 30, 40, 20, 50])
+    assert np.array_equal(lbmx.dmda_global_to_natural(glob, (3, 2), [2, 1], [2]), natural)
+    # one rank: the orders coincide
+    v = np.random.default_rng(3).standard_normal(7 * 5 * 3 * 2)
+    assert np.array_equal(lbmx.dmda_natural_to_global(v, (7, 5, 3), [7], [5], [3], dof=2), v)
+    # 3D, dof = 19, 3 x 2 x 2 ranks with uneven ranges: the node at natural (i, j, k) lands in
+    # the block of the rank owning it, at its natural position inside that block
+    dims, lx, ly, lz, dof = (7, 5, 4), [3, 2, 2], [2, 3], [1, 3], 19
+    nat = np.random.default_rng(4).standard_normal(7 * 5 * 4 * dof)
+    glob = lbmx.dmda_natural_to_global(nat, dims, lx, ly, lz, dof=dof)
+    sx, sy, sz = np.cumsum([0] + lx), np.cumsum([0] + ly), np.cumsum([0] + lz)
+    for (i, j, k) in [(0, 0, 0), (6, 4, 3), (3, 2, 1), (4, 1, 2), (2, 4, 0)]:
+        ri, rj, rk = (np.searchsorted(sx, i, "right") - 1, np.searchsorted(sy, j, "right") - 1,
+                      np.searchsorted(sz, k, "right") - 1)
+        blocks_before = 0
+        for kk in range(len(lz)):
+            for jj in range(len(ly)):
+                for ii in range(len(lx)):
+                    if (kk, jj, ii) < (rk, rj, ri):
+                        blocks_before += lx[ii] * ly[jj] * lz[kk]
+        local = ((k - sz[rk]) * ly[rj] + (j - sy[rj])) * lx[ri] + (i - sx[ri])
+        g = blocks_before + local
+        n = (k * 5 + j) * 7 + i
+        assert np.array_equal(glob[g * dof:(g + 1) * dof], nat[n * dof:(n + 1) * dof])
+    assert np.array_equal(lbmx.dmda_global_to_natural(glob, dims, lx, ly, lz, dof=dof), nat)
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.dmda_natural_to_global(nat, dims, [3, 2, 1], ly, lz, dof=dof)  # ranges do not add up
+
+
 def test_error_paths_without_device():
     lib = lbmx.library()
     import ctypes as C
