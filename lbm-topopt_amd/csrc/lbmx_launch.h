@@ -87,5 +87,7 @@ hipError_t launch_mma_change(const double* xnew, const double* xo1, long long n,
 // Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
 hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 hipError_t launch_batch_d2q9_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+hipError_t launch_batch_d3q19_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+hipError_t launch_batch_d3q19_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 
 }  // namespace lbmx
