@@ -207,6 +207,13 @@ struct Engine {
     m.ramp_p = d.ramp_p;
     m.ramp_ck = d.ramp_ck;
     m.design = d.design;
+    m.omega_f = (float)m.omega;
+    m.alpha_max_f = (float)m.alpha_max;
+    m.interp_q_f = (float)m.interp_q;
+    for (int k = 0; k < 3; ++k) m.body_f[k] = (float)m.body[k];
+    m.tdiff_f = (float)m.tdiff;
+    m.ramp_p_f = (float)m.ramp_p;
+    m.ramp_ck_f = (float)m.ramp_ck;
     return m;
   }
 

@@ -19,6 +19,11 @@ enum : int { D_NONE = LBMX_DESIGN_NONE, D_PBB = LBMX_DESIGN_PBB, D_BRINK = LBMX_
 #define LBMX_BAS BasisOf<L::QF>::B
 
 template <class T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
+// a model parameter in the arithmetic type (FP32 forms read the copy rounded on the host)
+template <class T> __device__ __forceinline__ T mpar(double d, float f) {
+  if constexpr (sizeof(T) == 4) return f;
+  else return T(d);
+}
 template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // BPInterpolation (src/core/interpolationFunctions.hh:18-25)
@@ -266,7 +271,7 @@ __device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3
 // (src/dynamics/thermalObstacleCollision.hh:47-53, src/dynamics/thermalMRT2d.hh:76-101)
 template <class T>
 __device__ __forceinline__ T thermal_omega(const ModelParams& mp, T gamma) {
-  T diff = T(mp.tdiff) * ramp_interp<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
+  T diff = mpar<T>(mp.tdiff, mp.tdiff_f) * ramp_interp<T>(mpar<T>(mp.ramp_p, mp.ramp_p_f), mpar<T>(mp.ramp_ck, mp.ramp_ck_f), gamma);
   return T(1) / (T(5) * diff + T(0.5));
 }
 
@@ -447,11 +452,11 @@ __device__ __forceinline__ void collide_cascaded(T (&f)[L::Q], T gamma, const Mo
   static_assert(L::Q == 9 && L::QF == 9, "the cascaded operator is defined for D2Q9");
   flow_macros<L>(f, rho, j);
   T k[6];
-  casc_k<T>(f, rho, j[0], j[1], T(mp.omega), k);
+  casc_k<T>(f, rho, j[0], j[1], mpar<T>(mp.omega, mp.omega_f), k);
   T g[9];
   casc_apply<T>(f, k, g);
   if constexpr (DES == D_PBB) {
-    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T imult = T(0.5) * mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
     const T sc = T(1) - T(2) * imult;
     static_for<1, 5>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
@@ -469,12 +474,12 @@ __device__ __forceinline__ void reverse_cascaded(const T (&f)[L::Q], T gamma, co
   static_assert(L::Q == 9 && L::QF == 9, "the cascaded operator is defined for D2Q9");
   T rho, j[3];
   flow_macros<L>(f, rho, j);
-  const T om = T(mp.omega);
+  const T om = mpar<T>(mp.omega, mp.omega_f);
   if constexpr (DES == D_PBB) {
     T k[6], g[9];
     casc_k<T>(f, rho, j[0], j[1], om, k);
     casc_apply<T>(f, k, g);
-    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T imult = T(0.5) * mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
     const T sc = T(1) - T(2) * imult;
     T mbar = 0;
     static_for<1, 5>([&](auto dc) {
@@ -485,7 +490,7 @@ __device__ __forceinline__ void reverse_cascaded(const T (&f)[L::Q], T gamma, co
       lam[d] = T(0.5) * (ls + las);
       lam[d + 4] = T(0.5) * (ls - las);
     });
-    dgam += mbar * T(0.5) * T(mp.alpha_max) * bp_der<T>(T(mp.interp_q), gamma);
+    dgam += mbar * T(0.5) * mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_der<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
   }
   T rb = srcA, xb = srcB[0], yb = srcB[1];
   casc_reverse<T>(f, rho, j[0], j[1], om, lam, rb, xb, yb);
@@ -503,10 +508,10 @@ __device__ __forceinline__ void reverse_cascaded(const T (&f)[L::Q], T gamma, co
 // through the operator's closed-form polynomials (MRT rates s1 = s2 = 1.4).
 template <class T>
 __device__ __forceinline__ void darcy_force(T (&f)[9], T rho, const T (&j)[3], T gamma, const ModelParams& mp) {
-  const T s1 = T(1.4), s2 = T(1.4), om = T(mp.omega);
+  const T s1 = T(1.4), s2 = T(1.4), om = mpar<T>(mp.omega, mp.omega_f);
   const T ux = j[0], uy = j[1];
-  const T im = -T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma) * rho;
-  const T Fx = im * ux + T(mp.body[0]), Fy = im * uy + T(mp.body[1]);
+  const T im = -mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma) * rho;
+  const T Fx = im * ux + mpar<T>(mp.body[0], mp.body_f[0]), Fy = im * uy + mpar<T>(mp.body[1], mp.body_f[1]);
   const T o3 = T(1) / T(3), o6 = T(1) / T(6), o12 = T(1) / T(12), o24 = T(1) / T(24);
   const T ax = (-T(4) * s1 + T(2) * s2 + T(4)) * ux;           // common x-part of the diagonal terms
   const T by = (s1 - T(0.5) * s2 - T(1)) * uy;
@@ -529,15 +534,15 @@ __device__ __forceinline__ void collide_moment(T (&f)[L::Q], T gamma, const Mode
   Pairs<L, T> p;
   to_pairs<L>(f, p);
   pair_macros<L>(p, rho, j);
-  const T omega = T(mp.omega);
+  const T omega = mpar<T>(mp.omega, mp.omega_f);
   T alpha = 0, beta, v[3];
-  if constexpr (DES == D_BRINK) alpha = T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+  if constexpr (DES == D_BRINK) alpha = mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
   eq_velocity<L, COLL, DES>(rho, j, alpha, v, beta);
   base_relax<L, COLL, DES>(p, rho, v, omega, alpha);
   if constexpr (DES == D_PBB) {
     // PartialBouncebackCollision (src/dynamics/partialBouncebackCollision.hh:53-65): the
     // pair blend keeps f_d + f_d' and scales f_d - f_d' by 1 - 2 imult
-    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T imult = T(0.5) * mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
     const T sc = T(1) - T(2) * imult;
     static_for<1, L::HALF + 1>([&](auto dc) { p.a[decltype(dc)::value] *= sc; });
   }
@@ -566,9 +571,9 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
   to_pairs<L>(f, pf);
   T rho, j[3];
   pair_macros<L>(pf, rho, j);
-  const T omega = T(mp.omega);
+  const T omega = mpar<T>(mp.omega, mp.omega_f);
   T alpha = 0, beta, v[3];
-  if constexpr (DES == D_BRINK) alpha = T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+  if constexpr (DES == D_BRINK) alpha = mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
   eq_velocity<L, COLL, DES>(rho, j, alpha, v, beta);
   const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   T gbar = 0;
@@ -602,7 +607,7 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
       });
       static_for<1, H + 1>([&](auto dc) { gd[decltype(dc)::value] = g.a[decltype(dc)::value]; });
     }
-    const T imult = T(0.5) * T(mp.alpha_max) * bp_val<T>(T(mp.interp_q), gamma);
+    const T imult = T(0.5) * mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
     const T sc = T(1) - T(2) * imult;
     T mbar = 0;
     static_for<1, H + 1>([&](auto dc) {
@@ -610,7 +615,7 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
       mbar -= pl.a[d] * gd[d];
       pl.a[d] *= sc;
     });
-    gbar += mbar * T(0.5) * T(mp.alpha_max) * bp_der<T>(T(mp.interp_q), gamma);
+    gbar += mbar * T(0.5) * mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_der<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
   }
 
   T rho_bar = 0, vb[3] = {0, 0, 0}, alpha_bar = 0;
@@ -711,7 +716,7 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
   rho_bar += srcA;
   if constexpr (DES == D_BRINK) {
     alpha_bar -= T(0.5) * beta * beta * (vb[0] * j[0] + vb[1] * j[1] + vb[2] * j[2]);
-    gbar += alpha_bar * T(mp.alpha_max) * bp_der<T>(T(mp.interp_q), gamma);
+    gbar += alpha_bar * mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_der<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
   }
 
   if constexpr (L::QT > 0) {
@@ -758,8 +763,8 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
       });
       lam[o + q] = acc + Tbar;
     });
-    T dR = ramp_diff<T>(T(mp.ramp_p), T(mp.ramp_ck), gamma);
-    gbar += ombar * (-omT * omT * T(5) * T(mp.tdiff) * dR);
+    T dR = ramp_diff<T>(mpar<T>(mp.ramp_p, mp.ramp_p_f), mpar<T>(mp.ramp_ck, mp.ramp_ck_f), gamma);
+    gbar += ombar * (-omT * omT * T(5) * mpar<T>(mp.tdiff, mp.tdiff_f) * dR);
   }
 
   // lam_pre = lam_f + rho_bar + c.j_bar (pair space: f0 += rb, s += 2 rb, a += 2 c.jb)

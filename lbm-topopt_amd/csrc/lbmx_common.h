@@ -45,6 +45,9 @@ struct ModelParams {
   double tdiff, ramp_p, ramp_ck;
   int design;
   int pad;
+  // the same values rounded to FP32 on the host, read by the FP32-arithmetic forms (no
+  // per-wave conversions in the sweeps)
+  float omega_f, alpha_max_f, interp_q_f, body_f[3], tdiff_f, ramp_p_f, ramp_ck_f, pad_f;
 };
 
 // Padded structure-of-arrays layout of one population field in HBM. Interior node (x,y,z)
