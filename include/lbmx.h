@@ -149,8 +149,10 @@ typedef enum {
   LBMX_OBJ_OUTFLOW = 2,           /* OutflowEast2d (src/functionals/outflow2d.hh:15-54): -mean(u_x, box_a) */
   LBMX_OBJ_DISSIPATED_POWER = 3,  /* total-pressure flux through the ports, defined here:
                                      sum_box_a (p+|u|^2/2) u_x - sum_box_b (p+|u|^2/2) u_x      */
-  LBMX_OBJ_HEAT_FLUX = 4          /* ThermalOutflowEast2d (src/functionals/temperatureFlow2d.hh:18-54):
+  LBMX_OBJ_HEAT_FLUX = 4,         /* ThermalOutflowEast2d (src/functionals/temperatureFlow2d.hh:18-54):
                                      -sum_box_a u_x * T                                          */
+  LBMX_OBJ_THROUGHFLOW = 5        /* ThroughFlow2d (src/functionals/throughFlow2d.hh:16-88):
+                                     -mean(u_x, box_a) - mean(u_x, box_b)                        */
 } lbmx_objective_kind;
 
 typedef struct {

@@ -377,6 +377,10 @@ struct Engine {
         term(o.box_b, -1.0 / size(o.box_b), 0);
         break;
       case LBMX_OBJ_OUTFLOW: term(o.box_a, -1.0 / size(o.box_a), 1); break;
+      case LBMX_OBJ_THROUGHFLOW:  // ThroughFlow2d (src/functionals/throughFlow2d.hh:16-88)
+        term(o.box_a, -1.0 / size(o.box_a), 1);
+        term(o.box_b, -1.0 / size(o.box_b), 1);
+        break;
       case LBMX_OBJ_DISSIPATED_POWER:
         term(o.box_a, 1.0, 2);
         term(o.box_b, -1.0, 2);

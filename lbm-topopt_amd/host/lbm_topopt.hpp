@@ -297,6 +297,7 @@ class LBMacroFunctional {
     switch (kind_) {
       case LBMX_OBJ_PRESSURE_DROP: return sum(a_, rho) / count(a_) - sum(b_, rho) / count(b_);
       case LBMX_OBJ_OUTFLOW: return -sum(a_, ux) / count(a_);
+      case LBMX_OBJ_THROUGHFLOW: return -sum(a_, ux) / count(a_) - sum(b_, ux) / count(b_);
       case LBMX_OBJ_DISSIPATED_POWER: return sum(a_, power) - sum(b_, power);
       case LBMX_OBJ_HEAT_FLUX: return -sum(a_, heat);
       default: return 0.0;
@@ -315,6 +316,9 @@ inline LBMacroFunctional PressureDrop3d(const Box3d& in, const Box3d& out) {
 }
 inline LBMacroFunctional OutflowEast2d(const Box2d& out) {  // outflow2d.hh:15-54
   return LBMacroFunctional(LBMX_OBJ_OUTFLOW, out.raw());
+}
+inline LBMacroFunctional ThroughFlow2d(const Box2d& in, const Box2d& out) {  // throughFlow2d.hh:16-88
+  return LBMacroFunctional(LBMX_OBJ_THROUGHFLOW, in.raw(), out.raw());
 }
 inline LBMacroFunctional DissipatedPower2d(const Box2d& in, const Box2d& out) {
   return LBMacroFunctional(LBMX_OBJ_DISSIPATED_POWER, in.raw(), out.raw());

@@ -23,7 +23,7 @@ BC_HECHT_VELOCITY, BC_HECHT_PRESSURE, BC_THERMAL_BOUNCEBACK, BC_THERMAL_DIRICHLE
 BC_THERMAL_NEUMANN = 8
 PROFILE_UNIFORM, PROFILE_PARABOLIC, PROFILE_PARABOLIC_DISK, PROFILE_PARABOLIC_DUCT = 0, 1, 2, 3
 TIME_CONSTANT, TIME_SINE = 0, 1
-OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX = 0, 1, 2, 3, 4
+OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX, OBJ_THROUGHFLOW = 0, 1, 2, 3, 4, 5
 
 LBMX_OK, ERR_ARG, ERR_SUP, ERR_MEM, ERR_STATE, ERR_DEVICE = 0, 62, 56, 55, 73, 76
 

@@ -319,6 +319,11 @@ int orc_set_objective(orc_problem* p, const lbmx_objective* o) {
       p->terms[0] = (orc_term){o->box_a, -1.0 / box_size(&o->box_a), 1};
       p->nterm = 1;
       break;
+    case LBMX_OBJ_THROUGHFLOW: /* ThroughFlow2d (src/functionals/throughFlow2d.hh:16-88) */
+      p->terms[0] = (orc_term){o->box_a, -1.0 / box_size(&o->box_a), 1};
+      p->terms[1] = (orc_term){o->box_b, -1.0 / box_size(&o->box_b), 1};
+      p->nterm = 2;
+      break;
     case LBMX_OBJ_DISSIPATED_POWER:
       p->terms[0] = (orc_term){o->box_a, 1.0, 2};
       p->terms[1] = (orc_term){o->box_b, -1.0, 2};

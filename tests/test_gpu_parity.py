@@ -165,3 +165,17 @@ def test_thermal_neumann_outlets_forward_and_gradient(lbmx_mod):
     assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
     assert relmax(gs, go) < GRAD_TOL
     assert relmax(s.get_adjoint(), ao) < GRAD_TOL
+
+
+def test_throughflow_objective_gradient(lbmx_mod):
+    """ThroughFlow2d (src/functionals/throughFlow2d.hh:16-88): J and dJ/dgamma against the oracle."""
+    s, o, _ = build_pair(lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_BRINKMAN, 44, 20, time_const=1.0 / 60)
+    inlet, outlet = lbmx.make_box(0, 0, 1, 18), lbmx.make_box(43, 43, 1, 18)
+    for obj in (s, o):
+        obj.set_objective(lbmx.make_objective(lbmx.OBJ_THROUGHFLOW, inlet, outlet, 1.0 / 60))
+    s.checkpoint_config(0)
+    Js, gs = s.gradient(60)
+    Jo, go, _ = o.gradient(60)
+    assert abs(Jo) > 1e-3
+    assert abs(Js - Jo) <= OBJ_TOL * abs(Jo)
+    assert relmax(gs, go) < GRAD_TOL

@@ -601,7 +601,8 @@ def test_boundary_piece_split_2d():
 
 
 # ================================================================ functionals
-@pytest.mark.parametrize("kind", [lbmx.OBJ_PRESSURE_DROP, lbmx.OBJ_OUTFLOW, lbmx.OBJ_DISSIPATED_POWER])
+@pytest.mark.parametrize("kind", [lbmx.OBJ_PRESSURE_DROP, lbmx.OBJ_OUTFLOW, lbmx.OBJ_DISSIPATED_POWER,
+                                  lbmx.OBJ_THROUGHFLOW])
 def test_functional_closed_form_and_gradient(kind):
     nx, ny = 6, 5
     o = mk(lbmx.D2Q9, nx=nx, ny=ny)
@@ -612,8 +613,9 @@ def test_functional_closed_form_and_gradient(kind):
     o.init_equilibrium(1.02, u)
     J = o.objective(o.f)
     p = 1.02 / 3 + 0.5 * (u @ u)
+    # ThroughFlow2d: -(mean u_x over the inflow) - (mean u_x over the outflow), times timeConst
     exp = {lbmx.OBJ_PRESSURE_DROP: 0.0, lbmx.OBJ_OUTFLOW: -0.5 * 0.03,
-           lbmx.OBJ_DISSIPATED_POWER: 0.0}[kind]
+           lbmx.OBJ_DISSIPATED_POWER: 0.0, lbmx.OBJ_THROUGHFLOW: -0.5 * 0.06}[kind]
     assert abs(J - exp) < 1e-15
     # non-uniform state: central differences of J
     f = o.f + 0.001 * RNG.standard_normal(o.f.size)
