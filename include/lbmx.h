@@ -212,6 +212,14 @@ lbmx_status lbmx_get_timestep(lbmx_solver* s, int* t);
 /* Wang dynamic checkpointing with nslots HBM slots (LBDynamicCheckpointing,
    src/topOpt/LBCheckpointing.cc:37-65); nslots <= 0 stores every step (NaiveCheckpointing). */
 lbmx_status lbmx_checkpoint_config(lbmx_solver* s, int nslots);
+/* Placement of the nslots checkpoints: LBMX_CHECKPOINT_DYNAMIC (default) places them online
+   as the forward pass runs (LBDynamicCheckpointing, src/topOpt/LBCheckpointing.cc:37-65);
+   LBMX_CHECKPOINT_STATIC keeps the checkpoints the same level-based placement gives when it
+   plans the known horizon at once (LBStaticCheckpointing / StaticCheckpointing,
+   src/topOpt/staticCheckpointing.cc:3-55, used by applications/RobustPump2d/robustPump2d.cc:166).
+   The reverse pass and the results are the same; only the recompute count differs. */
+typedef enum { LBMX_CHECKPOINT_DYNAMIC = 0, LBMX_CHECKPOINT_STATIC = 1 } lbmx_checkpoint_mode_t;
+lbmx_status lbmx_checkpoint_mode(lbmx_solver* s, int mode);
 /* One design-gradient evaluation, the loop of applications/RobustPump2d/robustPump2d.cc:233-277:
    initializeAtEquilibrium(rho0,u0,T0), nsteps recorded forward steps, resetAdjoints, then
    nsteps reverse steps (adjointCollide, computeCollideSource, adjointStream) with checkpoint
@@ -286,6 +294,8 @@ lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, i
    buffers and verifies that every reverse step receives the right state. *advances = forward
    steps run in the reverse phase (recomputation), *max_slots = slots in use at most. */
 lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances, int* max_slots);
+/* The same for either placement (lbmx_checkpoint_mode). */
+lbmx_status lbmx_checkpoint_simulate_mode(int nslots, int nsteps, int mode, long long* advances, int* max_slots);
 
 /* ---- batched robust-ensemble form ----
    Several solver handles of one configuration (same lattice, operator, design coupling and
@@ -294,13 +304,15 @@ lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances
    share one checkpoint schedule. Replaces the MPISplit communicators over which the reference
    runs robust realisations and design candidates side by side (src/core/MPISplit.cc:150-234,
    applications/RobustPump2d/robustPump2d.cc:47,146). Built for D2Q9 incompressible BGK, MRT and
-   cascaded in the FP64 and FP32 forms. Members stay owned by their handles and must outlive
+   cascaded and D3Q19 incompressible BGK and MRT, in the FP64 and FP32 forms. Members stay owned by their handles and must outlive
    the batch; calls on members and on the batch must not overlap. */
 typedef struct lbmx_batch lbmx_batch;
 lbmx_status lbmx_batch_create(lbmx_solver* const* members, int count, lbmx_batch** out);
 lbmx_status lbmx_batch_destroy(lbmx_batch* b);
 /* checkpoint slots of the shared schedule (each slot holds every member's state) */
 lbmx_status lbmx_batch_checkpoint_config(lbmx_batch* b, int nslots);
+/* placement of the shared schedule's checkpoints (lbmx_checkpoint_mode) */
+lbmx_status lbmx_batch_checkpoint_mode(lbmx_batch* b, int mode);
 /* rows of the 64-wide workgroup tile of the batched sweeps: 1, 2 or 4 (default 4) */
 lbmx_status lbmx_batch_set_tile_rows(lbmx_batch* b, int rows);
 /* enable != 0: the first lbmx_batch_gradient captures its whole launch sequence into a HIP

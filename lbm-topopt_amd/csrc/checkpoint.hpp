@@ -1,6 +1,7 @@
 // Online level-based checkpoint placement with recomputation (Wang, Moin & Iaccarino 2009),
 // the schedule of the reference's DynamicCheckpointing + WangCheckpointing
-// (src/topOpt/dynamicCheckpointing.cc:9-54, src/topOpt/WangCheckpointing.cc:36-126).
+// (src/topOpt/dynamicCheckpointing.cc:9-54, src/topOpt/WangCheckpointing.cc:36-126), and its
+// StaticCheckpointing variant for a known horizon (src/topOpt/staticCheckpointing.cc:3-55).
 //
 // Storage is abstract: the schedule hands out slot ids 0..nslots-1 and the engine maps each
 // slot to an HBM buffer. Placing a state in a slot is done by exchanging buffer handles
@@ -45,11 +46,35 @@ class WangSchedule {
     free_.clear();
     for (int s = nslots_ - 1; s >= 0; --s) free_.push_back(s);
     have_reader_ = false;
+    next_planned_ = 0;
   }
 
+  // StaticCheckpointing: the forward pass keeps exactly the checkpoints that the level-based
+  // placement holds after planning the whole horizon 0..last at once (computeNewCheckpoints
+  // from the initial state), with their levels; the reverse pass is the same as for the
+  // dynamic schedule.
+  void plan_static(int last) {
+    reset();
+    list_.push_front({0, INT_MAX, take()});
+    place(0, last);
+    planned_.clear();
+    for (auto it = list_.rbegin(); it != list_.rend(); ++it) planned_.push_back({it->step, it->level, -1});
+    reset();
+    static_ = true;
+  }
+  bool is_static() const { return static_; }
+
   // DynamicCheckpointing::makeCheckpoint: state `index` always becomes the newest checkpoint.
-  // Returns its slot. The slot's previous content (if any) is dropped.
+  // Returns its slot. The slot's previous content (if any) is dropped. A static schedule keeps
+  // only its planned steps (StaticCheckpointing::makeCheckpoint) and returns -1 for the others.
   int make_checkpoint(int index) {
+    if (static_) {
+      if (next_planned_ >= planned_.size() || planned_[next_planned_].step != index) return -1;
+      int s = take();
+      list_.push_front({index, planned_[next_planned_].level, s});
+      ++next_planned_;
+      return s;
+    }
     if (list_.empty()) {
       int s = take();
       list_.push_front({index, INT_MAX, s});
@@ -156,6 +181,9 @@ class WangSchedule {
   }
 
   int nslots_;
+  bool static_ = false;
+  std::vector<Entry> planned_;  // static schedule: forward checkpoints, increasing step
+  size_t next_planned_ = 0;
   std::list<Entry> list_;  // newest first
   std::vector<int> free_;
   std::list<Entry>::iterator reader_;

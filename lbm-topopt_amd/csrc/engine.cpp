@@ -55,17 +55,22 @@ struct PassResult {
 // WangCheckpointing::getForwardSolution, src/topOpt/WangCheckpointing.cc:36-126). Storing a
 // checkpoint swaps buffer handles (no copy). On return, lin holds the final adjoint nu_0.
 static PassResult checkpointed_pass(SweepOps& ops, int nsteps, bool naive, std::vector<void*>& slotbuf,
-                                    void* work0, void* work1, void*& lin, void*& lout, bool with_obj) {
+                                    void* work0, void* work1, void*& lin, void*& lout, bool with_obj,
+                                    bool static_plan = false) {
   const int slots = (int)slotbuf.size();
   void* work[2] = {work0, work1};
   auto other_work = [&](const void* src) -> void*& { return src == work[0] ? work[1] : work[0]; };
   std::unique_ptr<WangSchedule> sched;
-  if (!naive) sched.reset(new WangSchedule(slots));
+  if (!naive) {
+    sched.reset(new WangSchedule(slots));
+    if (static_plan) sched->plan_static(nsteps - 1);
+  }
   PassResult r;
   auto used = [&]() { r.max_slots = std::max(r.max_slots, naive ? slots : slots - sched->free_slots()); };
   ops.init(work[0]);
   auto store = [&](int step, void*& holder) -> void* {
     int s = naive ? step : sched->make_checkpoint(step);
+    if (s < 0) return holder;       // a static schedule does not keep this step
     std::swap(slotbuf[s], holder);  // the slot now owns the state; holder gets a spare buffer
     return slotbuf[s];
   };
@@ -144,6 +149,7 @@ struct Engine {
   int state_kind = -1;   // -1 none, 0 p_{t-1} valid, 1 uniform equilibrium, 2 raw AoS in raw_aos
   double init_rho = 1, init_u[3] = {0, 0, 0}, init_T = 0;
   int nslots = 64;
+  bool static_ckpt = false;  // StaticCheckpointing placement instead of the dynamic one
   bool grad_valid = false;
   int adj_t = -1;        // step of the next per-step reverse sweep (-1: lbmx_adjoint_reset not called)
   lbmx_stats stats{};
@@ -756,7 +762,8 @@ struct Engine {
     auto tf0 = std::chrono::steady_clock::now();
     void* lin = lam_a.p;
     void* lout = lam_b.p;
-    PassResult pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, with_obj);
+    PassResult pr =
+        checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, with_obj, static_ckpt);
     HIP_OK(hipStreamSynchronize(stream));
     auto tr1 = std::chrono::steady_clock::now();
     const double Jv = with_obj ? reduce_acc() : 0.0;
@@ -955,6 +962,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   int ty = kTileY;         // rows of the workgroup tile
   int nslots = 64;
+  bool static_ckpt = false;
   int nblocks = 0;
   int tf_stride = 1;
   size_t state_total = 0, lam_total = 0;
@@ -1010,7 +1018,7 @@ struct Batch {
 
   // every address and size the captured launch sequence depends on
   std::vector<long long> graph_key(int nsteps, int slots, bool naive, double rho0, const double* u0, double T0) const {
-    std::vector<long long> k = {nsteps, slots, naive ? 1 : 0, ty, nblocks, tf_stride};
+    std::vector<long long> k = {nsteps, slots, naive ? 1 : 0, static_ckpt ? 1 : 0, ty, nblocks, tf_stride};
     auto bits = [&](double v) {
       long long b;
       std::memcpy(&b, &v, sizeof b);
@@ -1196,7 +1204,7 @@ struct Batch {
     void* lout = lam_b.p;
     PassResult pr;
     if (!use_graph) {
-      pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true);
+      pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true, static_ckpt);
     } else {
       // the launch sequence of a pass is fixed by (steps, slots, tile height) and the buffer
       // addresses: capture it once, replay it on later calls
@@ -1207,7 +1215,7 @@ struct Batch {
         HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         hipGraph_t graph = nullptr;
         try {
-          pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true);
+          pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true, static_ckpt);
         } catch (...) {
           (void)hipStreamEndCapture(stream, &graph);
           if (graph) (void)hipGraphDestroy(graph);
@@ -1554,7 +1562,25 @@ lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, i
 // Runs the gradient's checkpointed pass (the same code as lbmx_gradient) on labelled host
 // buffers: a buffer holds the step of the state it contains; every forward sweep must read
 // p_{t-1} and every adjoint sweep must receive p_{t-1}.
+lbmx_status lbmx_checkpoint_mode(lbmx_solver* s, int mode) {
+  CHECK_HANDLE(s);
+  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC) {
+    lbmx::g_last_error = "unknown checkpoint mode";
+    return LBMX_ERR_ARG;
+  }
+  s->e->static_ckpt = mode == LBMX_CHECKPOINT_STATIC;
+  return LBMX_OK;
+}
+
 lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances, int* max_slots) {
+  return lbmx_checkpoint_simulate_mode(nslots, nsteps, LBMX_CHECKPOINT_DYNAMIC, advances, max_slots);
+}
+
+lbmx_status lbmx_checkpoint_simulate_mode(int nslots, int nsteps, int mode, long long* advances, int* max_slots) {
+  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC) {
+    lbmx::g_last_error = "unknown checkpoint mode";
+    return LBMX_ERR_ARG;
+  }
   if (!advances || !max_slots || nsteps < 1 || nslots < 2) {
     lbmx::g_last_error = "bad argument";
     return LBMX_ERR_ARG;
@@ -1582,7 +1608,8 @@ lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances
     void* lin = &storage[slots + 2];
     void* lout = &storage[slots + 3];
     lbmx::PassResult r =
-        lbmx::checkpointed_pass(ops, nsteps, false, slotbuf, &storage[slots], &storage[slots + 1], lin, lout, false);
+        lbmx::checkpointed_pass(ops, nsteps, false, slotbuf, &storage[slots], &storage[slots + 1], lin, lout, false,
+                                mode == LBMX_CHECKPOINT_STATIC);
     *advances = r.recompute;
     *max_slots = r.max_slots;
   });
@@ -1620,6 +1647,16 @@ lbmx_status lbmx_batch_destroy(lbmx_batch* b) {
     lbmx::g_last_error = "null batch handle";                \
     return LBMX_ERR_ARG;                                     \
   }
+
+lbmx_status lbmx_batch_checkpoint_mode(lbmx_batch* b, int mode) {
+  if (!b || !b->b) { lbmx::g_last_error = "null batch handle"; return LBMX_ERR_ARG; }
+  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC) {
+    lbmx::g_last_error = "unknown checkpoint mode";
+    return LBMX_ERR_ARG;
+  }
+  b->b->static_ckpt = mode == LBMX_CHECKPOINT_STATIC;
+  return LBMX_OK;
+}
 
 lbmx_status lbmx_batch_checkpoint_config(lbmx_batch* b, int nslots) {
   CHECK_BATCH(b);

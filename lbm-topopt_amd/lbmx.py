@@ -26,6 +26,7 @@ TIME_CONSTANT, TIME_SINE = 0, 1
 OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX, OBJ_THROUGHFLOW = 0, 1, 2, 3, 4, 5
 
 LBMX_OK, ERR_ARG, ERR_SUP, ERR_MEM, ERR_STATE, ERR_DEVICE = 0, 62, 56, 55, 73, 76
+CHECKPOINT_DYNAMIC, CHECKPOINT_STATIC = 0, 1
 
 Q_OF = {D2Q9: 9, D3Q19: 19, D2Q9_D2Q5: 14}
 
@@ -175,6 +176,9 @@ def library():
         "lbmx_batch_device_bytes": [C.c_void_p, P(C.c_size_t)],
         "lbmx_describe_boundaries": [C.c_void_p, P(C.c_int), C.c_int, P(C.c_int)],
         "lbmx_checkpoint_simulate": [C.c_int, C.c_int, P(C.c_longlong), P(C.c_int)],
+        "lbmx_checkpoint_simulate_mode": [C.c_int, C.c_int, C.c_int, P(C.c_longlong), P(C.c_int)],
+        "lbmx_checkpoint_mode": [C.c_void_p, C.c_int],
+        "lbmx_batch_checkpoint_mode": [C.c_void_p, C.c_int],
         "lbmx_filter_create": [P(FilterDesc), P(C.c_void_p)],
         "lbmx_filter_destroy": [C.c_void_p],
         "lbmx_filter_set_projection": [C.c_void_p, C.c_int, C.c_double, C.c_double],
@@ -235,14 +239,15 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_set_design_device", "lbmx_get_sensitivity_device", "lbmx_filter_design_device",
             "lbmx_filter_design_to_solver_device", "lbmx_filter_sensitivities_device",
             "lbmx_filter_solver_sensitivities_device", "lbmx_mma_update_device", "lbmx_mma_design_change_device",
-            "lbmx_dmda_global_to_natural", "lbmx_dmda_natural_to_global"]
+            "lbmx_dmda_global_to_natural", "lbmx_dmda_natural_to_global", "lbmx_checkpoint_mode",
+            "lbmx_checkpoint_simulate_mode", "lbmx_batch_checkpoint_mode"]
 
 
-def checkpoint_simulate(nslots, nsteps):
+def checkpoint_simulate(nslots, nsteps, mode=CHECKPOINT_DYNAMIC):
     """Host replay of the gradient's checkpoint schedule: (recompute advances, max slots used)."""
     adv = C.c_longlong(0)
     mx = C.c_int(0)
-    _check(library().lbmx_checkpoint_simulate(nslots, nsteps, C.byref(adv), C.byref(mx)))
+    _check(library().lbmx_checkpoint_simulate_mode(nslots, nsteps, mode, C.byref(adv), C.byref(mx)))
     return adv.value, mx.value
 
 
@@ -414,8 +419,10 @@ class Solver:
         _check(self.lib.lbmx_get_timestep(self.h, C.byref(t)))
         return t.value
 
-    def checkpoint_config(self, nslots):
+    def checkpoint_config(self, nslots, mode=None):
         _check(self.lib.lbmx_checkpoint_config(self.h, nslots))
+        if mode is not None:
+            _check(self.lib.lbmx_checkpoint_mode(self.h, mode))
 
     def gradient(self, nsteps, rho0=1.0, u0=(0.0, 0.0, 0.0), T0=0.0, want_sens=True):
         u = np.asarray(u0, dtype=np.float64)
@@ -515,8 +522,10 @@ class Batch:
         except Exception:
             pass
 
-    def checkpoint_config(self, nslots):
+    def checkpoint_config(self, nslots, mode=None):
         _check(self.lib.lbmx_batch_checkpoint_config(self.h, nslots))
+        if mode is not None:
+            _check(self.lib.lbmx_batch_checkpoint_mode(self.h, mode))
 
     def set_tile_rows(self, rows):
         _check(self.lib.lbmx_batch_set_tile_rows(self.h, rows))

@@ -72,13 +72,14 @@ def test_checkpointed_gradient_equals_stored(lbmx_mod):
     s, _, _ = build_pair(lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_PBB, 40, 20, time_const=1.0 / 100)
     s.checkpoint_config(0)
     J0, g0 = s.gradient(100)
-    for slots in (2, 5, 17):
-        s.checkpoint_config(slots)
-        J1, g1 = s.gradient(100)
-        st = s.stats()
-        assert J1 == J0
-        assert np.array_equal(g1, g0), slots
-        assert st["recompute_sweeps"] > 0
+    for mode in (lbmx.CHECKPOINT_DYNAMIC, lbmx.CHECKPOINT_STATIC):
+        for slots in (2, 5, 17):
+            before = s.stats()["recompute_sweeps"]
+            s.checkpoint_config(slots, mode)
+            J1, g1 = s.gradient(100)
+            assert J1 == J0
+            assert np.array_equal(g1, g0), (mode, slots)
+            assert s.stats()["recompute_sweeps"] - before == lbmx.checkpoint_simulate(slots, 100, mode)[0]
 
 
 def test_thermal_forward_and_gradient(lbmx_mod):

@@ -213,6 +213,21 @@ def test_checkpoint_invariants_and_bound(slots):
     assert a2 <= a1
 
 
+@pytest.mark.parametrize("slots", [2, 3, 5, 16, 48, 64])
+def test_static_checkpointing_schedule(slots):
+    """LBStaticCheckpointing: the forward pass keeps the checkpoints of a plan over the known
+    horizon; the replay checks that every reverse step still gets the right state. Planning
+    the horizon at once with the level rule ends in the placement the online rule reaches after
+    the same steps, so the recompute count equals the dynamic schedule's."""
+    for n in (2, 9, 64, 500, 3000):
+        adv_s, mx_s = lbmx.checkpoint_simulate(slots, n, lbmx.CHECKPOINT_STATIC)
+        adv_d, _ = lbmx.checkpoint_simulate(slots, n, lbmx.CHECKPOINT_DYNAMIC)
+        assert mx_s <= slots
+        assert adv_s == adv_d, (slots, n)
+    with pytest.raises(lbmx.LbmxError):
+        lbmx.checkpoint_simulate(slots, 10, 7)                   # unknown mode
+
+
 def test_checkpoint_headline_schedules():
     """The checkpoint counts of the benchmark workloads stay within twice the offline optimum."""
     def revolve_recompute(l, s):
