@@ -712,10 +712,14 @@ class UnsteadyLBTopOpt {
   UnsteadyLBTopOpt(LBSolver& solver, AdjointLBSolver& adj, const LBMacroFunctional& obj, PetscInt numTimesteps,
                    PetscScalar timeConst, Filter* filter = nullptr)
       : solver_(solver), adj_(adj), obj_(obj), nt_(numTimesteps), tc_(timeConst), filter_(filter) {}
-  // numCheckpoints = -1 stores every state (ForwardSolutionStorage); otherwise Wang dynamic
-  // checkpointing with that many slots (LBDynamicCheckpointing, src/topOpt/LBCheckpointing.cc:37-40)
-  PetscErrorCode allocateMemoryForForwardSolution(PetscInt numCheckpoints = -1) {
-    return lbmx_checkpoint_config(solver_.handle(), numCheckpoints < 0 ? 0 : numCheckpoints);
+  // numCheckpoints = -1 stores every state (ForwardSolutionStorage); otherwise Wang
+  // checkpointing with that many slots, placed online (LBDynamicCheckpointing,
+  // src/topOpt/LBCheckpointing.cc:37-40) or, with staticPlacement, planned for the known
+  // horizon (LBStaticCheckpointing, src/topOpt/staticCheckpointing.cc:3-55)
+  PetscErrorCode allocateMemoryForForwardSolution(PetscInt numCheckpoints = -1, bool staticPlacement = false) {
+    PetscErrorCode e = lbmx_checkpoint_config(solver_.handle(), numCheckpoints < 0 ? 0 : numCheckpoints);
+    if (e) return e;
+    return lbmx_checkpoint_mode(solver_.handle(), staticPlacement ? LBMX_CHECKPOINT_STATIC : LBMX_CHECKPOINT_DYNAMIC);
   }
   // J only (UnsteadyLBTopOpt2d::computeObjective)
   PetscErrorCode computeObjective() {

@@ -60,7 +60,7 @@ int main() {
   const auto objective = PressureDrop2d(Box2d(0, 0, 1, ny - 2), Box2d(nx - 1, nx - 1, 1, ny - 2));
   const PetscScalar timeAveraging = 1.0 / numSteps;
   UnsteadyLBTopOpt topOpt(solver, adjSolver, objective, numSteps, timeAveraging, &filter);
-  CHK(topOpt.allocateMemoryForForwardSolution(16));
+  CHK(topOpt.allocateMemoryForForwardSolution(16, true));  // LBStaticCheckpointing, as robustPump2d.cc:166
 
   MMA mma(numDesign, 1, designDomain);
   CHK(mma.status);
