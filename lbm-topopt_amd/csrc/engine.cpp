@@ -982,8 +982,8 @@ struct Batch {
     const lbmx_desc& d0 = m[0]->d;
     for (size_t i = 0; i < m.size(); ++i) {
       const lbmx_desc& d = m[i]->d;
-      if (d.lattice != LBMX_D2Q9 && d.lattice != LBMX_D3Q19)
-        throw Error{LBMX_ERR_SUP, "batched sweeps are built for D2Q9 and D3Q19"};
+      if (d.lattice != LBMX_D2Q9 && d.lattice != LBMX_D3Q19 && d.lattice != LBMX_D2Q9_D2Q5)
+        throw Error{LBMX_ERR_SUP, "unknown lattice"};
       if (d.lattice != d0.lattice) throw Error{LBMX_ERR_ARG, "batch members must share the lattice"};
       if (m[i]->nside_count > 0) throw Error{LBMX_ERR_SUP, "batched sweeps do not take Zou-He Neumann pieces"};
       if (d.collision != d0.collision || d.design != d0.design || d.precision != d0.precision)
@@ -996,7 +996,7 @@ struct Batch {
     if (d0.design == LBMX_DESIGN_DARCY) throw Error{LBMX_ERR_SUP, "DarcyMRT2d has no reverse operator"};
     if (d0.collision != LBMX_BGK && d0.collision != LBMX_MRT && d0.collision != LBMX_CASCADED)
       throw Error{LBMX_ERR_SUP, "batched sweeps are built for incompressible BGK, MRT and cascaded"};
-    if (d0.lattice == LBMX_D3Q19 && d0.collision == LBMX_CASCADED)
+    if (d0.lattice != LBMX_D2Q9 && d0.collision == LBMX_CASCADED)
       throw Error{LBMX_ERR_SUP, "the cascaded operator is defined for D2Q9"};
     // a stream of its own (default priority): graph capture needs a non-null stream
     HIP_OK(hipStreamCreate(&stream));
@@ -1129,6 +1129,10 @@ struct Batch {
     if (d.lattice == LBMX_D3Q19) {
       if (d.precision == LBMX_FP64) return launch_batch_d3q19_f64(d.collision, d.design, ty, kind, st, stream);
       return launch_batch_d3q19_f32(d.collision, d.design, ty, kind, st, stream);
+    }
+    if (d.lattice == LBMX_D2Q9_D2Q5) {
+      if (d.precision == LBMX_FP64) return launch_batch_thermal_f64(d.collision, d.design, ty, kind, st, stream);
+      return launch_batch_thermal_f32(d.collision, d.design, ty, kind, st, stream);
     }
     if (d.precision == LBMX_FP64) return launch_batch_d2q9_f64(d.collision, d.design, ty, kind, st, stream);
     return launch_batch_d2q9_f32(d.collision, d.design, ty, kind, st, stream);

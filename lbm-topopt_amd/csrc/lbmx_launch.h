@@ -89,5 +89,7 @@ hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const Batc
 hipError_t launch_batch_d2q9_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 hipError_t launch_batch_d3q19_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 hipError_t launch_batch_d3q19_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+hipError_t launch_batch_thermal_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+hipError_t launch_batch_thermal_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
 
 }  // namespace lbmx
