@@ -150,7 +150,8 @@ struct Engine {
   double init_rho = 1, init_u[3] = {0, 0, 0}, init_T = 0;
   int nslots = 64;
   bool static_ckpt = false;  // StaticCheckpointing placement instead of the dynamic one
-  bool grad_valid = false;
+  bool sens_valid = false;  // sens_dev holds dJ/dgamma of the last gradient or reverse steps
+  bool adj_valid = false;   // lam_a holds the adjoint of the last gradient or reverse step
   int adj_t = -1;        // step of the next per-step reverse sweep (-1: lbmx_adjoint_reset not called)
   lbmx_stats stats{};
 
@@ -773,7 +774,7 @@ struct Engine {
     stats.forward_phase_ms = std::chrono::duration<double, std::milli>(ops.t_fwd_end - tf0).count();
     stats.reverse_phase_ms = std::chrono::duration<double, std::milli>(tr1 - ops.t_fwd_end).count();
     if (lin != lam_a.p) std::swap(lam_a.p, lam_b.p);  // final adjoint in lam_a
-    grad_valid = true;
+    sens_valid = adj_valid = true;
     adj_t = -1;
     state_kind = -1;  // the forward buffers were reused
     if (J) *J = Jv;
@@ -795,7 +796,7 @@ struct Engine {
     zero_async(sens_dev.p, (size_t)N * 8, stream);
     HIP_OK(hipStreamSynchronize(stream));
     adj_t = timestep;
-    grad_valid = true;
+    sens_valid = adj_valid = true;
   }
 
   // AdjointLBSolver::setCurrentTimestep (src/adjointLatticeBoltzmann/adjointLBSolver.cc:147-151)
@@ -831,13 +832,13 @@ struct Engine {
   }
 
   void get_sensitivity(double* s, size_t n) {
-    if (!grad_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
+    if (!sens_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
     if ((long long)n != N) throw Error{LBMX_ERR_ARG, "sensitivity length != nodes"};
     HIP_OK(hipMemcpy(s, sens_dev.p, n * 8, hipMemcpyDeviceToHost));
   }
 
   void get_adjoint(double* out, size_t n) {
-    if (!grad_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
+    if (!adj_valid) throw Error{LBMX_ERR_STATE, "no adjoint computed by this solver (a batch keeps its own)"};
     if ((long long)n != N * Q) throw Error{LBMX_ERR_ARG, "adjoint length != nodes * Q"};
     const size_t eb = prec_lam_bytes(d.precision);
     std::vector<char> h(lam_bytes());
@@ -1246,7 +1247,8 @@ struct Batch {
     for (size_t i = 0; i < m.size(); ++i) {
       if (J) J[i] = reduce_acc(i);
       if (sens && sens[i]) HIP_OK(hipMemcpy(sens[i], inst[i].sens, (size_t)m[i]->N * 8, hipMemcpyDeviceToHost));
-      m[i]->grad_valid = false;  // the members' own adjoint buffers were not used
+      m[i]->sens_valid = true;    // the batch accumulated into the member's own dJ/dgamma array
+      m[i]->adj_valid = false;    // but the adjoint lives in the batch's buffers
       m[i]->state_kind = -1;
       nodes += m[i]->N;
     }
@@ -1355,7 +1357,7 @@ void solver_set_design_device(lbmx_solver* s, const double* dev) { engine_of(s).
 
 const double* solver_sensitivity_device(lbmx_solver* s) {
   Engine& e = engine_of(s);
-  if (!e.grad_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
+  if (!e.sens_valid) throw Error{LBMX_ERR_STATE, "no gradient computed"};
   return static_cast<const double*>(e.sens_dev.p);
 }
 

@@ -196,3 +196,23 @@ def test_batch_thermal_equals_single_instances(lbmx_mod, coll, design, prec):
         assert abs(J[i] - Js) <= 1e-13 * abs(Js), (i, J[i], Js)
         assert np.array_equal(sens[i], gs), (i, float(np.max(np.abs(sens[i] - gs))))
     b.close()
+
+
+def test_batch_members_keep_their_sensitivities(lbmx_mod):
+    """After a batched gradient each member hands out its own dJ/dgamma (getSensitivities, and
+    the device array the filter's sensitivity map reads); its adjoint stays with the batch."""
+    members = build(lbmx.MRT, lbmx.DESIGN_PBB, lbmx.FP64)[:2]
+    b = lbmx.Batch(members)
+    b.checkpoint_config(5)
+    _, sens = b.gradient(STEPS)
+    for m, g in zip(members, sens):
+        assert np.array_equal(m.get_sensitivity(), g)
+        assert np.array_equal(m.sensitivity_device().cpu().numpy(), g)
+        with pytest.raises(lbmx.LbmxError) as e:
+            m.get_adjoint()
+        assert e.value.code == lbmx.ERR_STATE
+    nx, ny = members[0].desc.nx, members[0].desc.ny
+    box = lbmx.make_box(8, nx - 9, 1, ny - 2)
+    filt = lbmx.Filter(lbmx.make_filter_desc(nx, ny, 1, box, box, ((2, False, 1.0, 0.5),)))
+    assert np.array_equal(filt.solver_sensitivities(members[0]), filt.sensitivities(sens[0]))
+    b.close()
