@@ -305,7 +305,7 @@ lbmx_status lbmx_checkpoint_simulate_mode(int nslots, int nsteps, int mode, long
    runs robust realisations and design candidates side by side (src/core/MPISplit.cc:150-234,
    applications/RobustPump2d/robustPump2d.cc:47,146). Built for D2Q9 incompressible BGK, MRT and
    cascaded, D3Q19 incompressible BGK and MRT and the thermal D2Q9 + D2Q5 lattice (BGK or MRT
-   flow part), in the FP64 and FP32 forms. Members stay owned by their handles and must outlive
+   flow part), in every precision form. Members stay owned by their handles and must outlive
    the batch; calls on members and on the batch must not overlap. */
 typedef struct lbmx_batch lbmx_batch;
 lbmx_status lbmx_batch_create(lbmx_solver* const* members, int count, lbmx_batch** out);

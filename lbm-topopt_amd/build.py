@@ -24,7 +24,7 @@ COMMON = [
     "-Wno-unused-command-line-argument",
 ]
 UNITS = ([f"kernels_{lat}_{p}.hip" for lat in ("d3q19", "d2q9", "thermal") for p in ("f64", "f32", "f16", "bf16")]
-         + [f"kernels_batch_{lat}_{p}.hip" for lat in ("d2q9", "d3q19", "thermal") for p in ("f32", "f64")]
+         + [f"kernels_batch_{lat}_{p}.hip" for lat in ("d2q9", "d3q19", "thermal") for p in ("f32", "f64", "f16", "bf16")]
          + ["engine.cpp", "probe.hip", "design.cpp", "design_kernels.hip", "mma.cpp", "mma_kernels.hip", "io.cpp"])
 
 

@@ -951,6 +951,14 @@ struct Engine {
   }
 };
 
+#define LBMX_BATCH_PREC(LAT, CALL)                  \
+  switch (d.precision) {                             \
+    case LBMX_FP64: return CALL(LAT##_f64);          \
+    case LBMX_FP32: return CALL(LAT##_f32);          \
+    case LBMX_FP16_SHIFTED: return CALL(LAT##_f16);  \
+    default: return CALL(LAT##_bf16);                \
+  }
+
 // ---- batch: several lattices of one configuration advanced in lockstep ----
 // Every sweep of a design-gradient pass is one launch over all members (the realisations and
 // design candidates that the reference spreads over MPISplit communicators,
@@ -992,8 +1000,6 @@ struct Batch {
       for (size_t k = 0; k < i; ++k)
         if (m[k] == m[i]) throw Error{LBMX_ERR_ARG, "a solver appears twice in the batch"};
     }
-    if (d0.precision != LBMX_FP32 && d0.precision != LBMX_FP64)
-      throw Error{LBMX_ERR_SUP, "batched sweeps are built for the FP64 and FP32 forms"};
     if (d0.design == LBMX_DESIGN_DARCY) throw Error{LBMX_ERR_SUP, "DarcyMRT2d has no reverse operator"};
     if (d0.collision != LBMX_BGK && d0.collision != LBMX_MRT && d0.collision != LBMX_CASCADED)
       throw Error{LBMX_ERR_SUP, "batched sweeps are built for incompressible BGK, MRT and cascaded"};
@@ -1127,16 +1133,13 @@ struct Batch {
     st.t = t;
     st.with_obj = with_obj ? 1 : 0;
     const lbmx_desc& d = m[0]->d;
-    if (d.lattice == LBMX_D3Q19) {
-      if (d.precision == LBMX_FP64) return launch_batch_d3q19_f64(d.collision, d.design, ty, kind, st, stream);
-      return launch_batch_d3q19_f32(d.collision, d.design, ty, kind, st, stream);
+#define CALL(S) launch_batch_##S(d.collision, d.design, ty, kind, st, stream)
+    switch (d.lattice) {
+      case LBMX_D2Q9: LBMX_BATCH_PREC(d2q9, CALL)
+      case LBMX_D3Q19: LBMX_BATCH_PREC(d3q19, CALL)
+      default: LBMX_BATCH_PREC(thermal, CALL)
     }
-    if (d.lattice == LBMX_D2Q9_D2Q5) {
-      if (d.precision == LBMX_FP64) return launch_batch_thermal_f64(d.collision, d.design, ty, kind, st, stream);
-      return launch_batch_thermal_f32(d.collision, d.design, ty, kind, st, stream);
-    }
-    if (d.precision == LBMX_FP64) return launch_batch_d2q9_f64(d.collision, d.design, ty, kind, st, stream);
-    return launch_batch_d2q9_f32(d.collision, d.design, ty, kind, st, stream);
+#undef CALL
   }
 
   void init_all(void* dst, double rho0, const double* u0, double T0) {

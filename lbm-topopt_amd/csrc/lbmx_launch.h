@@ -85,11 +85,11 @@ hipError_t launch_mma_change(const double* xnew, const double* xo1, long long n,
                              hipStream_t s);
 
 // Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
-hipError_t launch_batch_d2q9_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
-hipError_t launch_batch_d2q9_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
-hipError_t launch_batch_d3q19_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
-hipError_t launch_batch_d3q19_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
-hipError_t launch_batch_thermal_f32(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
-hipError_t launch_batch_thermal_f64(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+#define LBMX_DECLARE_BATCH(SUFFIX) \
+  hipError_t launch_batch_##SUFFIX(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+LBMX_DECLARE_BATCH(d2q9_f64) LBMX_DECLARE_BATCH(d2q9_f32) LBMX_DECLARE_BATCH(d2q9_f16) LBMX_DECLARE_BATCH(d2q9_bf16)
+LBMX_DECLARE_BATCH(d3q19_f64) LBMX_DECLARE_BATCH(d3q19_f32) LBMX_DECLARE_BATCH(d3q19_f16) LBMX_DECLARE_BATCH(d3q19_bf16)
+LBMX_DECLARE_BATCH(thermal_f64) LBMX_DECLARE_BATCH(thermal_f32) LBMX_DECLARE_BATCH(thermal_f16) LBMX_DECLARE_BATCH(thermal_bf16)
+#undef LBMX_DECLARE_BATCH
 
 }  // namespace lbmx

@@ -216,3 +216,21 @@ def test_batch_members_keep_their_sensitivities(lbmx_mod):
     filt = lbmx.Filter(lbmx.make_filter_desc(nx, ny, 1, box, box, ((2, False, 1.0, 0.5),)))
     assert np.array_equal(filt.solver_sensitivities(members[0]), filt.sensitivities(sens[0]))
     b.close()
+
+
+@pytest.mark.parametrize("prec", [lbmx.FP16_SHIFTED, lbmx.BF16_SHIFTED], ids=["fp16", "bf16"])
+def test_batch_shifted_storage_forms(lbmx_mod, prec):
+    """The shifted-storage forms in a batch: the stochastic rounding is keyed by each member's
+    own node offsets and steps, so every member's dJ/dgamma is bitwise its single-instance one."""
+    members = build(lbmx.CASCADED, lbmx.DESIGN_PBB, prec)
+    members3 = [member3d(30, 14, 12, lbmx.MRT, lbmx.DESIGN_BRINKMAN, prec, 4000 + i, eta)
+                for i, eta in enumerate((0.4, 0.6))]
+    for group in (members, members3):
+        ref = singles(group, 6)
+        b = lbmx.Batch(group)
+        b.checkpoint_config(6)
+        J, sens = b.gradient(STEPS)
+        for i, (Js, gs) in enumerate(ref):
+            assert abs(J[i] - Js) <= 1e-12 * abs(Js), (i, J[i], Js)
+            assert np.array_equal(sens[i], gs), (i, float(np.max(np.abs(sens[i] - gs))))
+        b.close()
