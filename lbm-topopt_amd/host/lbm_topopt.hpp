@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -377,6 +378,14 @@ class LBSolver {
   }
   // LBSolver::setFieldFromDesignVec (src/latticeBoltzmann/LBSolver.cc:333-348)
   PetscErrorCode setFieldFromDesignVec(const std::vector<double>& g) { return lbmx_set_design(h_, g.data(), g.size()); }
+  // NewLBSolver2d/LBSolver3d::setUniformFieldValues (src/latticeBoltzmann/NewLBSolver2d.cc:43-65,
+  // LBSolver3d.cc:43): the material field takes values[0] on the box (clipped to the grid)
+  PetscErrorCode setUniformFieldValues(const Box2d& b, const PetscScalar* values) {
+    return setUniformFieldValuesRaw(b.raw(), values);
+  }
+  PetscErrorCode setUniformFieldValues(const Box3d& b, const PetscScalar* values) {
+    return setUniformFieldValuesRaw(b.raw(), values);
+  }
   // LBSolver::createDesignVec (LBSolver.cc:318-331): a copy of the material field
   PetscErrorCode createDesignVec(std::vector<double>* v) const {
     v->assign((size_t)nodes(), 1.0);
@@ -421,6 +430,16 @@ class LBSolver {
   PetscInt nz() const { return nz_; }
 
  protected:
+  PetscErrorCode setUniformFieldValuesRaw(const lbmx_box& b, const PetscScalar* values) {
+    std::vector<double> g((size_t)nodes());
+    PetscErrorCode e = lbmx_get_design(h_, g.data(), g.size());
+    if (e) return e;
+    for (int z = std::max(b.z0, 0); z <= std::min(b.z1, (int)nz_ - 1); ++z)
+      for (int y = std::max(b.y0, 0); y <= std::min(b.y1, (int)ny_ - 1); ++y)
+        for (int x = std::max(b.x0, 0); x <= std::min(b.x1, (int)nx_ - 1); ++x)
+          g[((size_t)z * ny_ + y) * nx_ + x] = values[0];
+    return lbmx_set_design(h_, g.data(), g.size());
+  }
   PetscErrorCode create(lbmx_desc d) {
     nx_ = d.nx, ny_ = d.ny, nz_ = d.nz;
     numDOF_ = d.lattice == LBMX_D3Q19 ? 19 : (d.lattice == LBMX_D2Q9_D2Q5 ? 14 : 9);
