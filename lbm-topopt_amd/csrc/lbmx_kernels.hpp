@@ -106,14 +106,22 @@ __device__ __forceinline__ bool on_boundary(int x, int y, int z, int nx, int ny,
   return b;
 }
 
-template <class L, class P>
+// Pull of the populations of one node. NT selects the planes read with non-temporal loads:
+// 0 none, 1 those without an x shift, 2 all. Each element of the source field is read once in
+// a sweep; only x-shifted rows share lines with the neighbouring workgroup. Measured on the
+// D3Q19 MRT sweeps: the forward gains with all planes non-temporal, the adjoint (whose λ
+// gathers compete for L2) with the unshifted planes only.
+template <class L, class P, int NT>
 __device__ __forceinline__ void pull_node(const void* src, const Layout& l, unsigned int off,
                                           typename P::T (&f)[L::Q]) {
   using S = typename P::S;
   const unsigned int ob = off * (unsigned int)sizeof(S);
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    f[q] = P::template dec<L, q>(ld<S>(pull_base<L, q, S>(src, l), ob));
+    if constexpr (NT == 2 || (NT == 1 && L::ex[q] == 0))
+      f[q] = P::template dec<L, q>(__builtin_nontemporal_load(reinterpret_cast<const S*>(pull_base<L, q, S>(src, l) + ob)));
+    else
+      f[q] = P::template dec<L, q>(ld<S>(pull_base<L, q, S>(src, l), ob));
   });
 }
 
@@ -363,7 +371,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
       const double* s = static_cast<const double*>(a.src);
       static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
     } else {
-      pull_node<L, P>(a.src, a.ls, off, f);
+      pull_node<L, P, 2>(a.src, a.ls, off, f);
       if (bnd) apply_bcs<L, P>(f, a.bcs, a.nbc, tfac, x, y, z, a.src, a.ls);
     }
     if (a.mode == 1) {
@@ -456,7 +464,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   if (a.fwd_aos) {
     static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(a.fwd_aos[(long long)n * L::Q + decltype(qc)::value]); });
   } else {
-    pull_node<L, P>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
+    pull_node<L, P, 1>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
     if (bnd) apply_bcs<L, P>(f, a.bcs, a.nbc, tfac, x, y, z, a.fwd_src, a.ls);
   }
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
