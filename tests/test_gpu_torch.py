@@ -50,3 +50,19 @@ def test_gpu_design_tensor_stays_on_device(lbmx_mod):
     assert Jd.item() == Jh.item()
     assert np.array_equal(gd.grad.cpu().numpy(), gh.grad.numpy())
     s.close()
+
+
+def test_engine_on_a_torch_stream(lbmx_mod):
+    """The engine's launches on a torch stream (one shared HIP runtime): same results."""
+    nx, ny, steps = 40, 18, 30
+    d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_PBB, omega=1.25)
+    s = lbmx.Solver(d)
+    workloads.channel2d(s, nx, ny, time_const=1.0 / steps)
+    s.set_design(workloads.design_field(nx, ny, 1, seed=3))
+    J0, g0 = s.gradient(steps)
+    st = torch.cuda.Stream()
+    s.set_stream(st.cuda_stream)
+    J1, g1 = s.gradient(steps)
+    assert J1 == J0 and np.array_equal(g1, g0)
+    s.set_stream(0)
+    s.close()
