@@ -897,9 +897,8 @@ struct Engine {
     if (kind == 1 && (!inc || d.design == LBMX_DESIGN_DARCY))
       throw Error{LBMX_ERR_SUP, "no adjoint sweep for this operator"};
     prepare();
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     float ms = 0;
     if (kind == 0) {
       const bool with_obj = obj_set && obj.nterm > 0;
@@ -937,8 +936,6 @@ struct Engine {
     }
     HIP_OK(hipEventSynchronize(e1));
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (kind == 0) stats.fwd_kernel_ms = ms; else stats.adj_kernel_ms = ms;
     return ms;
   }
@@ -1269,9 +1266,8 @@ struct Batch {
     const int cand[3] = {1, 2, 4};
     double best = 1e300;
     int best_ty = ty;
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     for (int c = 0; c < 3; ++c) {
       ty = cand[c];
       build();
@@ -1308,8 +1304,6 @@ struct Batch {
         best_ty = cand[c];
       }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     ty = best_ty;
     for (Engine* e : m) e->state_kind = -1;
     return ty;

@@ -52,6 +52,24 @@ struct DeviceBuffer {
   T* as() const { return static_cast<T*>(p); }
 };
 
+// A pair of timing events, destroyed on every exit path.
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EventPair() {
+    HIP_OK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      throw Error{LBMX_ERR_DEVICE, "hipEventCreate failed"};
+    }
+  }
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+  ~EventPair() {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+};
+
 // Runs f, mapping the exceptions of the engine to status codes and the last-error message.
 template <class F>
 inline lbmx_status guarded(F&& f) {
