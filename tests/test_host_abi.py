@@ -118,6 +118,43 @@ def test_error_paths_without_device():
         lbmx.checkpoint_simulate(1, 10)
 
 
+def test_host_side_checks_of_the_newer_entry_points():
+    """Argument and state checks that fail before any device work (no GPU needed)."""
+    lib = lbmx.library()
+    s = lbmx.Solver(lbmx.make_desc(12, 8, lattice=lbmx.D2Q9, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
+                                   alpha_max=0.1))
+    for call, code in ((lambda: s.adjoint_step(), lbmx.ERR_STATE),            # no lbmx_adjoint_reset
+                       (lambda: s.adjoint_set_timestep(3), lbmx.ERR_STATE),
+                       (lambda: s.get_sensitivity(), lbmx.ERR_STATE),         # no gradient yet
+                       (lambda: s.checkpoint_config(8, mode=5), lbmx.ERR_ARG)):
+        with pytest.raises(lbmx.LbmxError) as e:
+            call()
+        assert e.value.code == code
+    assert lib.lbmx_set_design_device(s.h, None, 96) == lbmx.ERR_ARG
+    assert lib.lbmx_get_sensitivity_device(s.h, None, None) == lbmx.ERR_ARG
+    assert lib.lbmx_mma_update_device(None, None, None, None, None, 0.1) == lbmx.ERR_ARG
+    assert lib.lbmx_filter_design_device(None, None, 0, None, 0) == lbmx.ERR_ARG
+    # Zou-He Neumann: east/west of a 2D incompressible grid with nx >= 4; the thermal one needs D2Q5
+    B = lbmx.make_box
+    s.add_boundary(B(11, 11, 1, 6), lbmx.make_bc(lbmx.BC_ZOUHE_NEUMANN))
+    with pytest.raises(lbmx.LbmxError) as e:
+        s.add_boundary(B(0, 11, 7, 7), lbmx.make_bc(lbmx.BC_ZOUHE_NEUMANN))  # north: not defined
+    assert e.value.code == lbmx.ERR_SUP
+    with pytest.raises(lbmx.LbmxError) as e:
+        s.add_boundary(B(0, 0, 1, 6), lbmx.make_bc(lbmx.BC_THERMAL_NEUMANN))
+    assert e.value.code == lbmx.ERR_SUP
+    narrow = lbmx.Solver(lbmx.make_desc(3, 8, lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_PBB))
+    with pytest.raises(lbmx.LbmxError) as e:
+        narrow.add_boundary(B(2, 2, 1, 6), lbmx.make_bc(lbmx.BC_ZOUHE_NEUMANN))  # the neighbours would be boundary nodes
+    assert e.value.code == lbmx.ERR_SUP
+    cube = lbmx.Solver(lbmx.make_desc(8, 6, 6, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_PBB))
+    with pytest.raises(lbmx.LbmxError) as e:
+        cube.add_boundary(B(7, 7, 1, 4, 1, 4), lbmx.make_bc(lbmx.BC_ZOUHE_NEUMANN))
+    assert e.value.code == lbmx.ERR_SUP
+    for h in (s, narrow, cube):
+        h.close()
+
+
 def _pieces_engine_vs_oracle(desc, setup, *dims):
     s = lbmx.Solver(desc)
     o = oracle.Oracle(desc)
