@@ -19,10 +19,15 @@
  *                            source moments (q rows: the full source) -- the polynomials as coded
  *   pull stream            : brute-force permutation on tiny grids
  *   boundary conditions    : prescribed u / rho / T recovered at the node; bounce-back
- *                            against the rule "incoming <- opposite, tangential swapped"
+ *                            against the rule "incoming <- opposite, tangential swapped";
+ *                            Zou-He Neumann (flow, D2Q5 temperature): u_x or T of the node equals
+ *                            (4 m_1 - m_2)/3 of the two nodes inwards, u_y = 0, only the unknowns
+ *                            change, whole-field dot test including the neighbour blocks
  *   reverse operators,
  *   adjoint BCs, stream    : dot-product tests against central differences of the forward
- *   functionals + gradient : central finite differences of J on random cells
+ *   functionals + gradient : closed forms on uniform states (pressure drop, outflow, through-
+ *                            flow, total-pressure flux); central finite differences of J on
+ *                            random cells, also with Neumann outlets
  *   cascaded collision     : conservation; shear moments relax with omega to rho u_a u_b for
  *                            any state; at u = 0 every moment reaches its equilibrium (rate 1);
  *                            the u-dependent terms of k0, k3, k4, k5 as coded: parity unpinned;
