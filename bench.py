@@ -365,7 +365,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but "
-                                   "not built (slab decomposition, see DESIGN.md)", "n_gpus": args.gpus}))
+                                   "not built (slab decomposition along the slowest axis, one face exchange per step)", "n_gpus": args.gpus}))
         sys.exit(2)
     import lbmx
     import workloads

@@ -59,7 +59,7 @@ def design_field(nx, ny, nz=1, seed=0, smooth=True):
 
 def channel2d(obj, nx, ny, u_max=0.05, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_DROP, time_const=1.0,
               modulation=None, outlet="pressure"):
-    """Config-0/1 channel: Zou-He parabolic velocity inlet (x=0, 1<=y<=ny-2), Zou-He pressure
+    """Channel (the 256 x 128 and 1024 x 512 workloads): Zou-He parabolic velocity inlet (x=0, 1<=y<=ny-2), Zou-He pressure
     outlet (x=nx-1) or, with outlet="neumann", the Zou-He Neumann outflow of
     applications/pump2d/pump2d.cc:82 (u_x extrapolated from the two nodes inwards), on-grid
     bounce-back walls at y=0 and y=ny-1 (corners included)."""
@@ -83,7 +83,7 @@ def channel2d(obj, nx, ny, u_max=0.05, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_
 
 
 def duct3d(obj, nx, ny, nz, u_in=0.04, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_DROP, time_const=1.0):
-    """Config-3 duct: Hecht velocity inlet on the west face, Hecht pressure outlet on the east
+    """Duct (the 384 x 192 x 192 workload): Hecht velocity inlet on the west face, Hecht pressure outlet on the east
     face, on-grid bounce-back on the four side walls (edges and corners included), as
     applications/Diffuser3d/diffuser3d.cc:60-99 arranges its walls."""
     B = lbmx.make_box
@@ -102,7 +102,7 @@ def duct3d(obj, nx, ny, nz, u_in=0.04, rho_out=1.0, objective=lbmx.OBJ_PRESSURE_
 
 def thermal_channel2d(obj, nx, ny, u_max=0.05, T_in=1.0, T_wall=0.0, time_const=1.0, thermal_outlet="adiabatic",
                       outlet="pressure"):
-    """Config-2 thermal channel: flow as channel2d; hot inlet T=1 (Dirichlet, west), cold
+    """Thermal channel (the 2048 x 1024 workload): flow as channel2d; hot inlet T=1 (Dirichlet, west), cold
     walls T=0 (Dirichlet, north/south), adiabatic outlet (bounce-back of the D2Q5 part) or,
     with thermal_outlet="neumann", the temperature extrapolated from the two nodes inwards
     (ThermalZouHeNeumann2d, as applications/Regenerator2d/regenerator2d.cc:429 uses it);
@@ -154,7 +154,7 @@ def ensemble_members(make_solver, grids=ENSEMBLE_GRIDS, etas=ENSEMBLE_ETAS, beta
     return out
 
 
-def config1_channel(obj, nx=1024, ny=512, steps=5000, period=1000.0, amp=0.5):
+def modulated_channel(obj, nx=1024, ny=512, steps=5000, period=1000.0, amp=0.5):
     """Unsteady channel: the inlet amplitude is modulated by 1 + amp sin(2 pi t / period)
     (the functor pattern of applications/RobustPump2d/robustPump2d.cc:14-19); objective: the
     time-integrated dissipated power (total-pressure flux in minus out), timeConst = 1/steps."""

@@ -35,7 +35,7 @@ CASES = {
                                       alpha_max=1.0),
     "d2q9_bgk_brinkman_channel": dict(lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_BRINKMAN, nx=32,
                                       ny=16, nz=1, setup="channel", steps=40, seed=0, omega=1.25, alpha_max=0.1),
-    # config-4 operator: PBB<IncompressibleCascaded2d, BP(q=0.1)> at tau = 0.6
+    # the ensemble operator: PBB<IncompressibleCascaded2d, BP(q=0.1)> at tau = 0.6
     "d2q9_cascaded_pbb_channel": dict(lattice=lbmx.D2Q9, collision=lbmx.CASCADED, design=lbmx.DESIGN_PBB, nx=32,
                                       ny=16, nz=1, setup="channel", steps=40, seed=2, omega=1.0 / 0.6,
                                       alpha_max=1.0),

@@ -1,5 +1,5 @@
 """Full-size checks through size-independent properties (the oracle is too slow at these
-sizes): the configuration-1 gradient against central finite differences on 16 seed-chosen
+sizes): the 1024 x 512 modulated-channel gradient against central finite differences on 16 seed-chosen
 cells, mass conservation and schedule invariance at the 384 x 192 x 192 D3Q19 size."""
 import numpy as np
 import pytest
@@ -10,23 +10,23 @@ import workloads
 pytestmark = pytest.mark.gpu
 
 
-def _config1(precision):
+def _channel_1024x512(precision):
     nx, ny = 1024, 512
     d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_BRINKMAN,
                        precision=precision, omega=1.0 / 0.8, interp_q=0.1, alpha_max=0.1)
     s = lbmx.Solver(d)
-    workloads.config1_channel(s, nx, ny, steps=5000)
+    workloads.modulated_channel(s, nx, ny, steps=5000)
     g = workloads.design_field(nx, ny, 1, seed=0)
     s.set_design(g)
     s.checkpoint_config(64)
     return s, g
 
 
-def test_config1_gradient_vs_central_differences(lbmx_mod):
+def test_channel_1024x512_gradient_vs_central_differences(lbmx_mod):
     """1024 x 512 D2Q9 BGK-Brinkman, sinusoidal inlet (period 1000), time-integrated dissipated
     power, 5000 steps, 64 HBM checkpoints: dJ/dgamma against central differences on 16 cells."""
     steps = 5000
-    s, g = _config1(lbmx.FP64)
+    s, g = _channel_1024x512(lbmx.FP64)
     J, sens = s.gradient(steps)
     rng = np.random.default_rng(0)
     cells = rng.choice(g.size, 16, replace=False)
@@ -45,23 +45,23 @@ def test_config1_gradient_vs_central_differences(lbmx_mod):
     fd = np.array(fd)
     scale = np.max(np.abs(sens))
     err = np.max(np.abs(fd - sens[cells])) / scale
-    print(f"config 1: J = {J:.12e}, max |dJ/dgamma| = {scale:.3e}, FD rel. error on 16 cells = {err:.2e}")
+    print(f"1024x512 channel: J = {J:.12e}, max |dJ/dgamma| = {scale:.3e}, FD rel. error on 16 cells = {err:.2e}")
     assert err < 1e-6
     st = s.stats()
     assert st["recompute_sweeps"] > 0
     s.close()
 
 
-def test_config1_fp16_form_gradient_error(lbmx_mod):
-    s64, _ = _config1(lbmx.FP64)
+def test_channel_1024x512_fp16_form_gradient_error(lbmx_mod):
+    s64, _ = _channel_1024x512(lbmx.FP64)
     J64, g64 = s64.gradient(5000)
     s64.close()
-    s16, _ = _config1(lbmx.FP16_SHIFTED)
+    s16, _ = _channel_1024x512(lbmx.FP16_SHIFTED)
     J16, g16 = s16.gradient(5000)
     s16.close()
     ej = abs(J16 - J64) / abs(J64)
     eg = np.max(np.abs(g16 - g64)) / np.max(np.abs(g64))
-    print(f"config 1 FP16-shifted form: J rel. error {ej:.2e}, gradient rel. max-norm error {eg:.2e}")
+    print(f"1024x512 channel, FP16-shifted form: J rel. error {ej:.2e}, gradient rel. max-norm error {eg:.2e}")
     assert ej < 1e-4 and eg < 2e-4  # measured 1.3e-5 and 9.7e-5
 
 
@@ -105,7 +105,7 @@ def test_fullsize_d3q19_checkpointed_equals_stored(lbmx_mod):
     s.close()
 
 
-def test_config3_fp16_form_meets_gradient_target(lbmx_mod):
+def test_duct_384x192x192_fp16_form_meets_gradient_target(lbmx_mod):
     """The FP16-shifted storage form (FP32 collide, stochastic rounding, FP64 objective and
     sensitivity) on the 384 x 192 x 192 D3Q19 MRT Brinkman duct, 3000 steps, 48 checkpoints:
     relative gradient error <= 1e-4 against the FP64 plain form (max-norm)."""
@@ -120,12 +120,12 @@ def test_config3_fp16_form_meets_gradient_target(lbmx_mod):
     ej = abs(J16 - J64) / abs(J64)
     eg = np.max(np.abs(g16 - g64)) / np.max(np.abs(g64))
     e2 = np.linalg.norm(g16 - g64) / np.linalg.norm(g64)
-    print(f"config 3 FP16 form: J rel. error {ej:.2e}, gradient rel. error max-norm {eg:.2e}, L2 {e2:.2e}")
+    print(f"384x192x192 duct, FP16 form: J rel. error {ej:.2e}, gradient rel. error max-norm {eg:.2e}, L2 {e2:.2e}")
     assert eg <= 1e-4 and ej <= 1e-4
 
 
-def test_config0_parity_full_size(lbmx_mod):
-    """Configuration 0 at its full size: D2Q9 BGK with Brinkman porosity (alpha_max = 0.1,
+def test_channel_256x128_parity_full_size(lbmx_mod):
+    """The 256 x 128 channel at its full size: D2Q9 BGK with Brinkman porosity (alpha_max = 0.1,
     q = 0.1), 256 x 128 channel, tau = 0.8, 2000 forward steps from rest, against the oracle."""
     import os
     import oracle
@@ -142,13 +142,13 @@ def test_config0_parity_full_size(lbmx_mod):
     Js = s.forward(steps); Jo = o.forward(steps)
     f = s.get_state()
     err = float(np.max(np.abs(f - o.f)) / np.max(np.abs(o.f)))
-    print(f"config 0: populations rel. max-norm error {err:.2e}, J {Js:.15e} vs {Jo:.15e}")
+    print(f"256x128 channel: populations rel. max-norm error {err:.2e}, J {Js:.15e} vs {Jo:.15e}")
     assert err < 1e-12 and abs(Js - Jo) <= 1e-10 * abs(Jo)
     s.close()
 
 
-def test_config2_thermal_forms_full_size(lbmx_mod):
-    """Configuration 2 at full size: D2Q9 MRT + D2Q5 MRT thermal, 2048 x 1024, 4000 steps,
+def test_thermal_channel_2048x1024_forms_full_size(lbmx_mod):
+    """The 2048 x 1024 thermal channel at full size: D2Q9 MRT + D2Q5 MRT thermal, 2048 x 1024, 4000 steps,
     k_s/k_f = 10 (ramp C_k = 0.1), hot inlet T = 1, cold walls T = 0, outlet heat flux; the
     BF16- and FP16-shifted storage forms against the FP64 form. The run starts from a moving,
     uniformly warm state (u = 0.03, T = 0.5): from rest, heat from the inlet does not reach
@@ -172,6 +172,6 @@ def test_config2_thermal_forms_full_size(lbmx_mod):
         ej = abs(J - J64) / abs(J64)
         eg = np.max(np.abs(g - g64)) / np.max(np.abs(g64))
         e2 = np.linalg.norm(g - g64) / np.linalg.norm(g64)
-        print(f"config 2 precision {prec}: J rel. error {ej:.2e}, gradient rel. error max-norm {eg:.2e}, L2 {e2:.2e}")
+        print(f"2048x1024 thermal channel, precision {prec}: J rel. error {ej:.2e}, gradient rel. error max-norm {eg:.2e}, L2 {e2:.2e}")
         assert ej <= tj and eg <= tg
 

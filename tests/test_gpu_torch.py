@@ -25,7 +25,7 @@ def test_autograd_matches_central_differences(lbmx_mod, coll, design):
     J.backward()
     grad = gamma.grad.numpy()
     rng = np.random.default_rng(4)
-    cells = rng.choice(nx * ny, 16, replace=False)  # 16 seed-chosen cells, as config 1
+    cells = rng.choice(nx * ny, 16, replace=False)  # 16 seed-chosen cells, as in the 1024x512 gradient check
     fd = lbmx_torch.central_difference(s, g0, steps, cells, h=1e-5)
     scale = np.max(np.abs(grad))
     assert np.max(np.abs(fd - grad[cells])) <= 1e-6 * scale, (fd, grad[cells])
