@@ -30,8 +30,9 @@
  *                            random cells, also with Neumann outlets
  *   cascaded collision     : conservation; shear moments relax with omega to rho u_a u_b for
  *                            any state; at u = 0 every moment reaches its equilibrium (rate 1);
- *                            the u-dependent terms of k0, k3, k4, k5 as coded: parity unpinned;
- *                            its reverse: dot-product test (true transpose)
+ *                            for any rho and u = j, the central moments of orders 2-4 relax to
+ *                            their equilibria (Geier, Greiner & Korvink 2006), which fixes every
+ *                            term of k0..k5; its reverse: dot-product test (true transpose)
  */
 #include "lbm_oracle.h"
 

@@ -5,6 +5,8 @@ this file from its defining polynomials, an invariant, brute force, or central f
 differences of the oracle's forward map -- never from a copy of the oracle's expressions.
 Reference constants that no published source fixes are cited at their file:line.
 """
+from math import comb
+
 import numpy as np
 import pytest
 
@@ -236,8 +238,7 @@ def test_cascaded_closed_forms():
     """IncompressibleCascaded2d: conserves rho and j; the shear moments relax with omega to
     rho u_a u_b for any state (u = j, incompressible macros); at u = 0 every higher moment
     reaches its equilibrium with rate 1 (trace 2 rho/3, third order 0, fourth order rho/9).
-    The u-dependent terms of k0, k3, k4, k5 are as coded in src/dynamics/cascaded2d.hh:44-74
-    and not fixed by these checks."""
+    The u-dependent terms of k0, k3, k4, k5 are fixed by the central-moment test below."""
     om = 1.0 / 0.6
     o = mk(lbmx.D2Q9, lbmx.CASCADED, omega=om)
     c, _, w = lat(o)
@@ -262,6 +263,50 @@ def test_cascaded_closed_forms():
         assert abs((cx * cy ** 2) @ out) < 1e-15 and abs((cx ** 2 * cy) @ out) < 1e-15
         assert abs((cx ** 2 * cy ** 2) @ out - rho / 9) < 1e-15
         assert abs((cx ** 2 - cy ** 2) @ out - (1 - om) * ((cx ** 2 - cy ** 2) @ f)) < 1e-15
+
+
+def _central_moment(g, c, rho, u, m, n):
+    """Central moment about u of order (m, n) by the binomial transform of the raw moments, with
+    the zeroth and first raw moments taken as rho and rho u: the consistency under which the
+    cascaded collision is derived (Geier, Greiner & Korvink, Phys. Rev. E 73, 066705 (2006))."""
+    cx, cy = c[:, 0], c[:, 1]
+    total = 0.0
+    for a in range(m + 1):
+        for b in range(n + 1):
+            raw = {(0, 0): rho, (1, 0): rho * u[0], (0, 1): rho * u[1]}.get((a, b))
+            if raw is None:
+                raw = (cx ** a * cy ** b) @ g
+            total += comb(m, a) * comb(n, b) * (-u[0]) ** (m - a) * (-u[1]) ** (n - b) * raw
+    return total
+
+
+@pytest.mark.parametrize("rho_scale", [0.93, 1.0, 1.07, 1.2])
+def test_cascaded_central_moment_relaxation(rho_scale):
+    """IncompressibleCascaded2d is Geier's cascaded collision with the incompressible macros
+    u = j: about that u, the post-collision central moments of orders 2-4 relax with the rates
+    the reference assigns (shear with omega, the rest with 1) to the equilibria rho/3 per
+    axis, 0 off-diagonal, 0 at third order and rho/9 at fourth. With rho and j conserved these
+    six conditions fix all nine outputs, so every term of k0..k5 is pinned, u-dependent ones
+    included, for any rho."""
+    om = 1.0 / 0.6
+    o = mk(lbmx.D2Q9, lbmx.CASCADED, omega=om)
+    c, _, w = lat(o)
+    for _ in range(4):
+        f = near_eq(c, w, amp=0.03)
+        f = f * (rho_scale / f.sum())
+        rho, u = f.sum(), (c.T @ f)[:2]
+        out, _ = o.collide_node(f, 1.0)
+
+        def k(g, m, n):
+            return _central_moment(g, c, rho, u, m, n)
+
+        assert abs(k(out, 2, 0) + k(out, 0, 2) - 2 * rho / 3) < 1e-15
+        assert abs(k(out, 2, 0) - k(out, 0, 2) - (1 - om) * (k(f, 2, 0) - k(f, 0, 2))) < 1e-15
+        assert abs(k(out, 1, 1) - (1 - om) * k(f, 1, 1)) < 1e-15
+        assert abs(k(out, 2, 1)) < 1e-15 and abs(k(out, 1, 2)) < 1e-15
+        assert abs(k(out, 2, 2) - rho / 9) < 1e-15
+        # the pre-collision state is not already relaxed, so the check has something to see
+        assert abs(k(f, 2, 1)) > 1e-6 and abs(k(f, 2, 2) - rho / 9) > 1e-6
 
 
 # ================================================================ design couplings
