@@ -8,7 +8,8 @@
  * How each part is pinned (tests/test_oracle_*.py), independently of this code:
  *   lattice constants      : isotropy sums (sum w = 1, sum w c = 0, sum w cc = I/3), opp
  *   equilibrium            : the 9/19 equilibrium moments of the second-order polynomial
- *   BGK                    : affine in omega; omega=1 gives the equilibrium moments
+ *   BGK                    : affine in omega; omega=1 gives the equilibrium moments; the
+ *                            compressible form against the textbook equilibrium at u = j/rho
  *   MRT 2D / 3D            : moment-space relaxation m' = m - S(m - m_eq) with the textbook
  *                            Lallemand-Luo (D2Q9) and d'Humieres 2002 (D3Q19) bases built in
  *                            the test from their polynomials; equal rates reduce to BGK

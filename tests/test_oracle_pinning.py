@@ -151,6 +151,27 @@ def test_bgk_closed_form(lattice):
     assert np.allclose(o.collide_node(fe, 1.0)[0], fe, atol=1e-17)
 
 
+@pytest.mark.parametrize("lattice", [lbmx.D2Q9, lbmx.D3Q19])
+def test_standard_bgk_closed_form(lattice):
+    """The compressible BGK: f' = f + omega (f_eq - f) with the textbook equilibrium
+    w rho (1 + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2) at the barycentric velocity u = j / rho."""
+    om = 1.37
+    o = mk(lattice, lbmx.BGK_STANDARD, omega=om, nz=2 if lattice == lbmx.D3Q19 else 1)
+    c, _, w = lat(o)
+    for scale in (0.9, 1.0, 1.15):
+        f = near_eq(c, w, amp=0.02) * scale
+        rho, j = f.sum(), c.T @ f
+        u = j / rho
+        cu = c @ u
+        fe = w * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u @ u))
+        out, mac = o.collide_node(f, 1.0)
+        assert np.allclose(out, fe + (1 - om) * (f - fe), rtol=0, atol=2e-15)
+        assert abs(out.sum() - rho) < 2e-15 and np.allclose(c.T @ out, j, atol=1e-16)
+        # differs from the incompressible operator once rho is away from 1
+        if scale != 1.0:
+            assert np.max(np.abs(out - mk(lattice, lbmx.BGK, omega=om, nz=o.desc.nz).collide_node(f, 1.0)[0])) > 1e-6
+
+
 # ================================================================ MRT: basis, rates, equilibria
 def _mrt_check(o, rows, rates, om, gamma=1.0):
     c, op, w, Qf = flow_dims(o)
