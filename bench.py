@@ -3,9 +3,10 @@
 
 Workload (the headline configuration): D3Q19 incompressible MRT with Brinkman design
 coupling, 384 x 192 x 192 duct (Hecht velocity inlet u=0.04, Hecht pressure outlet rho=1,
-bounce-back side walls), objective = pressure drop, T = 3000 time steps, dynamic (Wang)
-checkpointing with 48 HBM slots. Synthetic design field: SplitMix64 uniform, seed 0,
-5x5x5 box filter.
+bounce-back side walls), objective = pressure drop, T = 3000 time steps, 48 HBM checkpoint
+slots placed by binomial checkpointing for the known horizon (the fewest recomputations; the
+reference's Wang dynamic placement is timed beside it on the same solver and gives the same
+gradient bit for bit). Synthetic design field: SplitMix64 uniform, seed 0, 5x5x5 box filter.
 
 One bench step = one design-gradient evaluation: equilibrium init, T recorded forward
 sweeps, T reverse sweeps with checkpoint recomputation. MLUPS counts every grid node once
@@ -39,7 +40,7 @@ def bytes_per_lu(form, Q=19):
     return 2 * Q * sf + sg, Q * sf + 2 * Q * sl + sg + 2 * 8
 
 
-def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, sweeps):
+def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, sweeps, mode, compare=None):
     prec = {"fp64": lbmx.FP64, "fp16": lbmx.FP16_SHIFTED, "bf16": lbmx.BF16_SHIFTED, "fp32": lbmx.FP32}[form]
     d = lbmx.make_desc(nx, ny, nz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
                        precision=prec, omega=1.6, interp_q=0.1, alpha_max=0.1)
@@ -47,7 +48,7 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
     workloads.duct3d(s, nx, ny, nz, u_in=0.04, time_const=1.0 / horizon)
     gamma = workloads.design_field(nx, ny, nz, seed=0)
     s.set_design(gamma)
-    s.checkpoint_config(slots)
+    s.checkpoint_config(slots, mode)
     N = nx * ny * nz
     # kernel-only sweep timing (HIP events on the engine's stream)
     s.init_equilibrium()
@@ -75,12 +76,26 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
         rphase.append(st["reverse_phase_ms"] / 1e3)
         log(f"{form}: step {k} {dt:.2f} s (forward {fphase[-1]:.2f} s, reverse {rphase[-1]:.2f} s, "
             f"recompute sweeps {recompute[-1]})")
+    other = None
+    if compare is not None:
+        # the same gradient with another checkpoint placement: same states, same sweeps
+        name, cmode = compare
+        s.checkpoint_config(slots, cmode)
+        before = s.stats()["recompute_sweeps"]
+        t0 = time.perf_counter()
+        J2, sens2 = s.gradient(horizon)
+        dt = time.perf_counter() - t0
+        other = {"placement": name, "ms_per_step": 1e3 * dt, "mlups_fwd_adj": N * horizon / dt / 1e6,
+                 "recompute_sweeps_per_gradient": s.stats()["recompute_sweeps"] - before,
+                 "bitwise_equal": bool(J2 == J and np.array_equal(sens2, sens))}
+        log(f"{form}: {name} placement {dt:.2f} s, recompute sweeps {other['recompute_sweeps_per_gradient']}, "
+            f"bitwise equal {other['bitwise_equal']}")
     st = s.stats()
     dev_bytes = s.device_bytes()
     s.close()
     t_step = max(times)
     bf, ba = bytes_per_lu(form)
-    return {
+    res = {
         "ms_per_step": 1e3 * t_step,
         "mlups_fwd_adj": N * horizon / t_step / 1e6,
         "mlups_forward_phase": N * horizon / max(fphase) / 1e6,
@@ -101,9 +116,12 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
         "pass_gbs": ((horizon + recompute[-1]) * bf + horizon * ba) * N / t_step / 1e9,
         "device_gb": dev_bytes / 1e9,
     }
+    if other is not None:
+        res["other_placement"] = other
+    return res
 
 
-def run_ensemble(lbmx, workloads, steps, slots, warmup):
+def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement):
     """Robust ensemble (24 members: 3 projection realisations x 8 candidates on uneven grids,
     PBB<IncompressibleCascaded2d> at tau = 0.6, FP32): batched gradient vs the same 24
     gradients as sequential single-instance runs. Aggregate MLUPS = total nodes x steps / s."""
@@ -114,7 +132,7 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup):
     members = [s for s, _, _ in workloads.ensemble_members(mk, steps=steps)]
     nodes = sum(m.nodes for m in members)
     b = lbmx.Batch(members)
-    b.checkpoint_config(slots)
+    b.checkpoint_config(slots, mode)
     rows, tune_ms = b.autotune(10)
     log(f"ensemble: {len(members)} members, {nodes} nodes, autotuned tile rows {rows} "
         f"(ms per fwd+adj sweep for rows 1/2/4: {', '.join(f'{v:.3f}' for v in tune_ms)})")
@@ -134,7 +152,7 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup):
     b.set_graph(False)
     log(f"ensemble: batched gradient replayed from a HIP graph {tg:.2f} s")
     for m in members:
-        m.checkpoint_config(slots)
+        m.checkpoint_config(slots, mode)
     for _ in range(warmup):
         for m in members:
             m.gradient(steps, want_sens=False)
@@ -148,6 +166,7 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup):
     for m in members:
         m.close()
     return {"members": len(members), "nodes": nodes, "time_steps": steps, "checkpoints": slots,
+            "placement": placement,
             "tile_rows": rows, "autotune_ms_rows_1_2_4": [round(v, 4) for v in tune_ms],
             "batched_s": round(tb, 3), "batched_graph_s": round(tg, 3), "sequential_s": round(ts, 3),
             "batched_mlups": round(nodes * steps / tb / 1e6, 1),
@@ -353,6 +372,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--horizon", type=int, default=3000)
     ap.add_argument("--slots", type=int, default=48)
+    ap.add_argument("--placement", choices=["binomial", "dynamic", "static"], default="binomial",
+                    help="checkpoint placement of every gradient pass")
     ap.add_argument("--grid", type=str, default="384x192x192")
     ap.add_argument("--sweeps", type=int, default=50)
     ap.add_argument("--forms", type=str, default="fp64,fp16,fp32")
@@ -371,10 +392,16 @@ def main():
     import workloads
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
     forms = [f for f in args.forms.split(",") if f]
+    placements = {"binomial": lbmx.CHECKPOINT_BINOMIAL, "dynamic": lbmx.CHECKPOINT_DYNAMIC,
+                  "static": lbmx.CHECKPOINT_STATIC}
+    mode = placements[args.placement]
+    # the headline form also runs the reference's own (dynamic) placement once, or the binomial one
+    cmp_name = "dynamic" if args.placement != "dynamic" else "binomial"
+    head_form = "fp64" if "fp64" in forms else forms[0]
     res = {}
     for f in forms:
         res[f] = run_form(lbmx, workloads, f, nx, ny, nz, args.horizon, args.slots, args.steps, args.warmup,
-                          args.sweeps)
+                          args.sweeps, mode, (cmp_name, placements[cmp_name]) if f == head_form else None)
     # accuracy of the reduced-storage forms against the FP64 plain form on the same workload
     if "fp64" in res:
         g64, J64 = res["fp64"]["_sens"], res["fp64"]["objective"]
@@ -390,8 +417,7 @@ def main():
             log(f"{f} vs fp64: {res[f]['vs_fp64']}")
     for f in forms:
         res[f].pop("_sens", None)
-    head = res.get("fp64", res[forms[0]])
-    head_form = "fp64" if "fp64" in res else forms[0]
+    head = res[head_form]
     traffic = read_traffic(f"{head_form}_fwd_traffic_bytes")
     roof = {"bound": "hbm", "achieved": round(head["fwd_gbs"], 1), "peak": PEAK_GBS, "unit": "GB/s",
             "frac": round(head["fwd_gbs"] / PEAK_GBS, 4), "traffic": traffic,
@@ -414,8 +440,9 @@ def main():
         "dtype": "f64" if head_form == "fp64" else head_form,
         "data": "synthetic (SplitMix64 design field seed 0, 5x5x5 box filter)",
         "config": {"workload": f"D3Q19 MRT Brinkman duct {nx}x{ny}x{nz}, {args.horizon} time steps, "
-                               f"forward+adjoint with {args.slots} HBM checkpoints",
-                   "nodes": nx * ny * nz, "time_steps": args.horizon, "checkpoints": args.slots},
+                               f"forward+adjoint with {args.slots} HBM checkpoints ({args.placement} placement)",
+                   "nodes": nx * ny * nz, "time_steps": args.horizon, "checkpoints": args.slots,
+                   "placement": args.placement},
         "roofline": roof,
         "forms": {f: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for f, r in res.items()},
     }
@@ -424,7 +451,8 @@ def main():
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = round(res["fp32"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
     if args.ensemble_steps > 0:
-        out["ensemble"] = run_ensemble(lbmx, workloads, args.ensemble_steps, args.ensemble_slots, min(args.warmup, 1))
+        out["ensemble"] = run_ensemble(lbmx, workloads, args.ensemble_steps, args.ensemble_slots, min(args.warmup, 1),
+                                       mode, args.placement)
     if not args.no_design_pipeline:
         out["design_pipeline"] = run_design_pipeline(lbmx, cpu=not args.no_cpu_baseline)
     if not args.no_mma:

@@ -217,8 +217,16 @@ lbmx_status lbmx_checkpoint_config(lbmx_solver* s, int nslots);
    LBMX_CHECKPOINT_STATIC keeps the checkpoints the same level-based placement gives when it
    plans the known horizon at once (LBStaticCheckpointing / StaticCheckpointing,
    src/topOpt/staticCheckpointing.cc:3-55, used by applications/RobustPump2d/robustPump2d.cc:166).
+   LBMX_CHECKPOINT_BINOMIAL has no counterpart in the reference: binomial checkpointing for the
+   known horizon (Griewank & Walther, ACM TOMS 26 (2000) 19-45, "revolve"), the fewest forward
+   steps any placement of nslots checkpoints allows; for callers whose time to gradient is
+   dominated by recomputation.
    The reverse pass and the results are the same; only the recompute count differs. */
-typedef enum { LBMX_CHECKPOINT_DYNAMIC = 0, LBMX_CHECKPOINT_STATIC = 1 } lbmx_checkpoint_mode_t;
+typedef enum {
+  LBMX_CHECKPOINT_DYNAMIC = 0,
+  LBMX_CHECKPOINT_STATIC = 1,
+  LBMX_CHECKPOINT_BINOMIAL = 2
+} lbmx_checkpoint_mode_t;
 lbmx_status lbmx_checkpoint_mode(lbmx_solver* s, int mode);
 /* One design-gradient evaluation, the loop of applications/RobustPump2d/robustPump2d.cc:233-277:
    initializeAtEquilibrium(rho0,u0,T0), nsteps recorded forward steps, resetAdjoints, then
@@ -294,7 +302,7 @@ lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, i
    buffers and verifies that every reverse step receives the right state. *advances = forward
    steps run in the reverse phase (recomputation), *max_slots = slots in use at most. */
 lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances, int* max_slots);
-/* The same for either placement (lbmx_checkpoint_mode). */
+/* The same for any placement (lbmx_checkpoint_mode). */
 lbmx_status lbmx_checkpoint_simulate_mode(int nslots, int nsteps, int mode, long long* advances, int* max_slots);
 
 /* ---- batched robust-ensemble form ----

@@ -56,14 +56,15 @@ struct PassResult {
 // checkpoint swaps buffer handles (no copy). On return, lin holds the final adjoint nu_0.
 static PassResult checkpointed_pass(SweepOps& ops, int nsteps, bool naive, std::vector<void*>& slotbuf,
                                     void* work0, void* work1, void*& lin, void*& lout, bool with_obj,
-                                    bool static_plan = false) {
+                                    int mode = LBMX_CHECKPOINT_DYNAMIC) {
   const int slots = (int)slotbuf.size();
   void* work[2] = {work0, work1};
   auto other_work = [&](const void* src) -> void*& { return src == work[0] ? work[1] : work[0]; };
   std::unique_ptr<WangSchedule> sched;
   if (!naive) {
     sched.reset(new WangSchedule(slots));
-    if (static_plan) sched->plan_static(nsteps - 1);
+    if (mode == LBMX_CHECKPOINT_STATIC) sched->plan_static(nsteps - 1);
+    if (mode == LBMX_CHECKPOINT_BINOMIAL) sched->plan_binomial(nsteps);
   }
   PassResult r;
   auto used = [&]() { r.max_slots = std::max(r.max_slots, naive ? slots : slots - sched->free_slots()); };
@@ -76,8 +77,10 @@ static PassResult checkpointed_pass(SweepOps& ops, int nsteps, bool naive, std::
   };
   void* src = store(0, work[0]);
   used();
+  const void* last_input = nullptr;  // p_{nsteps-1}, the input of the last forward sweep
   for (int k = 1; k <= nsteps; ++k) {
     void*& dst = other_work(src);
+    if (k == nsteps) last_input = src;
     ops.fwd(src, dst, k, with_obj);
     if (k < nsteps) src = store(k, dst);
     else src = dst;
@@ -91,7 +94,8 @@ static PassResult checkpointed_pass(SweepOps& ops, int nsteps, bool naive, std::
     } else {
       WangSchedule::Plan plan = sched->get_forward_solution(index);
       if (plan.stored()) {
-        fsrc = slotbuf[plan.from_slot];
+        // a binomial plan may hand the first reverse step the forward pass's last input
+        fsrc = plan.from_slot >= 0 ? slotbuf[plan.from_slot] : last_input;
       } else {
         const void* s2 = slotbuf[plan.from_slot];
         size_t next = 0;
@@ -149,7 +153,7 @@ struct Engine {
   int state_kind = -1;   // -1 none, 0 p_{t-1} valid, 1 uniform equilibrium, 2 raw AoS in raw_aos
   double init_rho = 1, init_u[3] = {0, 0, 0}, init_T = 0;
   int nslots = 64;
-  bool static_ckpt = false;  // StaticCheckpointing placement instead of the dynamic one
+  int ckpt_mode = LBMX_CHECKPOINT_DYNAMIC;  // placement of the checkpoints (lbmx_checkpoint_mode_t)
   bool sens_valid = false;  // sens_dev holds dJ/dgamma of the last gradient or reverse steps
   bool adj_valid = false;   // lam_a holds the adjoint of the last gradient or reverse step
   int adj_t = -1;        // step of the next per-step reverse sweep (-1: lbmx_adjoint_reset not called)
@@ -764,7 +768,7 @@ struct Engine {
     void* lin = lam_a.p;
     void* lout = lam_b.p;
     PassResult pr =
-        checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, with_obj, static_ckpt);
+        checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, with_obj, ckpt_mode);
     HIP_OK(hipStreamSynchronize(stream));
     auto tr1 = std::chrono::steady_clock::now();
     const double Jv = with_obj ? reduce_acc() : 0.0;
@@ -968,7 +972,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   int ty = kTileY;         // rows of the workgroup tile
   int nslots = 64;
-  bool static_ckpt = false;
+  int ckpt_mode = LBMX_CHECKPOINT_DYNAMIC;
   int nblocks = 0;
   int tf_stride = 1;
   size_t state_total = 0, lam_total = 0;
@@ -1022,7 +1026,7 @@ struct Batch {
 
   // every address and size the captured launch sequence depends on
   std::vector<long long> graph_key(int nsteps, int slots, bool naive, double rho0, const double* u0, double T0) const {
-    std::vector<long long> k = {nsteps, slots, naive ? 1 : 0, static_ckpt ? 1 : 0, ty, nblocks, tf_stride};
+    std::vector<long long> k = {nsteps, slots, naive ? 1 : 0, ckpt_mode, ty, nblocks, tf_stride};
     auto bits = [&](double v) {
       long long b;
       std::memcpy(&b, &v, sizeof b);
@@ -1209,7 +1213,7 @@ struct Batch {
     void* lout = lam_b.p;
     PassResult pr;
     if (!use_graph) {
-      pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true, static_ckpt);
+      pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true, ckpt_mode);
     } else {
       // the launch sequence of a pass is fixed by (steps, slots, tile height) and the buffer
       // addresses: capture it once, replay it on later calls
@@ -1220,7 +1224,7 @@ struct Batch {
         HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         hipGraph_t graph = nullptr;
         try {
-          pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true, static_ckpt);
+          pr = checkpointed_pass(ops, nsteps, naive, slotbuf, pool[0]->p, pool[1]->p, lin, lout, true, ckpt_mode);
         } catch (...) {
           (void)hipStreamEndCapture(stream, &graph);
           if (graph) (void)hipGraphDestroy(graph);
@@ -1567,11 +1571,11 @@ lbmx_status lbmx_describe_boundaries(lbmx_solver* s, int* out, int max_pieces, i
 // p_{t-1} and every adjoint sweep must receive p_{t-1}.
 lbmx_status lbmx_checkpoint_mode(lbmx_solver* s, int mode) {
   CHECK_HANDLE(s);
-  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC) {
+  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC && mode != LBMX_CHECKPOINT_BINOMIAL) {
     lbmx::g_last_error = "unknown checkpoint mode";
     return LBMX_ERR_ARG;
   }
-  s->e->static_ckpt = mode == LBMX_CHECKPOINT_STATIC;
+  s->e->ckpt_mode = mode;
   return LBMX_OK;
 }
 
@@ -1580,7 +1584,7 @@ lbmx_status lbmx_checkpoint_simulate(int nslots, int nsteps, long long* advances
 }
 
 lbmx_status lbmx_checkpoint_simulate_mode(int nslots, int nsteps, int mode, long long* advances, int* max_slots) {
-  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC) {
+  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC && mode != LBMX_CHECKPOINT_BINOMIAL) {
     lbmx::g_last_error = "unknown checkpoint mode";
     return LBMX_ERR_ARG;
   }
@@ -1612,7 +1616,7 @@ lbmx_status lbmx_checkpoint_simulate_mode(int nslots, int nsteps, int mode, long
     void* lout = &storage[slots + 3];
     lbmx::PassResult r =
         lbmx::checkpointed_pass(ops, nsteps, false, slotbuf, &storage[slots], &storage[slots + 1], lin, lout, false,
-                                mode == LBMX_CHECKPOINT_STATIC);
+                                mode);
     *advances = r.recompute;
     *max_slots = r.max_slots;
   });
@@ -1653,11 +1657,11 @@ lbmx_status lbmx_batch_destroy(lbmx_batch* b) {
 
 lbmx_status lbmx_batch_checkpoint_mode(lbmx_batch* b, int mode) {
   if (!b || !b->b) { lbmx::g_last_error = "null batch handle"; return LBMX_ERR_ARG; }
-  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC) {
+  if (mode != LBMX_CHECKPOINT_DYNAMIC && mode != LBMX_CHECKPOINT_STATIC && mode != LBMX_CHECKPOINT_BINOMIAL) {
     lbmx::g_last_error = "unknown checkpoint mode";
     return LBMX_ERR_ARG;
   }
-  b->b->static_ckpt = mode == LBMX_CHECKPOINT_STATIC;
+  b->b->ckpt_mode = mode;
   return LBMX_OK;
 }
 

@@ -740,6 +740,13 @@ class UnsteadyLBTopOpt {
     if (e) return e;
     return lbmx_checkpoint_mode(solver_.handle(), staticPlacement ? LBMX_CHECKPOINT_STATIC : LBMX_CHECKPOINT_DYNAMIC);
   }
+  // Any placement of lbmx_checkpoint_mode_t, LBMX_CHECKPOINT_BINOMIAL included (binomial
+  // checkpointing for the known horizon, which the reference does not have).
+  PetscErrorCode allocateMemoryForForwardSolution(PetscInt numCheckpoints, lbmx_checkpoint_mode_t placement) {
+    PetscErrorCode e = lbmx_checkpoint_config(solver_.handle(), numCheckpoints < 0 ? 0 : numCheckpoints);
+    if (e) return e;
+    return lbmx_checkpoint_mode(solver_.handle(), placement);
+  }
   // J only (UnsteadyLBTopOpt2d::computeObjective)
   PetscErrorCode computeObjective() {
     const lbmx_objective o = obj_.objective(tc_);

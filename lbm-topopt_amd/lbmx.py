@@ -26,7 +26,7 @@ TIME_CONSTANT, TIME_SINE = 0, 1
 OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX, OBJ_THROUGHFLOW = 0, 1, 2, 3, 4, 5
 
 LBMX_OK, ERR_ARG, ERR_SUP, ERR_MEM, ERR_STATE, ERR_DEVICE = 0, 62, 56, 55, 73, 76
-CHECKPOINT_DYNAMIC, CHECKPOINT_STATIC = 0, 1
+CHECKPOINT_DYNAMIC, CHECKPOINT_STATIC, CHECKPOINT_BINOMIAL = 0, 1, 2
 
 Q_OF = {D2Q9: 9, D3Q19: 19, D2Q9_D2Q5: 14}
 

@@ -7,7 +7,8 @@
 // state on the host) and one written here that keeps every fourth state and re-advances the
 // solver between them. Both must give the objective, dJ/dgamma and the adjoint populations of
 // the fused pass (UnsteadyLBTopOpt::forwardAndAdjoint) bit for bit: the same sweeps run on the
-// same states. Prints the comparison lines and RESULT OK / RESULT FAIL.
+// same states. So must the fused pass with the binomial checkpoint placement. Prints the
+// comparison lines and RESULT OK / RESULT FAIL.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -123,11 +124,20 @@ int main() {
   double sensScale = 0.0;
   for (double v : sensFused) sensScale = std::max(sensScale, std::fabs(v));
 
+  // the same fused pass with the binomial placement of the 8 checkpoints: same states, same sweeps
+  CHK(topOpt.allocateMemoryForForwardSolution(8, LBMX_CHECKPOINT_BINOMIAL));
+  CHK(topOpt.forwardAndAdjoint());
+  std::vector<double> sensBinomial;
+  CHK(adjSolver.getSensitivities(&sensBinomial));
+  const bool binomial_ok = topOpt.getObjective() == Jfused && max_abs_diff(sensBinomial, sensFused) == 0.0;
+  std::printf("binomial placement bitwise equal %d\n", binomial_ok ? 1 : 0);
+  CHK(topOpt.allocateMemoryForForwardSolution(8));
+
   LBNaiveCheckpointing naive(solver);
   StridedRecomputeCheckpointing strided(solver, 4);
   CheckpointingBase* schemes[2] = {&naive, &strided};
   const char* names[2] = {"host storage of every state", "every 4th state, recompute"};
-  bool ok = sensScale > 0.0 && blocks_ok;
+  bool ok = sensScale > 0.0 && blocks_ok && binomial_ok;
   std::printf("setUniformFieldValues blocks placed %d\n", blocks_ok ? 1 : 0);
   std::vector<double> f;
   for (int k = 0; k < 2; ++k) {

@@ -125,6 +125,27 @@ def test_batch_graph_replays_match_eager(lbmx_mod):
     b.close()
 
 
+def test_batch_checkpoint_placements_agree(lbmx_mod):
+    """The static and binomial placements of the shared schedule give the dynamic schedule's
+    results bit for bit, eager and replayed from a captured graph, with fewer or equal
+    recomputations."""
+    members = build(lbmx.CASCADED, lbmx.DESIGN_PBB, lbmx.FP32)
+    b = lbmx.Batch(members)
+    b.checkpoint_config(5, lbmx.CHECKPOINT_DYNAMIC)
+    J0, s0 = b.gradient(STEPS)
+    for mode in (lbmx.CHECKPOINT_STATIC, lbmx.CHECKPOINT_BINOMIAL):
+        b.checkpoint_config(5, mode)
+        for graph in (False, True, True):
+            b.set_graph(graph)
+            J, sens = b.gradient(STEPS)
+            assert np.array_equal(J, J0), (mode, graph)
+            for i in range(len(members)):
+                assert np.array_equal(sens[i], s0[i]), (mode, graph, i)
+        b.set_graph(False)
+    assert lbmx.checkpoint_simulate(5, STEPS, lbmx.CHECKPOINT_BINOMIAL)[0] <= lbmx.checkpoint_simulate(5, STEPS)[0]
+    b.close()
+
+
 def member3d(nx, ny, nz, coll, design, prec, seed, eta):
     d = lbmx.make_desc(nx, ny, nz, lattice=lbmx.D3Q19, collision=coll, design=design, precision=prec, omega=1.6,
                        interp_q=0.1, alpha_max=1.0 if design == lbmx.DESIGN_PBB else 0.1)

@@ -72,7 +72,7 @@ def test_checkpointed_gradient_equals_stored(lbmx_mod):
     s, _, _ = build_pair(lbmx.D2Q9, lbmx.MRT, lbmx.DESIGN_PBB, 40, 20, time_const=1.0 / 100)
     s.checkpoint_config(0)
     J0, g0 = s.gradient(100)
-    for mode in (lbmx.CHECKPOINT_DYNAMIC, lbmx.CHECKPOINT_STATIC):
+    for mode in (lbmx.CHECKPOINT_DYNAMIC, lbmx.CHECKPOINT_STATIC, lbmx.CHECKPOINT_BINOMIAL):
         for slots in (2, 5, 17):
             before = s.stats()["recompute_sweeps"]
             s.checkpoint_config(slots, mode)

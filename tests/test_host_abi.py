@@ -265,6 +265,39 @@ def test_static_checkpointing_schedule(slots):
         lbmx.checkpoint_simulate(slots, 10, 7)                   # unknown mode
 
 
+def _fewest_recomputes(n_max, free_max):
+    """Fewest reverse-phase forward steps over every checkpoint placement, by exhaustive search
+    over the first split: with state 0 held and s free slots, reversing m states costs
+    R(m, s) = min_k [k + R(m-k, s-1) + R(k, s)], R(1, s) = 0, R(m, 0) = m(m-1)/2. The forward
+    pass itself supplies the first descent's n-1 steps."""
+    R = [[m * (m - 1) // 2 for m in range(n_max + 1)]]
+    for s in range(1, free_max + 1):
+        row = [0, 0]
+        for m in range(2, n_max + 1):
+            row.append(min(k + R[s - 1][m - k] + row[k] for k in range(1, m)))
+        R.append(row)
+    return lambda n, slots: R[min(slots, max(2, n)) - 1][n] - (n - 1) if n > 1 else 0
+
+
+def test_binomial_checkpointing_is_optimal():
+    """The binomial placement delivers every state (the replay checks each one) with the
+    fewest forward steps any placement allows, never more than the level-based schedules."""
+    best = _fewest_recomputes(120, 7)
+    for slots in (2, 3, 4, 5, 8):
+        for n in list(range(1, 41)) + [77, 120]:
+            adv, mx = lbmx.checkpoint_simulate(slots, n, lbmx.CHECKPOINT_BINOMIAL)
+            assert mx <= slots
+            assert adv == best(n, slots), (slots, n)
+            assert adv <= lbmx.checkpoint_simulate(slots, n)[0]
+    # the benchmark workloads; 4726 from the same exhaustive search run to n = 3000
+    assert lbmx.checkpoint_simulate(48, 3000, lbmx.CHECKPOINT_BINOMIAL) == (4726, 48)
+    assert lbmx.checkpoint_simulate(48, 3000, lbmx.CHECKPOINT_DYNAMIC)[0] == 5161
+    for slots, n in ((64, 3000), (64, 5000), (64, 4000)):
+        adv_b, mx_b = lbmx.checkpoint_simulate(slots, n, lbmx.CHECKPOINT_BINOMIAL)
+        assert mx_b <= slots and adv_b < lbmx.checkpoint_simulate(slots, n)[0]
+    assert lbmx.checkpoint_simulate(30, 30, lbmx.CHECKPOINT_BINOMIAL)[0] == 0
+
+
 def test_checkpoint_headline_schedules():
     """The checkpoint counts of the benchmark workloads stay within twice the offline optimum."""
     def revolve_recompute(l, s):
