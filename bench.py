@@ -374,6 +374,8 @@ def main():
     ap.add_argument("--slots", type=int, default=48)
     ap.add_argument("--placement", choices=["binomial", "dynamic", "static"], default="binomial",
                     help="checkpoint placement of every gradient pass")
+    ap.add_argument("--no-placement-compare", action="store_true",
+                    help="skip the extra gradient of the headline form with the other placement")
     ap.add_argument("--grid", type=str, default="384x192x192")
     ap.add_argument("--sweeps", type=int, default=50)
     ap.add_argument("--forms", type=str, default="fp64,fp16,fp32")
@@ -401,7 +403,8 @@ def main():
     res = {}
     for f in forms:
         res[f] = run_form(lbmx, workloads, f, nx, ny, nz, args.horizon, args.slots, args.steps, args.warmup,
-                          args.sweeps, mode, (cmp_name, placements[cmp_name]) if f == head_form else None)
+                          args.sweeps, mode,
+                          (cmp_name, placements[cmp_name]) if f == head_form and not args.no_placement_compare else None)
     # accuracy of the reduced-storage forms against the FP64 plain form on the same workload
     if "fp64" in res:
         g64, J64 = res["fp64"]["_sens"], res["fp64"]["objective"]
