@@ -59,8 +59,6 @@ def _fused(s, steps, thermal):
 @pytest.mark.parametrize("name,kw,steps", CASES, ids=[c[0] for c in CASES])
 def test_stepwise_reverse_loop_equals_fused_gradient(lbmx_mod, name, kw, steps, precision):
     thermal = kw["lattice"] == lbmx.D2Q9_D2Q5
-    if thermal and precision != lbmx.FP64:
-        pytest.skip("the thermal case runs in the FP64 form only here")
     s, o, _ = _pair(kw, steps, precision)
     J, g, adj = _fused(s, steps, thermal)
 
