@@ -196,9 +196,16 @@ template <> struct BasisOf<9> { static constexpr const Basis<9>& B = kB9; };
 template <> struct BasisOf<19> { static constexpr const Basis<19>& B = kB19; };
 
 // ---------------------------------------------------------------------------------------
-// Storage codecs. Plain forms store f; shifted forms store g = f/w_q - 1 in 16 bits so
-// that near-equilibrium values use the full mantissa, decoded as f = w_q + w_q g.
+// Storage codecs. Plain forms store f; shifted forms store g = f/w_q - c_q in 16 bits, decoded
+// as f = w_q g + w_q c_q. Flow directions shift by the rest density, c_q = 1, so that
+// near-equilibrium values use the full mantissa. The D2Q5 temperature directions carry
+// f = w_q T with T in the normalised range [0, 1]; they shift by its middle, c_q = 1/2, which
+// halves the largest |g|, and with it the rounding step, against a shift of 1.
 // ---------------------------------------------------------------------------------------
+template <class L, int q>
+__host__ __device__ constexpr double store_shift() {
+  return q < L::QF ? 1.0 : 0.5;
+}
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
   return __uint_as_float(static_cast<unsigned int>(h) << 16);
 }
@@ -252,11 +259,13 @@ struct PrecF16 {
   static constexpr bool shifted = true;
   template <class L, int q> __device__ static T dec(S v) {
     constexpr float w = static_cast<float>(L::w[q]);
-    return __builtin_fmaf(w, static_cast<float>(v), w);
+    constexpr float wc = static_cast<float>(L::w[q] * store_shift<L, q>());
+    return __builtin_fmaf(w, static_cast<float>(v), wc);
   }
   template <class L, int q> __device__ static S enc(T v) {
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
-    return static_cast<_Float16>(__builtin_fmaf(v, iw, -1.0f));
+    constexpr float c = static_cast<float>(store_shift<L, q>());
+    return static_cast<_Float16>(__builtin_fmaf(v, iw, -c));
   }
   static constexpr bool stochastic = true;
   // FP16 keeps 10 of the 23 fraction bits: 13 random bits added to the FP32 pattern, then a
@@ -264,7 +273,8 @@ struct PrecF16 {
   // rounding for FP16-normal values; below 2^-14 the conversion truncates to the subnormal grid.
   template <class L, int q> __device__ static float sr_ready(T v, unsigned int rnd) {
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
-    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -1.0f)) + (rnd & 0x1fffu));
+    constexpr float c = static_cast<float>(store_shift<L, q>());
+    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -c)) + (rnd & 0x1fffu));
   }
   __device__ static void pack2(float a, float b, S& ea, S& eb) {
     // the packed pair read as one 32-bit word (element-wise access of the half vector has
@@ -279,17 +289,20 @@ struct PrecBF16 {
   static constexpr bool shifted = true;
   template <class L, int q> __device__ static T dec(S v) {
     constexpr float w = static_cast<float>(L::w[q]);
-    return __builtin_fmaf(w, bf16_to_f32(v), w);
+    constexpr float wc = static_cast<float>(L::w[q] * store_shift<L, q>());
+    return __builtin_fmaf(w, bf16_to_f32(v), wc);
   }
   template <class L, int q> __device__ static S enc(T v) {
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
-    return f32_to_bf16(__builtin_fmaf(v, iw, -1.0f));
+    constexpr float c = static_cast<float>(store_shift<L, q>());
+    return f32_to_bf16(__builtin_fmaf(v, iw, -c));
   }
   static constexpr bool stochastic = true;
   // BF16 keeps the top 16 bits of the FP32 pattern: 16 random bits added, then truncated
   template <class L, int q> __device__ static float sr_ready(T v, unsigned int rnd) {
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
-    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -1.0f)) + (rnd & 0xffffu));
+    constexpr float c = static_cast<float>(store_shift<L, q>());
+    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -c)) + (rnd & 0xffffu));
   }
   __device__ static void pack2(float a, float b, S& ea, S& eb) {
     ea = static_cast<unsigned short>(__float_as_uint(a) >> 16);

@@ -172,9 +172,10 @@ __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned 
       constexpr int q = decltype(qc)::value;
       if constexpr (L::ex[q] != 0 || L::ey[q] != 0 || L::ez[q] != 0) {
         // p[q] of this node is what the pull of plane opp(q) at this node reads from x + c_q;
-        // opposite directions share a weight, so the encoding (recomputed: the rounding bits
-        // are deterministic) is the stored one
-        static_assert(L::w[q] == L::w[L::opp[q]], "opposite directions must share a weight");
+        // opposite directions share a weight and a shift, so the encoding (recomputed: the
+        // rounding bits are deterministic) is the stored one
+        static_assert(L::w[q] == L::w[L::opp[q]] && store_shift<L, q>() == store_shift<L, L::opp[q]>(),
+                      "opposite directions must share a weight and a storage shift");
         if (leaves<L, q>(x, y, z, nx, ny, nz)) st<S>(pull_base_w<L, L::opp[q], S>(dst, l), ob, enc1(qc));
       }
     });

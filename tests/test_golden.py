@@ -30,6 +30,8 @@ NAMES = [os.path.splitext(os.path.basename(f))[0] for f in FILES]
 # gradient <= 8.4e-4. The 1e-4 gradient target of the FP16 form is asserted on the
 # 384 x 192 x 192, 3000-step workload itself (tests/test_gpu_fullsize.py).
 FORM_TOL = {lbmx.FP32: (1e-5, 5e-6), lbmx.FP16_SHIFTED: (2e-4, 3e-4), lbmx.BF16_SHIFTED: (1e-3, 2e-3)}
+# the temperature populations sit further from their storage shift than the flow ones
+FORM_TOL_THERMAL = {lbmx.FP32: (1e-5, 5e-6), lbmx.FP16_SHIFTED: (2e-4, 4e-4), lbmx.BF16_SHIFTED: (1e-3, 4e-3)}
 
 
 def load(name):
@@ -72,7 +74,8 @@ def test_engine_reproduces_golden_fp64(lbmx_mod, name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", [lbmx.FP32, lbmx.FP16_SHIFTED, lbmx.BF16_SHIFTED])
-@pytest.mark.parametrize("name", ["d2q9_bgk_pbb_channel", "d3q19_mrt_brinkman_duct", "d2q9_bgk_brinkman_channel"])
+@pytest.mark.parametrize("name", ["d2q9_bgk_pbb_channel", "d3q19_mrt_brinkman_duct", "d2q9_bgk_brinkman_channel",
+                                  "d2q9_d2q5_mrt_brinkman_thermal"])
 def test_engine_reduced_storage_forms(lbmx_mod, name, prec):
     c, J, sens, _ = load(name)
     s = make_golden.make_problem(lbmx.Solver, c, precision=prec)
@@ -80,5 +83,5 @@ def test_engine_reduced_storage_forms(lbmx_mod, name, prec):
     s.close()
     ej, es = abs(Jg - J) / abs(J), relmax(sg, sens)
     print(f"{name} precision {prec}: J rel err {ej:.2e}, sensitivity rel err {es:.2e}")
-    tj, ts = FORM_TOL[prec]
+    tj, ts = (FORM_TOL_THERMAL if "thermal" in name else FORM_TOL)[prec]
     assert ej <= tj and es <= ts
