@@ -198,7 +198,9 @@ def member_thermal(nx, ny, coll, design, prec, seed, eta):
 
 
 @pytest.mark.parametrize("coll,design,prec", [(lbmx.MRT, lbmx.DESIGN_PBB, lbmx.FP64),
-                                              (lbmx.BGK, lbmx.DESIGN_BRINKMAN, lbmx.FP32)])
+                                              (lbmx.BGK, lbmx.DESIGN_BRINKMAN, lbmx.FP32),
+                                              (lbmx.MRT, lbmx.DESIGN_BRINKMAN, lbmx.FP16_SHIFTED),
+                                              (lbmx.MRT, lbmx.DESIGN_PBB, lbmx.BF16_SHIFTED)])
 def test_batch_thermal_equals_single_instances(lbmx_mod, coll, design, prec):
     """Thermal realisations (D2Q9 + D2Q5, heat-flux objective) in one batch: each member's
     dJ/dgamma bitwise its single-instance gradient from a moving, warm initial state."""
@@ -214,7 +216,7 @@ def test_batch_thermal_equals_single_instances(lbmx_mod, coll, design, prec):
     J, sens = b.gradient(STEPS, u0=u0, T0=T0)
     for i, (Js, gs) in enumerate(ref):
         assert abs(Js) > 0
-        assert abs(J[i] - Js) <= 1e-13 * abs(Js), (i, J[i], Js)
+        assert abs(J[i] - Js) <= 1e-12 * abs(Js), (i, J[i], Js)
         assert np.array_equal(sens[i], gs), (i, float(np.max(np.abs(sens[i] - gs))))
     b.close()
 
