@@ -285,8 +285,8 @@ lbmx_status lbmx_time_sweeps(lbmx_solver* s, int kind, int nsweeps, double* ms);
 /* Measured HBM ceiling: iters launches of a streaming copy of `bytes` (read + write);
    *gbs = (2 * bytes * iters) / device time. Also the calibration kernel for byte counters. */
 lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs);
-/* Diagnostic: GB/s (read + write) of a copy with the sweep's access pattern, planes = 9 or 19
-   FP64 planes of `nodes` values read and written at the same offset, one node per thread.
+/* Diagnostic: GB/s (read + write) of a copy with the sweep's access pattern, planes = 9, 14 or
+   19 FP64 planes of `nodes` values read and written at the same offset, one node per thread.
    No counterpart in the reference. */
 lbmx_status lbmx_stream_copy_planes(int planes, size_t nodes, int iters, double* gbs);
 /* Number of visible HIP devices (0 when the runtime finds none); no handle needed. */

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 
+#include "host_common.hpp"
 #include "lbmx.h"
 
 namespace lbmx {
@@ -34,13 +35,23 @@ __global__ __launch_bounds__(256) void k_plane_copy(const double* __restrict__ i
 }  // namespace lbmx
 
 extern "C" lbmx_status lbmx_stream_copy_planes(int planes, size_t nodes, int iters, double* gbs) {
-  if (!gbs || iters < 1 || nodes < 256 || (planes != 9 && planes != 19)) return LBMX_ERR_ARG;
+  if (!gbs || iters < 1 || nodes < 256 || (planes != 9 && planes != 14 && planes != 19)) {
+    lbmx::g_last_error = "stream copy planes: needs gbs, iters >= 1, nodes >= 256 and planes 9, 14 or 19";
+    return LBMX_ERR_ARG;
+  }
   const long long n = (long long)nodes;
   const long long plane = (n + 31) / 32 * 32;  // 256-byte aligned planes
   const size_t bytes = (size_t)planes * plane * 8;
   void *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, bytes) != hipSuccess) return LBMX_ERR_MEM;
-  if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return LBMX_ERR_MEM; }
+  if (hipMalloc(&a, bytes) != hipSuccess) {
+    lbmx::g_last_error = "device allocation failed";
+    return LBMX_ERR_MEM;
+  }
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    lbmx::g_last_error = "device allocation failed";
+    return LBMX_ERR_MEM;
+  }
   (void)hipMemset(a, 0, bytes);
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
@@ -49,6 +60,9 @@ extern "C" lbmx_status lbmx_stream_copy_planes(int planes, size_t nodes, int ite
   auto launch = [&]() {
     if (planes == 19)
       hipLaunchKernelGGL(lbmx::k_plane_copy<19>, dim3(blocks), dim3(256), 0, nullptr, (const double*)a, (double*)b,
+                         plane, n);
+    else if (planes == 14)
+      hipLaunchKernelGGL(lbmx::k_plane_copy<14>, dim3(blocks), dim3(256), 0, nullptr, (const double*)a, (double*)b,
                          plane, n);
     else
       hipLaunchKernelGGL(lbmx::k_plane_copy<9>, dim3(blocks), dim3(256), 0, nullptr, (const double*)a, (double*)b,
@@ -65,17 +79,30 @@ extern "C" lbmx_status lbmx_stream_copy_planes(int planes, size_t nodes, int ite
   (void)hipEventDestroy(e1);
   (void)hipFree(a);
   (void)hipFree(b);
-  if (err != hipSuccess) return LBMX_ERR_DEVICE;
+  if (err != hipSuccess) {
+    lbmx::g_last_error = hipGetErrorString(err);
+    return LBMX_ERR_DEVICE;
+  }
   *gbs = 2.0 * planes * n * 8.0 * iters / (ms * 1e-3) / 1e9;
   return LBMX_OK;
 }
 
 extern "C" lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs) {
-  if (!gbs || iters < 1 || bytes < 1024) return LBMX_ERR_ARG;
+  if (!gbs || iters < 1 || bytes < 1024) {
+    lbmx::g_last_error = "stream copy: needs gbs, iters >= 1 and bytes >= 1024";
+    return LBMX_ERR_ARG;
+  }
   const long long n4 = (long long)(bytes / 32);
   void *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, n4 * 32) != hipSuccess) return LBMX_ERR_MEM;
-  if (hipMalloc(&b, n4 * 32) != hipSuccess) { (void)hipFree(a); return LBMX_ERR_MEM; }
+  if (hipMalloc(&a, n4 * 32) != hipSuccess) {
+    lbmx::g_last_error = "device allocation failed";
+    return LBMX_ERR_MEM;
+  }
+  if (hipMalloc(&b, n4 * 32) != hipSuccess) {
+    (void)hipFree(a);
+    lbmx::g_last_error = "device allocation failed";
+    return LBMX_ERR_MEM;
+  }
   (void)hipMemset(a, 0, n4 * 32);
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
@@ -93,7 +120,10 @@ extern "C" lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs) {
   (void)hipEventDestroy(e1);
   (void)hipFree(a);
   (void)hipFree(b);
-  if (err != hipSuccess) return LBMX_ERR_DEVICE;
+  if (err != hipSuccess) {
+    lbmx::g_last_error = hipGetErrorString(err);
+    return LBMX_ERR_DEVICE;
+  }
   *gbs = 2.0 * n4 * 32 * iters / (ms * 1e-3) / 1e9;
   return LBMX_OK;
 }

@@ -116,6 +116,12 @@ def test_error_paths_without_device():
     assert lib.lbmx_create(C.byref(bad), C.byref(h)) != lbmx.LBMX_OK
     with pytest.raises(lbmx.LbmxError):
         lbmx.checkpoint_simulate(1, 10)
+    # the copy diagnostics reject their arguments before touching a device, with a message
+    for call in (lambda: lbmx.plane_copy_gbs(7, 4096), lambda: lbmx.plane_copy_gbs(19, 16),
+                 lambda: lbmx.stream_copy_gbs(nbytes=100)):
+        with pytest.raises(lbmx.LbmxError) as e:
+            call()
+        assert e.value.code == lbmx.ERR_ARG and "needs" in str(e.value)
 
 
 def test_host_side_checks_of_the_newer_entry_points():
