@@ -144,6 +144,7 @@ struct Engine {
   DeviceBuffer gamma64_dev;  // FP64 copy of a design set on the device, for the FP32-design forms
   DeviceBuffer gamma_dev, bc_dev, term_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
   DeviceBuffer nside_dev;  // Neumann pieces: adjoint transfer coefficients of the last adjoint sweep
+  DeviceBuffer bcrow_dev;  // boundary pieces meeting each 64-node row segment (bc_row_index)
   int nside_count = 0;     // doubles in nside_dev
   std::vector<std::unique_ptr<DeviceBuffer>> pool;  // state buffers: [0],[1] working; rest slots
   bool bc_dirty = true;
@@ -473,6 +474,19 @@ struct Engine {
       bc_dev.ensure(std::max<size_t>(1, bcs.size()) * sizeof(DevBC));
       if (!bcs.empty())
         HIP_OK(hipMemcpyAsync(bc_dev.p, bcs.data(), bcs.size() * sizeof(DevBC), hipMemcpyHostToDevice, stream));
+      // the pieces meeting each 64-node row segment (a sweep wave visits only those)
+      const long long nbx = (d.nx + 63) / 64;
+      std::vector<unsigned long long> rows((size_t)(nbx * d.ny * d.nz), 0ull);
+      for (size_t b = 0; b < bcs.size(); ++b) {
+        const int* bx = bcs[b].box;
+        for (int z = std::max(bx[4], 0); z <= std::min(bx[5], d.nz - 1); ++z)
+          for (int y = std::max(bx[2], 0); y <= std::min(bx[3], d.ny - 1); ++y)
+            for (int sx = std::max(bx[0], 0) / 64; sx <= std::min(bx[1], d.nx - 1) / 64; ++sx)
+              rows[(size_t)bc_row_index(d.nx, d.ny, sx * 64, y, z)] |= 1ull << b;
+      }
+      bcrow_dev.ensure(rows.size() * sizeof(unsigned long long));
+      HIP_OK(hipMemcpyAsync(bcrow_dev.p, rows.data(), rows.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                            stream));
       HIP_OK(hipStreamSynchronize(stream));
       bc_dirty = false;
     }
@@ -503,6 +517,7 @@ struct Engine {
     a.dst = dst;
     a.gamma = gamma_dev.p;
     a.bcs = static_cast<const DevBC*>(bc_dev.p);
+    a.bcrows = static_cast<const unsigned long long*>(bcrow_dev.p);
     a.nbc = (int)bcs.size();
     a.t = tnew;
     a.mode = 0;
@@ -583,6 +598,7 @@ struct Engine {
     a.sens = static_cast<double*>(sens_dev.p);
     a.nside = static_cast<double*>(nside_dev.p);
     a.bcs = static_cast<const DevBC*>(bc_dev.p);
+    a.bcrows = static_cast<const unsigned long long*>(bcrow_dev.p);
     a.nbc = (int)bcs.size();
     a.t = tstep;
     time_factors(tstep, a.tfac);
@@ -946,7 +962,7 @@ struct Engine {
 
   size_t device_bytes() const {
     size_t b = gamma64_dev.bytes + gamma_dev.bytes + bc_dev.bytes + term_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
-               lam_b.bytes + raw_aos.bytes + nside_dev.bytes;
+               lam_b.bytes + raw_aos.bytes + nside_dev.bytes + bcrow_dev.bytes;
     for (auto& p : pool) b += p->bytes;
     return b;
   }
@@ -1081,6 +1097,7 @@ struct Batch {
       I.gamma = e.gamma_dev.p;
       I.sens = static_cast<double*>(e.sens_dev.p);
       I.bcs = static_cast<const DevBC*>(e.bc_dev.p);
+      I.bcrows = static_cast<const unsigned long long*>(e.bcrow_dev.p);
       I.nbc = (int)e.bcs.size();
       I.blk_begin = nblocks;
       I.state_off = (long long)st;

@@ -7,6 +7,13 @@ namespace lbmx {
 
 constexpr int kMaxTerms = 4;
 constexpr int kMaxBC = 64;  // boundary pieces per lattice
+
+// Boundary pieces by row segment: bit b of bcrows[bc_row_index(...)] is set when piece b
+// meets the 64 nodes x0..x0+63 (x0 = 64 floor(x/64)) of row (y, z). A sweep's wave covers
+// exactly one such segment, so it visits only the pieces that can hold its nodes.
+inline constexpr long long bc_row_index(int nx, int ny, int x, int y, int z) {
+  return ((long long)z * ny + y) * ((nx + 63) / 64) + x / 64;
+}
 constexpr int kTileX = 64;   // threads along x per workgroup (one wave)
 constexpr int kTileY = 4;    // waves per workgroup, stacked along y
 
@@ -82,6 +89,7 @@ struct FwdArgs {
   void* dst;
   const void* gamma;
   const DevBC* bcs;
+  const unsigned long long* bcrows;  // pieces meeting each 64-node row segment (bc_row_index)
   int nbc;
   int t;              // time of the new state (boundary values)
   double tfac[kMaxBC];  // time factor of each boundary piece at t (computed on the host)
@@ -106,6 +114,7 @@ struct AdjArgs {
   double* sens;
   double* nside;      // Neumann pieces: the two coefficients each node sends to its inward neighbours
   const DevBC* bcs;
+  const unsigned long long* bcrows;  // pieces meeting each 64-node row segment (bc_row_index)
   int nbc;
   int t;
   double tfac[kMaxBC];
@@ -125,6 +134,7 @@ struct BatchInst {
   const void* gamma;      // design field (element type of the precision form)
   double* sens;           // dJ/dgamma accumulator
   const DevBC* bcs;
+  const unsigned long long* bcrows;  // pieces meeting each 64-node row segment (bc_row_index)
   int nbc;
   int blk_begin;          // first global workgroup of this instance
   long long state_off;    // byte offset of the instance in the batch state buffers

@@ -229,11 +229,25 @@ __device__ __forceinline__ bool is_neumann(const DevBC& e) {
 
 // Boundary pieces in insertion order at node (x,y,z) of the post-stream state; src/ls: the
 // field the state was pulled from (the Neumann conditions read two inward neighbours).
+// The boundary pieces that meet this wave's row segment, read once per wave. Every lane of a
+// wave has the same y and z and x in the same 64-node segment (workgroups are 64 threads
+// wide in x, one wave per row), so the first active lane's word is the wave's.
+static_assert(kTileX == 64, "a wave must cover one 64-node row segment");
+__device__ __forceinline__ unsigned long long bc_pieces(const unsigned long long* rows, int nx, int ny, int x, int y,
+                                                        int z) {
+  const unsigned long long m = rows[bc_row_index(nx, ny, x, y, z)];
+  const unsigned int lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(m));
+  const unsigned int hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(m >> 32));
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
 template <class L, class P>
-__device__ __forceinline__ void apply_bcs(typename P::T (&f)[L::Q], const DevBC* __restrict__ bcs, int nbc,
-                                          const double* tfac, int x, int y, int z, const void* src, const Layout& ls) {
+__device__ __forceinline__ void apply_bcs(typename P::T (&f)[L::Q], const DevBC* __restrict__ bcs,
+                                          unsigned long long pieces, const double* tfac, int x, int y, int z,
+                                          const void* src, const Layout& ls) {
   using T = typename P::T;
-  for (int b = 0; b < nbc; ++b) {
+  for (unsigned long long m = pieces; m != 0; m &= m - 1) {  // increasing piece index
+    const int b = __builtin_ctzll(m);
     const DevBC& e = bcs[b];
     if (!in_box(e.box, x, y, z)) continue;
     if (is_neumann<L>(e)) {
@@ -250,9 +264,11 @@ __device__ __forceinline__ void apply_bcs(typename P::T (&f)[L::Q], const DevBC*
 // (AdjointIncompressibleZouHeNeumann{East,West}2d, src/adjointBoundaryConditions/adjointZouHe2d.cc:114-172;
 // AdjointThermalZouHeNeumann{East,West}2d, adjointThermalZouHe2d.hh:161-227).
 template <class L, class T>
-__device__ __forceinline__ void apply_bcs_adj(T (&l)[L::Q], const DevBC* __restrict__ bcs, int nbc,
+__device__ __forceinline__ void apply_bcs_adj(T (&l)[L::Q], const DevBC* __restrict__ bcs, unsigned long long pieces,
                                               int x, int y, int z, double* nside) {
-  for (int b = nbc - 1; b >= 0; --b) {
+  for (unsigned long long m = pieces; m != 0;) {  // decreasing piece index
+    const int b = 63 - __builtin_clzll(m);
+    m &= ~(1ull << b);
     const DevBC& e = bcs[b];
     if (!in_box(e.box, x, y, z)) continue;
     if constexpr (L::D == 2) {
@@ -373,7 +389,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
       static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
     } else {
       pull_node<L, P, 2>(a.src, a.ls, off, f);
-      if (bnd) apply_bcs<L, P>(f, a.bcs, a.nbc, tfac, x, y, z, a.src, a.ls);
+      if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(a.bcrows, nx, ny, x, y, z), tfac, x, y, z, a.src, a.ls);
     }
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
@@ -466,7 +482,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
     static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(a.fwd_aos[(long long)n * L::Q + decltype(qc)::value]); });
   } else {
     pull_node<L, P, 1>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
-    if (bnd) apply_bcs<L, P>(f, a.bcs, a.nbc, tfac, x, y, z, a.fwd_src, a.ls);
+    if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(a.bcrows, nx, ny, x, y, z), tfac, x, y, z, a.fwd_src, a.ls);
   }
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
   // The adjoint field between sweeps is m_t, the boundary-corrected adjoint of s_t before the
@@ -501,7 +517,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   T dg = 0;
   reverse_collide<L, COLL, DES, T>(f, gam, a.m, lam, dg, srcA, srcB, srcAT);
   __builtin_nontemporal_store(__builtin_nontemporal_load(a.sens + n) + double(dg), a.sens + n);
-  if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, a.nbc, x, y, z, a.nside);
+  if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, bc_pieces(a.bcrows, nx, ny, x, y, z), x, y, z, a.nside);
   // m_{t-1} at the own node (whole rows); the next adjoint sweep gathers its transposed stream
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -552,6 +568,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_fwd_batch(BatchStep st) {
   a.dst = static_cast<char*>(st.dst) + I.state_off;
   a.gamma = I.gamma;
   a.bcs = I.bcs;
+  a.bcrows = I.bcrows;
   a.nbc = I.nbc;
   a.t = st.t;
   a.mode = 0;
@@ -577,6 +594,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   a.gamma = I.gamma;
   a.sens = I.sens;
   a.bcs = I.bcs;
+  a.bcrows = I.bcrows;
   a.nbc = I.nbc;
   a.t = st.t;
   a.g = I.g;
