@@ -229,13 +229,17 @@ __device__ __forceinline__ bool is_neumann(const DevBC& e) {
 
 // Boundary pieces in insertion order at node (x,y,z) of the post-stream state; src/ls: the
 // field the state was pulled from (the Neumann conditions read two inward neighbours).
-// The boundary pieces that meet this wave's row segment, read once per wave. Every lane of a
-// wave has the same y and z and x in the same 64-node segment (workgroups are 64 threads
-// wide in x, one wave per row), so the first active lane's word is the wave's.
+// The boundary pieces that meet this wave's row segment. Every lane of a wave has the same y
+// and z and x in the same 64-node segment (workgroups are 64 threads wide in x, one wave per
+// row), so the first active lane's word is the wave's. The sweeps load the word with their
+// population loads (bc_row_word) and make it uniform only in the boundary branch (bc_pieces),
+// so its latency overlaps theirs.
 static_assert(kTileX == 64, "a wave must cover one 64-node row segment");
-__device__ __forceinline__ unsigned long long bc_pieces(const unsigned long long* rows, int nx, int ny, int x, int y,
-                                                        int z) {
-  const unsigned long long m = rows[bc_row_index(nx, ny, x, y, z)];
+__device__ __forceinline__ unsigned long long bc_row_word(const unsigned long long* rows, int nx, int ny, int x, int y,
+                                                          int z) {
+  return rows[bc_row_index(nx, ny, x, y, z)];
+}
+__device__ __forceinline__ unsigned long long bc_pieces(unsigned long long m) {
   const unsigned int lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(m));
   const unsigned int hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned int>(m >> 32));
   return (static_cast<unsigned long long>(hi) << 32) | lo;
@@ -383,13 +387,14 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
     const unsigned int off = layout_off(a.ls, x, y, z);
     const int n = (z * ny + y) * nx + x;
     const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
+    const unsigned long long rowword = bc_row_word(a.bcrows, nx, ny, x, y, z);
     T f[L::Q];
     if (a.mode == 2) {
       const double* s = static_cast<const double*>(a.src);
       static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
     } else {
       pull_node<L, P, 2>(a.src, a.ls, off, f);
-      if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(a.bcrows, nx, ny, x, y, z), tfac, x, y, z, a.src, a.ls);
+      if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(rowword), tfac, x, y, z, a.src, a.ls);
     }
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
@@ -477,12 +482,13 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   if (x >= nx || y >= ny) return;
   const int n = (z * ny + y) * nx + x;
   const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
+  const unsigned long long rowword = bc_row_word(a.bcrows, nx, ny, x, y, z);
   T f[L::Q];
   if (a.fwd_aos) {
     static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(a.fwd_aos[(long long)n * L::Q + decltype(qc)::value]); });
   } else {
     pull_node<L, P, 1>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
-    if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(a.bcrows, nx, ny, x, y, z), tfac, x, y, z, a.fwd_src, a.ls);
+    if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(rowword), tfac, x, y, z, a.fwd_src, a.ls);
   }
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
   // The adjoint field between sweeps is m_t, the boundary-corrected adjoint of s_t before the
@@ -517,7 +523,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   T dg = 0;
   reverse_collide<L, COLL, DES, T>(f, gam, a.m, lam, dg, srcA, srcB, srcAT);
   __builtin_nontemporal_store(__builtin_nontemporal_load(a.sens + n) + double(dg), a.sens + n);
-  if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, bc_pieces(a.bcrows, nx, ny, x, y, z), x, y, z, a.nside);
+  if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, bc_pieces(rowword), x, y, z, a.nside);
   // m_{t-1} at the own node (whole rows); the next adjoint sweep gathers its transposed stream
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
