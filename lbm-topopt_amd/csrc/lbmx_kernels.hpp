@@ -613,6 +613,78 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   adj_node<L, COLL, DES, P>(a, st.tfac + (long long)i * st.tf_stride, x, y, z);
 }
 
+// Batched forward sweep with two nodes per thread: a tile of 2H rows is swept by H waves, each
+// thread taking rows y and y + H. Both nodes' loads (pulls, boundary row word, design) are
+// issued before either node collides, which doubles the bytes in flight per wave at the same
+// occupancy; each node's arithmetic is that of fwd_node, in the same order. (The adjoint,
+// which already has three fields in flight per node, keeps one node per thread: paired, it
+// was slower.)
+template <class L, int COLL, int DES, class P>
+__device__ __forceinline__ double fwd_finish(const FwdArgs& a, const double* tfac, typename P::T (&f)[L::Q],
+                                             typename P::T gam, unsigned long long rowword, unsigned int off, int x,
+                                             int y, int z, bool objblk) {
+  using T = typename P::T;
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+  const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
+  if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(rowword), tfac, x, y, z, a.src, a.ls);
+  double objv = 0.0;
+  if (objblk && in_box(a.obj.bbox, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
+  T rho, j[3];
+  collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+  store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
+  return objv;
+}
+
+template <class L, int COLL, int DES, class P, int H>
+__global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
+  using T = typename P::T;
+  int i, x, y, z, tile;
+  const BatchInst& I = batch_locate<2 * H>(st, i, x, y, z, tile);
+  FwdArgs a;
+  a.src = static_cast<const char*>(st.src) + I.state_off;
+  a.dst = static_cast<char*>(st.dst) + I.state_off;
+  a.gamma = I.gamma;
+  a.bcs = I.bcs;
+  a.bcrows = I.bcrows;
+  a.nbc = I.nbc;
+  a.t = st.t;
+  a.mode = 0;
+  a.aos_out = nullptr;
+  a.g = I.g;
+  a.ls = I.ls;
+  a.m = I.m;
+  a.obj = I.obj;
+  if (!st.with_obj) a.obj.acc = nullptr;
+  const double* tfac = st.tfac + (long long)i * st.tf_stride;
+  const int nx = a.g.nx, ny = a.g.ny;
+  const int y2 = y + H;
+  const bool act1 = x < nx && y < ny, act2 = x < nx && y2 < ny;
+  const bool objblk = a.obj.acc != nullptr && tile_meets_terms<2 * H>(a.obj, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
+  const typename P::G* gp = static_cast<const typename P::G*>(a.gamma);
+  T f1[L::Q], f2[L::Q], g1 = 0, g2 = 0;
+  unsigned long long rw1 = 0, rw2 = 0;
+  unsigned int off1 = 0, off2 = 0;
+  if (act1) {
+    off1 = layout_off(a.ls, x, y, z);
+    rw1 = bc_row_word(a.bcrows, nx, ny, x, y, z);
+    pull_node<L, P, 2>(a.src, a.ls, off1, f1);
+    g1 = T(gp[(z * ny + y) * nx + x]);
+  }
+  if (act2) {
+    off2 = layout_off(a.ls, x, y2, z);
+    rw2 = bc_row_word(a.bcrows, nx, ny, x, y2, z);
+    pull_node<L, P, 2>(a.src, a.ls, off2, f2);
+    g2 = T(gp[(z * ny + y2) * nx + x]);
+  }
+  double objv = 0.0;
+  if (act1) objv += fwd_finish<L, COLL, DES, P>(a, tfac, f1, g1, rw1, off1, x, y, z, objblk);
+  if (act2) objv += fwd_finish<L, COLL, DES, P>(a, tfac, f2, g2, rw2, off2, x, y2, z, objblk);
+  if (objblk) {
+    double sum = block_sum<H>(objv);
+    if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += sum;
+  }
+}
+
 // ---- host-side launchers ----
 template <class L, int COLL, int DES, class P>
 hipError_t launch_fwd_t(const FwdArgs& a, hipStream_t s) {
@@ -646,12 +718,21 @@ hipError_t launch_neumann_fix_t(const DevBC& e, void* lam, const Layout& ll, con
   }
 }
 
+// A tile of TY rows. The forward takes one node per thread for TY = 1 and two per thread
+// (TY / 2 waves) above; the adjoint takes one node per thread (TY waves).
 template <class L, int COLL, int DES, class P, int TY>
 hipError_t launch_batch_t(int kind, const BatchStep& st, hipStream_t s) {
-  if (kind == 0)
-    hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);
-  else
+  if (kind == 0) {
+    if constexpr (TY == 1) {
+      hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, 1>), dim3(st.nblocks), dim3(kTileX, 1, 1), 0, s, st);
+    } else {
+      static_assert(TY % 2 == 0, "paired tiles have an even number of rows");
+      hipLaunchKernelGGL((k_fwd_batch_pair<L, COLL, DES, P, TY / 2>), dim3(st.nblocks), dim3(kTileX, TY / 2, 1), 0, s,
+                         st);
+    }
+  } else {
     hipLaunchKernelGGL((k_adj_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);
+  }
   return hipGetLastError();
 }
 
