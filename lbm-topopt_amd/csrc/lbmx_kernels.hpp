@@ -635,27 +635,10 @@ __device__ __forceinline__ double fwd_finish(const FwdArgs& a, const double* tfa
   return objv;
 }
 
+// Rows y and y + H of the node's tile (2H rows, H waves).
 template <class L, int COLL, int DES, class P, int H>
-__global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
+__device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, int x, int y, int z, int tile) {
   using T = typename P::T;
-  int i, x, y, z, tile;
-  const BatchInst& I = batch_locate<2 * H>(st, i, x, y, z, tile);
-  FwdArgs a;
-  a.src = static_cast<const char*>(st.src) + I.state_off;
-  a.dst = static_cast<char*>(st.dst) + I.state_off;
-  a.gamma = I.gamma;
-  a.bcs = I.bcs;
-  a.bcrows = I.bcrows;
-  a.nbc = I.nbc;
-  a.t = st.t;
-  a.mode = 0;
-  a.aos_out = nullptr;
-  a.g = I.g;
-  a.ls = I.ls;
-  a.m = I.m;
-  a.obj = I.obj;
-  if (!st.with_obj) a.obj.acc = nullptr;
-  const double* tfac = st.tfac + (long long)i * st.tf_stride;
   const int nx = a.g.nx, ny = a.g.ny;
   const int y2 = y + H;
   const bool act1 = x < nx && y < ny, act2 = x < nx && y2 < ny;
@@ -683,6 +666,28 @@ __global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
     double sum = block_sum<H>(objv);
     if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += sum;
   }
+}
+
+template <class L, int COLL, int DES, class P, int H>
+__global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
+  int i, x, y, z, tile;
+  const BatchInst& I = batch_locate<2 * H>(st, i, x, y, z, tile);
+  FwdArgs a;
+  a.src = static_cast<const char*>(st.src) + I.state_off;
+  a.dst = static_cast<char*>(st.dst) + I.state_off;
+  a.gamma = I.gamma;
+  a.bcs = I.bcs;
+  a.bcrows = I.bcrows;
+  a.nbc = I.nbc;
+  a.t = st.t;
+  a.mode = 0;
+  a.aos_out = nullptr;
+  a.g = I.g;
+  a.ls = I.ls;
+  a.m = I.m;
+  a.obj = I.obj;
+  if (!st.with_obj) a.obj.acc = nullptr;
+  fwd_pair<L, COLL, DES, P, H>(a, st.tfac + (long long)i * st.tf_stride, x, y, z, tile);
 }
 
 // ---- host-side launchers ----
