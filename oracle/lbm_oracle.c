@@ -20,7 +20,13 @@
  *                            source moments (q rows: the full source) -- the polynomials as coded
  *   pull stream            : brute-force permutation on tiny grids
  *   boundary conditions    : prescribed u / rho / T recovered at the node; bounce-back
- *                            against the rule "incoming <- opposite, tangential swapped";
+ *                            against the rule "incoming <- opposite, tangential swapped", and
+ *                            the (x0,y0,z0) D3Q19 corner against its as-coded copy-and-swap list;
+ *                            every Zou-He / Hecht velocity, pressure and Neumann piece (each
+ *                            orientation, incompressible and standard): only the unknowns
+ *                            change, the prescribed moments hold, and the non-equilibrium
+ *                            bounce-back holds along the normal and per tangential axis -- a
+ *                            rank check shows these equations fix every unknown;
  *                            Zou-He Neumann (flow, D2Q5 temperature): u_x or T of the node equals
  *                            (4 m_1 - m_2)/3 of the two nodes inwards, u_y = 0, only the unknowns
  *                            change, whole-field dot test including the neighbour blocks
@@ -976,8 +982,9 @@ static double bc_value(const orc_problem* p, const orc_bc* e, int t, int i, int 
 
 /* generic on-grid bounce-back rule; matches OnGridBounceback2d (src/boundaryConditions/
    bounceback2d.cc:7-193) and 25 of the 26 D3Q19 orientations of bounceback3d.cc:8-511;
-   the corner at (x0,y0,z0) (bounceback3d.cc:434-451) writes f18=f9 where the rule gives
-   f13=f4, reproduced here. */
+   the corner at (x0,y0,z0) (bounceback3d.cc:434-451) leaves f9 and f13 as they came in
+   (its swap of 9 and 18 follows the copy 18 <- 9) where the rule gives f9=f18 and f13=f4,
+   reproduced here. */
 static void bounceback_node(const orc_problem* p, const int* n, double* f) {
   int Q = p->Qf;
   double out[QMAX];

@@ -531,9 +531,9 @@ def test_boundary_conditions_prescribe_values(lattice):
                         exp = fi[:Qf].copy()
                         sel = (cn <= 0) & (np.arange(Qf) > 0)
                         exp[sel] = fi[op[:Qf]][sel]
-                        quirk = lattice == lbmx.D3Q19 and tuple(nrm) == (-1, -1, -1)
-                        if not quirk:
-                            assert np.array_equal(fo[:Qf], exp), (piece, nrm)
+                        if lattice == lbmx.D3Q19 and tuple(nrm) == (-1, -1, -1):
+                            exp = _corner_as_coded(fi[:Qf])
+                        assert np.array_equal(fo[:Qf], exp), (piece, nrm)
                     elif typ in (lbmx.BC_ZOUHE_VELOCITY, lbmx.BC_HECHT_VELOCITY):
                         assert np.allclose(c[:Qf].T @ fo[:Qf], _inlet_velocity(o, piece, i, j, k, lattice), atol=1e-15)
                         known = (c[:Qf] @ n) >= 0
@@ -546,6 +546,174 @@ def test_boundary_conditions_prescribe_values(lattice):
                         T_set = 1.0 if nrm[0] == -1 else 0.0  # hot inlet, cold walls
                         assert abs(fo[Qf:].sum() - T_set) < 1e-15
                         assert np.array_equal(fo[:Qf], fi[:Qf])
+
+
+def _corner_as_coded(fi):
+    """The D3Q19 corner at (x0, y0, z0) as the reference codes it, in place on one node
+    (bounceback3d.cc:434-451): six copies, then three swaps. The swap of 9 and 18 comes after
+    18 = 9, so both end as the incoming f9, and f13 is never written; the generic rule would
+    give f9 = f18 and f13 = f4."""
+    f = np.array(fi, dtype=float)
+    for dst, src in ((10, 1), (11, 2), (12, 3), (15, 6), (17, 8), (18, 9)):
+        f[dst] = f[src]
+    for a, b in ((5, 14), (7, 16), (9, 18)):
+        f[a], f[b] = f[b], f[a]
+    return f
+
+
+def test_d3q19_corner_quirk_differs_from_the_rule_only_where_coded():
+    """The as-coded corner map is a permutation-with-copy that differs from the generic
+    bounce-back rule in exactly f9 and f13."""
+    o = mk(lbmx.D3Q19)
+    c, op, w = lat(o)
+    fi = RNG.standard_normal(19)
+    cn = c @ np.array([-1.0, -1.0, -1.0])
+    rule = fi.copy()
+    sel = (cn <= 0) & (np.arange(19) > 0)
+    rule[sel] = fi[op][sel]
+    quirk = _corner_as_coded(fi)
+    assert list(np.nonzero(quirk != rule)[0]) == [9, 13]
+    assert quirk[9] == quirk[18] == fi[9] and quirk[13] == fi[13]
+
+
+# ---------------------------------------------------------------- Zou-He / Hecht closures
+# Every velocity, pressure and Neumann piece the oracle accepts: (lattice, collision, box
+# (as a function of the grid), bc type, value, direction). Zou-He (1997) and Hecht & Harting
+# (2010) close the unknown populations of a face node by the prescribed moments plus the
+# bounce-back of the non-equilibrium part along the normal: f_q - f_opp(q) equals its
+# equilibrium difference 6 w_q c_q.j for the inward normal direction and, summed per pair,
+# for the inward diagonal directions of each tangential axis. j is the node's momentum after
+# the condition (incompressible: u = j; standard: u = j / rho, so 6 w rho c.u = 6 w c.j).
+def _closure_cases():
+    G2, G3 = (7, 6, 1), (5, 5, 5)
+    X2, Y2 = G2[0] - 1, G2[1] - 1
+    X3 = G3[0] - 1
+    west2 = (0, 0, 1, Y2 - 1, 0, 0)
+    east2 = (X2, X2, 1, Y2 - 1, 0, 0)
+    north2 = (1, X2 - 1, Y2, Y2, 0, 0)
+    west3 = (0, 0, 1, 3, 1, 3)
+    east3 = (X3, X3, 1, 3, 1, 3)
+    D2, D3 = lbmx.D2Q9, lbmx.D3Q19
+    ZV, ZP, ZN = lbmx.BC_ZOUHE_VELOCITY, lbmx.BC_ZOUHE_PRESSURE, lbmx.BC_ZOUHE_NEUMANN
+    HV, HP = lbmx.BC_HECHT_VELOCITY, lbmx.BC_HECHT_PRESSURE
+    return [
+        ("zh-vel-W-inc", D2, lbmx.BGK, G2, west2, ZV, 0.05, (1.0, 0.4, 0.0)),
+        ("zh-vel-N-inc", D2, lbmx.BGK, G2, north2, ZV, 0.05, (0.3, -1.0, 0.0)),
+        ("zh-vel-E-inc", D2, lbmx.BGK, G2, east2, ZV, 0.05, (-1.0, 0.25, 0.0)),
+        ("zh-vel-W-std", D2, lbmx.BGK_STANDARD, G2, west2, ZV, 0.05, (1.0, 0.4, 0.0)),
+        ("zh-prs-E-inc", D2, lbmx.BGK, G2, east2, ZP, 1.013, (1.0, 0.0, 0.0)),
+        ("zh-prs-W-inc", D2, lbmx.BGK, G2, west2, ZP, 1.013, (1.0, 0.0, 0.0)),
+        ("zh-prs-E-std", D2, lbmx.BGK_STANDARD, G2, east2, ZP, 1.013, (1.0, 0.0, 0.0)),
+        ("zh-prs-W-std", D2, lbmx.BGK_STANDARD, G2, west2, ZP, 1.013, (1.0, 0.0, 0.0)),
+        ("zh-neu-E", D2, lbmx.BGK, G2, east2, ZN, 0.0, (1.0, 0.0, 0.0)),
+        ("zh-neu-W", D2, lbmx.BGK, G2, west2, ZN, 0.0, (1.0, 0.0, 0.0)),
+        ("hecht-vel-W-inc", D3, lbmx.MRT, G3, west3, HV, 0.04, (1.0, 0.3, -0.2)),
+        ("hecht-vel-W-std", D3, lbmx.MRT_STANDARD, G3, west3, HV, 0.04, (1.0, 0.3, -0.2)),
+        ("hecht-prs-E-inc", D3, lbmx.MRT, G3, east3, HP, 1.013, (1.0, 0.0, 0.0)),
+        ("hecht-prs-E-std", D3, lbmx.MRT_STANDARD, G3, east3, HP, 1.013, (1.0, 0.0, 0.0)),
+    ]
+
+
+def _closure_residuals(fo, c, w, op, n, typ, std, value, direction, m_neumann):
+    """Residuals of the closure's defining equations at one node, as a function of its
+    populations after the condition: prescribed moments, then the non-equilibrium
+    bounce-back along the normal and per tangential axis."""
+    D = 3 if len(w) == 19 else 2
+    rho = fo.sum()
+    j = c.T @ fo
+    unknown = (c @ n) < 0
+    res = []
+    if typ in (lbmx.BC_ZOUHE_VELOCITY, lbmx.BC_HECHT_VELOCITY):
+        u = j / rho if std else j
+        res += list(u[:D] - value * np.asarray(direction)[:D])
+    else:
+        tang = [a for a in range(D) if n[a] == 0]
+        res += [j[a] for a in tang]                  # tangential velocity 0
+        if typ == lbmx.BC_ZOUHE_NEUMANN:
+            res.append(j[0] - m_neumann)             # extrapolated normal velocity
+        else:
+            res.append(rho - value)                  # prescribed density
+    inward = -n
+    qn = int(np.nonzero(np.all(c == inward, axis=1))[0][0])
+    res.append((fo[qn] - fo[op[qn]]) - 6 * w[qn] * (c[qn] @ j))
+    for a in range(D):
+        if n[a] != 0:
+            continue
+        S = np.nonzero(unknown & (c[:, a] != 0))[0]
+        res.append(sum(fo[q] - fo[op[q]] for q in S) - sum(6 * w[q] * (c[q] @ j) for q in S))
+    return np.array(res), unknown
+
+
+@pytest.mark.parametrize("case", _closure_cases(), ids=lambda cs: cs[0])
+def test_boundary_closures_are_pinned(case):
+    """Zou-He / Hecht / Zou-He Neumann pieces (ZouHe2d.hh:29-44,69-84,109-124,149-174,252-270,
+    440-527; Hecht3d.hh:69-89,182-204): only the unknown populations change, the prescribed
+    moments hold, and the non-equilibrium bounce-back holds along the normal and per tangential
+    axis. The rank check shows these equations leave no unknown free, so any other closure
+    with the right moments fails."""
+    name, lattice, coll, (nx, ny, nz), box, typ, value, direction = case
+    o = mk(lattice, coll, lbmx.DESIGN_NONE, nx=nx, ny=ny, nz=nz)
+    o.add_boundary(lbmx.make_box(*box), lbmx.make_bc(typ, value, direction))
+    pieces = o.boundaries()
+    assert len(pieces) == 1
+    _, nrm, pbox = pieces[0]
+    assert tuple(pbox) == tuple(box)
+    c, op, w = lat(o)
+    n = np.array(nrm, float)
+    std = coll in (lbmx.BGK_STANDARD, lbmx.MRT_STANDARD)
+    field = np.empty((nz, ny, nx, len(w)))
+    for idx in np.ndindex(nz, ny, nx):
+        field[idx] = near_eq(c, w, 0.01)
+    out = o.apply_bc(0, 3, field.ravel()).reshape(field.shape)
+    x0, x1, y0, y1, z0, z1 = box
+    for k in range(z0, z1 + 1):
+        for jj in range(y0, y1 + 1):
+            for i in range(x0, x1 + 1):
+                fi, fo = field[k, jj, i], out[k, jj, i]
+                m_neu = 0.0
+                if typ == lbmx.BC_ZOUHE_NEUMANN:
+                    step = -int(nrm[0])
+                    m1 = c[:, 0] @ field[k, jj, i + step]
+                    m2 = c[:, 0] @ field[k, jj, i + 2 * step]
+                    m_neu = (4 * m1 - m2) / 3
+                res, unknown = _closure_residuals(fo, c, w, op, n, typ, std, value, direction, m_neu)
+                assert np.array_equal(fo[~unknown], fi[~unknown]), (name, "a known population changed")
+                assert np.max(np.abs(res)) < 2e-15, (name, res)
+                # the residuals fix every unknown: their Jacobian over the unknowns has full rank
+                U = np.nonzero(unknown)[0]
+                Jm = np.empty((len(res), len(U)))
+                for col, q in enumerate(U):
+                    h = 1e-6
+                    fp, fm = fo.copy(), fo.copy()
+                    fp[q] += h
+                    fm[q] -= h
+                    rp = _closure_residuals(fp, c, w, op, n, typ, std, value, direction, m_neu)[0]
+                    rm = _closure_residuals(fm, c, w, op, n, typ, std, value, direction, m_neu)[0]
+                    Jm[:, col] = (rp - rm) / (2 * h)
+                sv = np.linalg.svd(Jm, compute_uv=False)
+                assert len(U) == (5 if lattice == lbmx.D3Q19 else 3)
+                assert sv[len(U) - 1] > 1e-3 * sv[0], (name, sv)
+    # nothing outside the piece moves
+    rest = np.ones((nz, ny, nx), bool)
+    rest[z0:z1 + 1, y0:y1 + 1, x0:x1 + 1] = False
+    assert np.array_equal(out[rest], field[rest])
+
+
+def test_closure_check_rejects_a_wrong_closure():
+    """A closure with the right moments but another normal/diagonal split (the Zou-He west
+    velocity with 0.1 u_x moved from f8 to f1 and f7, which keeps j) fails the check."""
+    o = mk(lbmx.D2Q9)
+    c, op, w = lat(o)
+    fi = near_eq(c, w, 0.01)
+    ux, uy = 0.05, 0.02
+    g = fi.copy()
+    g[1] = 0.5 * (g[6] - g[2]) + g[5] + ux / 6 - 0.5 * uy + 0.05 * ux
+    g[7] = 0.5 * (g[2] - g[6]) + g[3] + ux / 6 + 0.5 * uy + 0.05 * ux
+    g[8] = g[4] + 2 / 3 * ux - 0.1 * ux
+    assert np.allclose(c.T @ g, [ux, uy, 0], atol=1e-15)
+    res, _ = _closure_residuals(g, c, w, op, np.array([-1.0, 0, 0]), lbmx.BC_ZOUHE_VELOCITY, False, 1.0,
+                                (ux, uy, 0), 0.0)
+    assert np.max(np.abs(res)) > 1e-3
 
 
 def _inlet_velocity(o, piece, i, j, k, lattice):
