@@ -362,6 +362,27 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;
 }
 
+// The paired form of block_sum: wave w holds rows w and w + H of a 2H-row tile. Each row is
+// reduced across its wave on its own and the 2H row sums are added in row order, which is the
+// order block_sum<2H> gives one node per thread: the tile's partial is bitwise the same.
+template <int H>
+__device__ __forceinline__ double block_sum_rows2(double v1, double v2) {
+  __shared__ double part[2 * H];
+  for (int off = 32; off > 0; off >>= 1) {
+    v1 += __shfl_down(v1, off, 64);
+    v2 += __shfl_down(v2, off, 64);
+  }
+  if (threadIdx.x == 0) {
+    part[threadIdx.y] = v1;
+    part[threadIdx.y + H] = v2;
+  }
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    for (int w = 0; w < 2 * H; ++w) s += part[w];
+  return s;
+}
+
 template <class L, int COLL, class T>
 __device__ __forceinline__ T node_objective(const ObjArgs& o, const T (&f)[L::Q], int x, int y, int z) {
   T rho, j[3];
@@ -659,11 +680,11 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
     pull_node<L, P, 2>(a.src, a.ls, off2, f2);
     g2 = T(gp[(z * ny + y2) * nx + x]);
   }
-  double objv = 0.0;
-  if (act1) objv += fwd_finish<L, COLL, DES, P>(a, tfac, f1, g1, rw1, off1, x, y, z, objblk);
-  if (act2) objv += fwd_finish<L, COLL, DES, P>(a, tfac, f2, g2, rw2, off2, x, y2, z, objblk);
+  double obj1 = 0.0, obj2 = 0.0;
+  if (act1) obj1 = fwd_finish<L, COLL, DES, P>(a, tfac, f1, g1, rw1, off1, x, y, z, objblk);
+  if (act2) obj2 = fwd_finish<L, COLL, DES, P>(a, tfac, f2, g2, rw2, off2, x, y2, z, objblk);
   if (objblk) {
-    double sum = block_sum<H>(objv);
+    double sum = block_sum_rows2<H>(obj1, obj2);
     if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += sum;
   }
 }
@@ -723,17 +744,21 @@ hipError_t launch_neumann_fix_t(const DevBC& e, void* lam, const Layout& ll, con
   }
 }
 
-// A tile of TY rows. The forward takes one node per thread for TY = 1 and two per thread
-// (TY / 2 waves) above; the adjoint takes one node per thread (TY waves).
+// A tile of TY rows. The forward takes one node per thread (TY waves) for TY = 1 or when
+// `pair` is off, and two per thread (TY / 2 waves) otherwise; the adjoint takes one node per
+// thread (TY waves). Both forward forms cover the same tiles with the same node arithmetic.
 template <class L, int COLL, int DES, class P, int TY>
-hipError_t launch_batch_t(int kind, const BatchStep& st, hipStream_t s) {
+hipError_t launch_batch_t(int kind, bool pair, const BatchStep& st, hipStream_t s) {
   if (kind == 0) {
     if constexpr (TY == 1) {
       hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, 1>), dim3(st.nblocks), dim3(kTileX, 1, 1), 0, s, st);
     } else {
       static_assert(TY % 2 == 0, "paired tiles have an even number of rows");
-      hipLaunchKernelGGL((k_fwd_batch_pair<L, COLL, DES, P, TY / 2>), dim3(st.nblocks), dim3(kTileX, TY / 2, 1), 0, s,
-                         st);
+      if (pair)
+        hipLaunchKernelGGL((k_fwd_batch_pair<L, COLL, DES, P, TY / 2>), dim3(st.nblocks), dim3(kTileX, TY / 2, 1), 0,
+                           s, st);
+      else
+        hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);
     }
   } else {
     hipLaunchKernelGGL((k_adj_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);

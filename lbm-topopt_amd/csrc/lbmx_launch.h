@@ -84,9 +84,10 @@ hipError_t launch_mma_dual(const MmaArgs& a, const MmaLambda& lm, bool hess, dou
 hipError_t launch_mma_change(const double* xnew, const double* xo1, long long n, double* part, double* out,
                              hipStream_t s);
 
-// Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows.
+// Batched sweeps (kind 0 forward, 1 adjoint) over every instance of a batch; ty = tile rows,
+// pair = the forward of a 2- or 4-row tile takes two nodes per thread.
 #define LBMX_DECLARE_BATCH(SUFFIX) \
-  hipError_t launch_batch_##SUFFIX(int coll, int des, int ty, int kind, const BatchStep& st, hipStream_t s);
+  hipError_t launch_batch_##SUFFIX(int coll, int des, int ty, bool pair, int kind, const BatchStep& st, hipStream_t s);
 LBMX_DECLARE_BATCH(d2q9_f64) LBMX_DECLARE_BATCH(d2q9_f32) LBMX_DECLARE_BATCH(d2q9_f16) LBMX_DECLARE_BATCH(d2q9_bf16)
 LBMX_DECLARE_BATCH(d3q19_f64) LBMX_DECLARE_BATCH(d3q19_f32) LBMX_DECLARE_BATCH(d3q19_f16) LBMX_DECLARE_BATCH(d3q19_bf16)
 LBMX_DECLARE_BATCH(thermal_f64) LBMX_DECLARE_BATCH(thermal_f32) LBMX_DECLARE_BATCH(thermal_f16) LBMX_DECLARE_BATCH(thermal_bf16)
