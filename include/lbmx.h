@@ -120,13 +120,16 @@ typedef enum {
                                      InflowVelocity functors of applications/diffuser/diffuser.cc:21-31 */
   LBMX_PROFILE_PARABOLIC_DISK = 2,/* value*(1-r^2/R^2) inside r<=R, else 0; r^2=(s-c0)^2+(t-c1)^2
                                      (DevFlowYZ, applications/Diffuser3d/diffuser3d.cc:22-46)   */
-  LBMX_PROFILE_PARABOLIC_DUCT = 3 /* value*(1-((s-c0)/R)^2)*(1-((t-c1)/R1)^2)                  */
+  LBMX_PROFILE_PARABOLIC_DUCT = 3,/* value*(1-((s-c0)/R)^2)*(1-((t-c1)/R1)^2)                  */
+  LBMX_PROFILE_TABLE = 4          /* value*a(i,j,k): per-node values handed to
+                                     lbmx_add_boundary_table                                    */
 } lbmx_profile_kind;
 
 typedef enum {
   LBMX_TIME_CONSTANT = 0,
-  LBMX_TIME_SINE = 1              /* factor = t_offset + t_amp*sin(2*pi*t/t_period + t_phase),
+  LBMX_TIME_SINE = 1,             /* factor = t_offset + t_amp*sin(2*pi*t/t_period + t_phase),
                                      as applications/RobustPump2d/robustPump2d.cc:14-19           */
+  LBMX_TIME_TABLE = 2             /* factor = b(t) from a table handed to lbmx_add_boundary_table */
 } lbmx_time_kind;
 
 typedef struct {
@@ -182,6 +185,21 @@ const char* lbmx_last_error(void);
    intersected with the 4 edges + 4 corners (2D) or 6 faces + 12 edges + 8 corners (3D)
    of the domain; each piece becomes one boundary loop, applied in insertion order. */
 lbmx_status lbmx_add_boundary(lbmx_solver* s, const lbmx_box* box, const lbmx_bc_desc* bc);
+/* The same for a boundary value given by any separable functor value * a(i,j,k) * b(t): the
+   velocity / pressure / temperature functors the applications hand to their boundary
+   descriptors (e.g. InflowVelocity, applications/diffuser/diffuser.cc:21-42; DevFlowYZ,
+   applications/Diffuser3d/diffuser3d.cc:11-54; the modulated inlet of
+   applications/RobustPump2d/robustPump2d.cc:13-25), tabulated once by the caller.
+   profile (may be NULL: bc->profile applies) holds a(i,j,k) for every node of *box in natural
+   order (i fastest), ncomp values per node: 1 = one factor (velocity: times bc->dir), or D (2 or
+   3) = one factor per velocity component (u_a = value * dir[a] * a_a(i,j,k) * b(t)); it is copied
+   to the device. time_factors (may be NULL: bc->time_mode applies) holds b(t) for the steps
+   t = 0 .. nt-1 at which the condition is applied (the timestep argument of the reference's
+   functors, the step the stream just reached); a forward or gradient call that would apply the
+   condition at a step t >= nt fails with LBMX_ERR_ARG. bc->profile and bc->time_mode are
+   ignored where a table is given. */
+lbmx_status lbmx_add_boundary_table(lbmx_solver* s, const lbmx_box* box, const lbmx_bc_desc* bc, const double* profile,
+                                    size_t nprofile, int ncomp, const double* time_factors, size_t nt);
 /* LBSolver::setFieldFromDesignVec (src/latticeBoltzmann/LBSolver.cc:333-348). */
 lbmx_status lbmx_set_design(lbmx_solver* s, const double* gamma, size_t n);
 /* LBSolver::createDesignVec (src/latticeBoltzmann/LBSolver.cc:318-331): copy out the field. */
@@ -322,16 +340,25 @@ lbmx_status lbmx_batch_destroy(lbmx_batch* b);
 lbmx_status lbmx_batch_checkpoint_config(lbmx_batch* b, int nslots);
 /* placement of the shared schedule's checkpoints (lbmx_checkpoint_mode) */
 lbmx_status lbmx_batch_checkpoint_mode(lbmx_batch* b, int mode);
-/* rows of the 64-wide workgroup tile of the batched sweeps: 1, 2 or 4 (default 4) */
+/* rows of the 64-wide workgroup tile of the batched sweeps: 1, 2 or 4 (default 4), with the
+   forward of 2 and 4 rows taking two nodes per thread */
 lbmx_status lbmx_batch_set_tile_rows(lbmx_batch* b, int rows);
+/* the same with the forward's pairing chosen: paired != 0 takes rows y and y + rows/2 in one
+   thread (2 and 4 rows), paired = 0 one node per thread. Results do not depend on the shape. */
+lbmx_status lbmx_batch_set_tile(lbmx_batch* b, int rows, int paired);
+/* the tile shape in use (set, or chosen by lbmx_batch_autotune); *paired = 0 for 1 row */
+lbmx_status lbmx_batch_get_tile(lbmx_batch* b, int* rows, int* paired);
 /* enable != 0: the first lbmx_batch_gradient captures its whole launch sequence into a HIP
    graph and later calls with the same steps, slots, tile height and buffers replay it (designs
    may change in between; they are read from the same device arrays). Forward/reverse phase
    times are then not separated in the stats. */
 lbmx_status lbmx_batch_set_graph(lbmx_batch* b, int enable);
-/* Autotuner: times nsweeps batched forward + adjoint sweeps for each tile height, keeps the
-   fastest; *rows = chosen height, ms[3] (may be NULL) = ms per forward+adjoint sweep pair for
-   heights 1, 2, 4. The members' current states are discarded. */
+/* Autotuner: times nsweeps batched forward + adjoint sweeps for each tile shape (height 1, 2 or
+   4; the forward of 2 and 4 rows paired or not), keeps the fastest; *rows = chosen height
+   (lbmx_batch_get_tile gives the pairing), ms[3] (may be NULL) = ms per forward+adjoint sweep
+   pair for heights 1, 2, 4, each with its better pairing. The members' current states, adjoints and dJ/dgamma arrays are discarded:
+   until their next gradient, lbmx_get_sensitivity and the other sensitivity reads on a member
+   return LBMX_ERR_STATE. */
 lbmx_status lbmx_batch_autotune(lbmx_batch* b, int nsweeps, int* rows, double* ms);
 /* lbmx_gradient for every member: J[count] (may be NULL) and sens[count] (array of pointers to
    per-member dJ/dgamma arrays in natural order; the array or any entry may be NULL). */

@@ -131,6 +131,8 @@ struct Engine {
   hipStream_t stream = nullptr;
 
   std::vector<DevBC> bcs;           // pieces in insertion order
+  std::vector<std::shared_ptr<DeviceBuffer>> bc_profiles;  // per-node tables of the pieces (shared by split pieces)
+  std::vector<std::shared_ptr<std::vector<double>>> bc_time;  // per-piece time-factor tables (null: none)
   std::vector<DevTerm> terms;       // functional terms
   bool terms_dirty = true;
   Layout ls{}, ll{};                // padded layouts of state and adjoint fields
@@ -313,9 +315,44 @@ struct Engine {
     return true;
   }
 
-  void add_boundary(const lbmx_box& box, const lbmx_bc_desc& bc) {
+  void add_boundary(const lbmx_box& box, const lbmx_bc_desc& bc) { add_boundary_table(box, bc, nullptr, 0, 0, nullptr, 0); }
+
+  // profile: ncomp values per node of `box` in natural order (null: bc.profile); tfac: b(t) for
+  // t = 0..nt-1 (null: bc.time_mode)
+  void add_boundary_table(const lbmx_box& box, const lbmx_bc_desc& bc, const double* profile, size_t nprofile,
+                          int ncomp, const double* tfac, size_t nt) {
     if (bc.type < LBMX_BC_BOUNCEBACK || bc.type > LBMX_BC_THERMAL_NEUMANN)
       throw Error{LBMX_ERR_ARG, "unknown boundary type"};
+    const bool valued = bc.type == LBMX_BC_ZOUHE_VELOCITY || bc.type == LBMX_BC_HECHT_VELOCITY ||
+                        bc.type == LBMX_BC_ZOUHE_PRESSURE || bc.type == LBMX_BC_HECHT_PRESSURE ||
+                        bc.type == LBMX_BC_THERMAL_DIRICHLET;
+    if ((profile || tfac) && !valued)
+      throw Error{LBMX_ERR_ARG, "value tables apply to velocity, pressure and thermal Dirichlet conditions"};
+    if (!profile && bc.profile == LBMX_PROFILE_TABLE) throw Error{LBMX_ERR_ARG, "table profile without a table"};
+    if (!tfac && bc.time_mode == LBMX_TIME_TABLE) throw Error{LBMX_ERR_ARG, "table time mode without a table"};
+    const long long bnx = (long long)box.x1 - box.x0 + 1, bny = (long long)box.y1 - box.y0 + 1,
+                    bnz = (long long)box.z1 - box.z0 + 1;
+    std::shared_ptr<DeviceBuffer> prof;
+    if (profile) {
+      const bool vel = bc.type == LBMX_BC_ZOUHE_VELOCITY || bc.type == LBMX_BC_HECHT_VELOCITY;
+      const int D = d.lattice == LBMX_D3Q19 ? 3 : 2;
+      if (!(ncomp == 1 || (vel && ncomp == D)))
+        throw Error{LBMX_ERR_ARG, "profile components must be 1 (or the lattice dimension for a velocity)"};
+      if (bnx < 1 || bny < 1 || bnz < 1 || (long long)nprofile != bnx * bny * bnz * ncomp)
+        throw Error{LBMX_ERR_ARG, "profile length != nodes of the box x components"};
+      for (size_t k = 0; k < nprofile; ++k)
+        if (!std::isfinite(profile[k])) throw Error{LBMX_ERR_ARG, "profile values must be finite"};
+      prof = std::make_shared<DeviceBuffer>();
+      prof->ensure(nprofile * sizeof(double));
+      HIP_OK(hipMemcpy(prof->p, profile, nprofile * sizeof(double), hipMemcpyHostToDevice));
+    }
+    std::shared_ptr<std::vector<double>> ttab;
+    if (tfac) {
+      if (nt < 1) throw Error{LBMX_ERR_ARG, "empty time-factor table"};
+      for (size_t k = 0; k < nt; ++k)
+        if (!std::isfinite(tfac[k])) throw Error{LBMX_ERR_ARG, "time factors must be finite"};
+      ttab = std::make_shared<std::vector<double>>(tfac, tfac + nt);
+    }
     int dom[6] = {0, d.nx - 1, 0, d.ny - 1, 0, d.nz - 1};
     int ub[6] = {box.x0, box.x1, box.y0, box.y1, box.z0, box.z1};
     int b[6];
@@ -328,10 +365,11 @@ struct Engine {
                                       " is not implemented for orientation (" + std::to_string(pc.n[0]) + "," +
                                       std::to_string(pc.n[1]) + "," + std::to_string(pc.n[2]) + ")"};
     }
-    if ((bc.type == LBMX_BC_ZOUHE_VELOCITY || bc.type == LBMX_BC_HECHT_VELOCITY) && bc.profile != LBMX_PROFILE_UNIFORM &&
-        (bc.radius[0] == 0.0))
+    if ((bc.type == LBMX_BC_ZOUHE_VELOCITY || bc.type == LBMX_BC_HECHT_VELOCITY) && !profile &&
+        bc.profile != LBMX_PROFILE_UNIFORM && (bc.radius[0] == 0.0))
       throw Error{LBMX_ERR_ARG, "profile radius must be non-zero"};
-    if (bc.time_mode == LBMX_TIME_SINE && bc.t_period == 0.0) throw Error{LBMX_ERR_ARG, "sine period must be non-zero"};
+    if (!tfac && bc.time_mode == LBMX_TIME_SINE && bc.t_period == 0.0)
+      throw Error{LBMX_ERR_ARG, "sine period must be non-zero"};
     int npieces = 0;
     for (auto& pc : ps) {
       int o[6];
@@ -354,6 +392,15 @@ struct Engine {
       e.center[0] = bc.center[0]; e.center[1] = bc.center[1];
       e.radius[0] = bc.radius[0]; e.radius[1] = bc.radius[1];
       e.t_period = bc.t_period; e.t_amp = bc.t_amp; e.t_offset = bc.t_offset; e.t_phase = bc.t_phase;
+      if (profile) {
+        e.profile = LBMX_PROFILE_TABLE;
+        e.table = static_cast<const double*>(prof->p);
+        e.ncomp = ncomp;
+        for (int k = 0; k < 6; ++k) e.tbox[k] = ub[k];
+      }
+      if (tfac) e.time_mode = LBMX_TIME_TABLE;
+      bc_profiles.push_back(prof);
+      bc_time.push_back(ttab);
       if (is_neumann(e.type)) {  // two adjoint transfer coefficients per node of the piece
         e.aux = nside_count;
         nside_count += 2 * (o[3] - o[2] + 1);
@@ -504,10 +551,23 @@ struct Engine {
   void time_factors(int tstep, double* tf) const {
     for (size_t b = 0; b < bcs.size(); ++b) {
       const DevBC& e = bcs[b];
+      if (e.time_mode == LBMX_TIME_TABLE) {  // steps past the table are refused up front (check_time_tables)
+        const std::vector<double>& v = *bc_time[b];
+        tf[b] = v[std::min<size_t>(std::max(tstep, 0), v.size() - 1)];
+        continue;
+      }
       tf[b] = e.time_mode == LBMX_TIME_SINE
                   ? e.t_offset + e.t_amp * std::sin(2.0 * M_PI * tstep / e.t_period + e.t_phase)
                   : 1.0;
     }
+  }
+
+  // a run that applies the boundary conditions up to step `last` needs every time table to reach it
+  void check_time_tables(long long last) const {
+    for (size_t b = 0; b < bcs.size(); ++b)
+      if (bcs[b].time_mode == LBMX_TIME_TABLE && (long long)bc_time[b]->size() <= last)
+        throw Error{LBMX_ERR_ARG, "boundary time-factor table holds steps 0.." + std::to_string(bc_time[b]->size() - 1) +
+                                      ", the run reaches step " + std::to_string(last)};
   }
 
   FwdArgs fwd_args(const void* src, void* dst, int tnew, bool with_obj) {
@@ -670,6 +730,7 @@ struct Engine {
   void forward(int nsteps, double* J) {
     require_state();
     if (nsteps < 0) throw Error{LBMX_ERR_ARG, "nsteps must be >= 0"};
+    check_time_tables((long long)t + nsteps);
     prepare();
     bool with_obj = J && obj_set && obj.nterm > 0;
     if (with_obj) zero_acc();
@@ -734,6 +795,7 @@ struct Engine {
       throw Error{LBMX_ERR_SUP, "DarcyMRT2d has no reverse operator (ReverseDarcyMRT2d is empty)"};
     if (nsteps < 1) throw Error{LBMX_ERR_ARG, "nsteps must be >= 1"};
     if (sens && (long long)n != N) throw Error{LBMX_ERR_ARG, "sensitivity length != nodes"};
+    check_time_tables(nsteps);
     prepare();
     const bool naive = nslots <= 0;
     const int slots = naive ? nsteps : std::min(nslots, std::max(2, nsteps));
@@ -838,6 +900,7 @@ struct Engine {
       throw Error{LBMX_ERR_STATE, "forward state is at step " + std::to_string(t) + ", the reverse step is " +
                                       std::to_string(adj_t)};
     if (state_kind != 0 && state_kind != 2) throw Error{LBMX_ERR_STATE, "no forward state at this step"};
+    check_time_tables(adj_t);
     prepare();
     ObjArgs oa = obj;
     oa.acc = nullptr;
@@ -916,6 +979,7 @@ struct Engine {
     if (nsweeps < 1) throw Error{LBMX_ERR_ARG, "nsweeps must be >= 1"};
     if (kind == 1 && (!inc || d.design == LBMX_DESIGN_DARCY))
       throw Error{LBMX_ERR_SUP, "no adjoint sweep for this operator"};
+    check_time_tables(kind == 0 ? (long long)t + nsweeps : (long long)t);
     prepare();
     EventPair ev;
     hipEvent_t e0 = ev.e0, e1 = ev.e1;
@@ -987,6 +1051,7 @@ struct Batch {
   std::vector<Engine*> m;  // members (owned by their handles)
   hipStream_t stream = nullptr;
   int ty = kTileY;         // rows of the workgroup tile
+  bool pair = true;        // forward of a 2- or 4-row tile: two nodes per thread
   int nslots = 64;
   int ckpt_mode = LBMX_CHECKPOINT_DYNAMIC;
   int nblocks = 0;
@@ -1042,7 +1107,7 @@ struct Batch {
 
   // every address and size the captured launch sequence depends on
   std::vector<long long> graph_key(int nsteps, int slots, bool naive, double rho0, const double* u0, double T0) const {
-    std::vector<long long> k = {nsteps, slots, naive ? 1 : 0, ckpt_mode, ty, nblocks, tf_stride};
+    std::vector<long long> k = {nsteps, slots, naive ? 1 : 0, ckpt_mode, ty, pair ? 1 : 0, nblocks, tf_stride};
     auto bits = [&](double v) {
       long long b;
       std::memcpy(&b, &v, sizeof b);
@@ -1151,7 +1216,7 @@ struct Batch {
     st.t = t;
     st.with_obj = with_obj ? 1 : 0;
     const lbmx_desc& d = m[0]->d;
-#define CALL(S) launch_batch_##S(d.collision, d.design, ty, kind, st, stream)
+#define CALL(S) launch_batch_##S(d.collision, d.design, ty, pair, kind, st, stream)
     switch (d.lattice) {
       case LBMX_D2Q9: LBMX_BATCH_PREC(d2q9, CALL)
       case LBMX_D3Q19: LBMX_BATCH_PREC(d3q19, CALL)
@@ -1188,6 +1253,7 @@ struct Batch {
 
   void gradient(int nsteps, double rho0, const double* u0, double T0, double* J, double* const* sens) {
     if (nsteps < 1) throw Error{LBMX_ERR_ARG, "nsteps must be >= 1"};
+    for (Engine* e : m) e->check_time_tables(nsteps);
     build();
     const bool naive = nslots <= 0;
     const int slots = naive ? nsteps : std::min(nslots, std::max(2, nsteps));
@@ -1281,16 +1347,23 @@ struct Batch {
     stats.reverse_phase_ms = std::chrono::duration<double, std::milli>(tr1 - ops.t_fwd_end).count();
   }
 
-  // Times nsweeps batched forward and adjoint sweeps for each tile height and keeps the fastest.
+  // Times nsweeps batched forward and adjoint sweeps for each tile shape (height 1, 2 or 4;
+  // the forward of 2 and 4 rows paired or not) and keeps the fastest. ms_out[h] (heights 1, 2,
+  // 4) receives the better pairing of each height.
   int autotune(int nsweeps, double* ms_out) {
     if (nsweeps < 1) throw Error{LBMX_ERR_ARG, "nsweeps must be >= 1"};
-    const int cand[3] = {1, 2, 4};
+    const int cand_ty[5] = {1, 2, 4, 2, 4};
+    const bool cand_pair[5] = {true, true, true, false, false};
     double best = 1e300;
     int best_ty = ty;
+    bool best_pair = pair;
+    if (ms_out)
+      for (int h = 0; h < 3; ++h) ms_out[h] = 1e300;
     EventPair ev;
     hipEvent_t e0 = ev.e0, e1 = ev.e1;
-    for (int c = 0; c < 3; ++c) {
-      ty = cand[c];
+    for (int c = 0; c < 5; ++c) {
+      ty = cand_ty[c];
+      pair = cand_pair[c];
       build();
       ensure_pool(2);
       lam_a.ensure(lam_total);
@@ -1319,14 +1392,22 @@ struct Batch {
       HIP_OK(hipEventSynchronize(e1));
       float ms = 0;
       HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms_out) ms_out[c] = ms / nsweeps;
+      const int h = ty == 1 ? 0 : ty == 2 ? 1 : 2;
+      if (ms_out) ms_out[h] = std::min(ms_out[h], double(ms) / nsweeps);
       if (ms < best) {
         best = ms;
-        best_ty = cand[c];
+        best_ty = ty;
+        best_pair = pair;
       }
     }
     ty = best_ty;
-    for (Engine* e : m) e->state_kind = -1;
+    pair = best_pair;
+    // the timed sweeps overwrote every member's state and accumulated into its dJ/dgamma array
+    for (Engine* e : m) {
+      e->state_kind = -1;
+      e->sens_valid = false;
+      e->adj_valid = false;
+    }
     return ty;
   }
 
@@ -1424,6 +1505,13 @@ lbmx_status lbmx_add_boundary(lbmx_solver* s, const lbmx_box* box, const lbmx_bc
   CHECK_HANDLE(s);
   if (!box || !bc) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   return guarded([&] { s->e->add_boundary(*box, *bc); });
+}
+
+lbmx_status lbmx_add_boundary_table(lbmx_solver* s, const lbmx_box* box, const lbmx_bc_desc* bc, const double* profile,
+                                    size_t nprofile, int ncomp, const double* time_factors, size_t nt) {
+  CHECK_HANDLE(s);
+  if (!box || !bc) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->add_boundary_table(*box, *bc, profile, nprofile, ncomp, time_factors, nt); });
 }
 
 lbmx_status lbmx_set_design(lbmx_solver* s, const double* gamma, size_t n) {
@@ -1695,6 +1783,25 @@ lbmx_status lbmx_batch_set_tile_rows(lbmx_batch* b, int rows) {
     return LBMX_ERR_ARG;
   }
   b->b->ty = rows;
+  b->b->pair = true;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_batch_set_tile(lbmx_batch* b, int rows, int paired) {
+  CHECK_BATCH(b);
+  if (rows != 1 && rows != 2 && rows != 4) {
+    lbmx::g_last_error = "tile rows must be 1, 2 or 4";
+    return LBMX_ERR_ARG;
+  }
+  b->b->ty = rows;
+  b->b->pair = paired != 0;
+  return LBMX_OK;
+}
+
+lbmx_status lbmx_batch_get_tile(lbmx_batch* b, int* rows, int* paired) {
+  CHECK_BATCH(b);
+  if (rows) *rows = b->b->ty;
+  if (paired) *paired = (b->b->ty > 1 && b->b->pair) ? 1 : 0;
   return LBMX_OK;
 }
 

@@ -32,6 +32,12 @@ struct DevBC {
   double center[2];
   double radius[2];
   double t_period, t_amp, t_offset, t_phase;
+  // LBMX_PROFILE_TABLE: per-node values over the box tbox of the lbmx_add_boundary_table call,
+  // natural order, ncomp per node (1: a scalar shape; D: one factor per velocity component)
+  const double* table;
+  int tbox[6];
+  int ncomp;
+  int pad0;
 };
 
 // One functional term: weight * sum over box of g_kind(state); kinds:

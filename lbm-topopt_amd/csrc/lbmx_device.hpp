@@ -348,8 +348,15 @@ template <class T> __device__ __forceinline__ T ramp_diff(T p, T ck, T s) {
 // ---------------------------------------------------------------------------------------
 // prescribed value of a piece at node (x,y,z): value * profile(x,y,z) * tf, where tf is the
 // piece's time factor at the current step (computed on the host, see tfac in the sweep args)
+// (LBMX_PROFILE_TABLE: the factor comes from the piece's per-node table; comp picks the
+// component of a table with one factor per velocity component)
 template <class T>
-__device__ __forceinline__ T bc_value(const DevBC& b, int D, T tf, int x, int y, int z) {
+__device__ __forceinline__ T bc_value(const DevBC& b, int D, T tf, int x, int y, int z, int comp = 0) {
+  if (b.profile == LBMX_PROFILE_TABLE) {
+    const long long i = ((long long)(z - b.tbox[4]) * (b.tbox[3] - b.tbox[2] + 1) + (y - b.tbox[2])) *
+                            (b.tbox[1] - b.tbox[0] + 1) + (x - b.tbox[0]);
+    return T(b.value) * T(b.table[i * b.ncomp + (b.ncomp > 1 ? comp : 0)]) * tf;
+  }
   T s, tt;
   if (D == 2) { s = T(b.nrm[0] != 0 ? y : x); tt = 0; }
   else if (b.nrm[0] != 0) { s = T(y); tt = T(z); }
@@ -603,8 +610,9 @@ __device__ __forceinline__ void bc_node(T (&fio)[L::Q], const DevBC& b, T tf, in
     case LBMX_BC_BOUNCEBACK: bc_bounceback<L>(f, b.nrm); break;
     case LBMX_BC_ZOUHE_VELOCITY:
       if constexpr (L::D == 2) {
-        T v = bc_value<T>(b, 2, tf, x, y, z);
-        zh_velocity<T>(f, b.nrm, b.inc, v * T(b.dir[0]), v * T(b.dir[1]));
+        const T ux = bc_value<T>(b, 2, tf, x, y, z, 0) * T(b.dir[0]);
+        const T uy = bc_value<T>(b, 2, tf, x, y, z, 1) * T(b.dir[1]);
+        zh_velocity<T>(f, b.nrm, b.inc, ux, uy);
       }
       break;
     case LBMX_BC_ZOUHE_PRESSURE:
@@ -612,8 +620,10 @@ __device__ __forceinline__ void bc_node(T (&fio)[L::Q], const DevBC& b, T tf, in
       break;
     case LBMX_BC_HECHT_VELOCITY:
       if constexpr (L::D == 3) {
-        T v = bc_value<T>(b, 3, tf, x, y, z);
-        hecht_velocity<T>(f, b.inc, v * T(b.dir[0]), v * T(b.dir[1]), v * T(b.dir[2]));
+        const T ux = bc_value<T>(b, 3, tf, x, y, z, 0) * T(b.dir[0]);
+        const T uy = bc_value<T>(b, 3, tf, x, y, z, 1) * T(b.dir[1]);
+        const T uz = bc_value<T>(b, 3, tf, x, y, z, 2) * T(b.dir[2]);
+        hecht_velocity<T>(f, b.inc, ux, uy, uz);
       }
       break;
     case LBMX_BC_HECHT_PRESSURE:

@@ -21,8 +21,8 @@ FP64, FP32, FP16_SHIFTED, BF16_SHIFTED = 0, 1, 2, 3
 BC_BOUNCEBACK, BC_ZOUHE_VELOCITY, BC_ZOUHE_PRESSURE, BC_ZOUHE_NEUMANN = 0, 1, 2, 3
 BC_HECHT_VELOCITY, BC_HECHT_PRESSURE, BC_THERMAL_BOUNCEBACK, BC_THERMAL_DIRICHLET = 4, 5, 6, 7
 BC_THERMAL_NEUMANN = 8
-PROFILE_UNIFORM, PROFILE_PARABOLIC, PROFILE_PARABOLIC_DISK, PROFILE_PARABOLIC_DUCT = 0, 1, 2, 3
-TIME_CONSTANT, TIME_SINE = 0, 1
+PROFILE_UNIFORM, PROFILE_PARABOLIC, PROFILE_PARABOLIC_DISK, PROFILE_PARABOLIC_DUCT, PROFILE_TABLE = 0, 1, 2, 3, 4
+TIME_CONSTANT, TIME_SINE, TIME_TABLE = 0, 1, 2
 OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX, OBJ_THROUGHFLOW = 0, 1, 2, 3, 4, 5
 
 LBMX_OK, ERR_ARG, ERR_SUP, ERR_MEM, ERR_STATE, ERR_DEVICE = 0, 62, 56, 55, 73, 76
@@ -140,6 +140,7 @@ def library():
         "lbmx_destroy": [C.c_void_p],
         "lbmx_set_stream": [C.c_void_p, C.c_void_p],
         "lbmx_add_boundary": [C.c_void_p, P(Box), P(BCDesc)],
+        "lbmx_add_boundary_table": [C.c_void_p, P(Box), P(BCDesc), dp, C.c_size_t, C.c_int, dp, C.c_size_t],
         "lbmx_set_design": [C.c_void_p, dp, C.c_size_t],
         "lbmx_get_design": [C.c_void_p, dp, C.c_size_t],
         "lbmx_set_objective": [C.c_void_p, P(Objective)],
@@ -169,6 +170,8 @@ def library():
         "lbmx_batch_destroy": [C.c_void_p],
         "lbmx_batch_checkpoint_config": [C.c_void_p, C.c_int],
         "lbmx_batch_set_tile_rows": [C.c_void_p, C.c_int],
+        "lbmx_batch_set_tile": [C.c_void_p, C.c_int, C.c_int],
+        "lbmx_batch_get_tile": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "lbmx_batch_set_graph": [C.c_void_p, C.c_int],
         "lbmx_batch_autotune": [C.c_void_p, C.c_int, P(C.c_int), dp],
         "lbmx_batch_gradient": [C.c_void_p, C.c_int, C.c_double, dp, C.c_double, dp, P(dp)],
@@ -240,7 +243,8 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_filter_design_to_solver_device", "lbmx_filter_sensitivities_device",
             "lbmx_filter_solver_sensitivities_device", "lbmx_mma_update_device", "lbmx_mma_design_change_device",
             "lbmx_dmda_global_to_natural", "lbmx_dmda_natural_to_global", "lbmx_checkpoint_mode",
-            "lbmx_checkpoint_simulate_mode", "lbmx_batch_checkpoint_mode"]
+            "lbmx_checkpoint_simulate_mode", "lbmx_batch_checkpoint_mode",
+            "lbmx_batch_set_tile", "lbmx_batch_get_tile", "lbmx_add_boundary_table"]
 
 
 def checkpoint_simulate(nslots, nsteps, mode=CHECKPOINT_DYNAMIC):
@@ -374,6 +378,15 @@ class Solver:
 
     def add_boundary(self, box, bc):
         _check(self.lib.lbmx_add_boundary(self.h, C.byref(box), C.byref(bc)))
+
+    def add_boundary_table(self, box, bc, profile=None, ncomp=1, time_factors=None):
+        """lbmx_add_boundary_table: value * a(i,j,k) * b(t); profile holds ncomp values per node
+        of box (natural order), time_factors b(t) for t = 0..len-1 (either may be None)."""
+        prof = None if profile is None else np.ascontiguousarray(profile, dtype=np.float64).ravel()
+        tf = None if time_factors is None else np.ascontiguousarray(time_factors, dtype=np.float64).ravel()
+        _check(self.lib.lbmx_add_boundary_table(self.h, C.byref(box), C.byref(bc),
+                                                None if prof is None else _dptr(prof), 0 if prof is None else prof.size,
+                                                ncomp, None if tf is None else _dptr(tf), 0 if tf is None else tf.size))
 
     def set_design(self, gamma):
         g = np.ascontiguousarray(gamma, dtype=np.float64).ravel()
@@ -529,6 +542,15 @@ class Batch:
 
     def set_tile_rows(self, rows):
         _check(self.lib.lbmx_batch_set_tile_rows(self.h, rows))
+
+    def set_tile(self, rows, paired):
+        _check(self.lib.lbmx_batch_set_tile(self.h, rows, 1 if paired else 0))
+
+    def tile(self):
+        """(rows, paired) of the tile shape in use."""
+        r, p = C.c_int(0), C.c_int(0)
+        _check(self.lib.lbmx_batch_get_tile(self.h, C.byref(r), C.byref(p)))
+        return r.value, bool(p.value)
 
     def set_graph(self, enable=True):
         _check(self.lib.lbmx_batch_set_graph(self.h, 1 if enable else 0))

@@ -93,6 +93,12 @@ typedef struct {
   int nrm[3];
   lbmx_box box;
   lbmx_bc_desc d;
+  /* value tables of lbmx_add_boundary_table (each piece owns a copy; NULL: none) */
+  double* prof;      /* ncomp values per node of tbox, natural order */
+  lbmx_box tbox;
+  int ncomp;
+  double* tfac;      /* b(t), t = 0..nt-1 */
+  int nt;
 } orc_bc;
 
 typedef struct {
@@ -116,10 +122,12 @@ struct orc_problem {
   double time_const;
   double* gamma;
   size_t nodes;
+  int reference_reverse; /* cascaded reverse as the reference codes it (test comparison only) */
 };
 
 static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+void orc_set_reference_reverse(orc_problem* p, int on) { p->reference_reverse = on != 0; }
 
 /* ------------------------------------------------------------------------- */
 /* MRT bases                                                                  */
@@ -292,6 +300,10 @@ orc_problem* orc_create(const lbmx_desc* d) {
 
 void orc_destroy(orc_problem* p) {
   if (!p) return;
+  for (int k = 0; k < p->nbc; ++k) {
+    free(p->bcs[k].prof);
+    free(p->bcs[k].tfac);
+  }
   free(p->gamma);
   free(p->bcs);
   free(p);
@@ -383,7 +395,15 @@ static int intersect(const lbmx_box* a, const lbmx_box* b, lbmx_box* o) {
 }
 
 int orc_add_boundary(orc_problem* p, const lbmx_box* box, const lbmx_bc_desc* bc) {
+  return orc_add_boundary_table(p, box, bc, NULL, 0, 0, NULL, 0);
+}
+
+int orc_add_boundary_table(orc_problem* p, const lbmx_box* box, const lbmx_bc_desc* bc, const double* prof,
+                           long long nprof, int ncomp, const double* tfac, int nt) {
   int X1 = p->d.nx - 1, Y1 = p->d.ny - 1, Z1 = p->d.nz - 1;
+  long long bn = (long long)(box->x1 - box->x0 + 1) * (box->y1 - box->y0 + 1) * (box->z1 - box->z0 + 1);
+  if (prof && (ncomp < 1 || nprof != bn * ncomp)) return LBMX_ERR_ARG;
+  if (tfac && nt < 1) return LBMX_ERR_ARG;
   /* pieces in the reference's order; each as (box, normal) */
   lbmx_box pb[26];
   int pn[26][3];
@@ -450,6 +470,19 @@ int orc_add_boundary(orc_problem* p, const lbmx_box* box, const lbmx_bc_desc* bc
     e->nrm[0] = pn[k][0]; e->nrm[1] = pn[k][1]; e->nrm[2] = pn[k][2];
     e->box = o;
     e->d = *bc;
+    e->prof = NULL;
+    e->tfac = NULL;
+    e->tbox = *box;
+    e->ncomp = ncomp;
+    e->nt = nt;
+    if (prof) {
+      e->prof = (double*)malloc(nprof * sizeof(double));
+      memcpy(e->prof, prof, nprof * sizeof(double));
+    }
+    if (tfac) {
+      e->tfac = (double*)malloc(nt * sizeof(double));
+      memcpy(e->tfac, tfac, nt * sizeof(double));
+    }
   }
   return 0;
 }
@@ -545,8 +578,8 @@ static void casc_k(const double* f, double r, double x, double y, double om, dou
    the reference's ReverseIncompressibleCascaded2d, src/adjointDynamics/reverseCascaded2d.hh,
    is not this transpose): lf = lg + (dk/df)^T T^T lg for the explicit f terms; the parts
    through rho and u are returned in *rb, vb[0..1]. */
-static void casc_reverse(const double* f, double r, double x, double y, double om, const double* lg,
-                         double* lf, double* rb, double* vb) {
+static void casc_reverse(const double* f, double r, double x, double y, double om, int as_coded,
+                         const double* lg, double* lf, double* rb, double* vb) {
   double k[6];
   casc_k(f, r, x, y, om, k);
   double kb[6] = {0, 0, 0, 0, 0, 0};
@@ -581,6 +614,19 @@ static void casc_reverse(const double* f, double r, double x, double y, double o
     kb[1] += b * (0.5 * (x * x - y * y));
     x_b += b * (-2. * k[4] + 4. * y * k[2] - 3. * k[0] * x + k[1] * x);
     y_b += b * (-2. * k[3] + 4. * x * k[2] - 3. * k[0] * y - k[1] * y);
+    if (as_coded) {
+      /* ReverseIncompressibleCascaded2d (src/adjointDynamics/reverseCascaded2d.hh:27-133) is this
+         transpose except in one place: the coefficients -2 k4 and -2 k3 of dk5/du_x and dk5/du_y
+         (the adjoint statements at :65 and :69) use its own recomputation of k3 and k4 (:45-52),
+         which lacks the forward's /4 on every term but the r u^2 one (cascaded2d.hh:53-62). That
+         recomputation exceeds the forward's k3 and k4 by d3 and d4 below, so the reference's
+         adjoint of u_x and u_y picks up -2 d4 b and -2 d3 b; its adjoint statements for k3 and
+         k4 themselves carry the forward's /4 (:81, :94) and need no change. */
+      const double d3 = -0.75 * (f[1] + f[3] - f[7] - f[5] + y * (r - f[0] - f[2] - f[6]));
+      const double d4 = -0.75 * (f[3] + f[5] - f[1] - f[7] + x * (r - f[0] - f[4] - f[8]));
+      x_b += b * (-2. * d4);
+      y_b += b * (-2. * d3);
+    }
   }
   /* k4 = -(P4/4 + y Q4/2) + x(-3 k0 + k1)/2 + 2 y k2 */
   {
@@ -840,7 +886,7 @@ void orc_reverse_collide_node(const orc_problem* p, const double* fin, double ga
   double rho_bar = 0, v_bar[3] = {0, 0, 0}, alpha_bar = 0;
   double lf[QMAX];
   if (p->d.collision == LBMX_CASCADED) {
-    casc_reverse(fin, rho, v[0], v[1], om, lg, lf, &rho_bar, v_bar);
+    casc_reverse(fin, rho, v[0], v[1], om, p->reference_reverse, lg, lf, &rho_bar, v_bar);
   } else if (p->d.collision == LBMX_BGK) {
     for (int q = 0; q < Q; ++q) {
       double cq[3] = {p->ex[q], p->ey[q], p->ez[q]};
@@ -954,11 +1000,16 @@ static void tangential(const orc_problem* p, const int* n, int i, int j, int k, 
   else { *s = i; *t = j; }
 }
 
-static double bc_value(const orc_problem* p, const orc_bc* e, int t, int i, int j, int k) {
+/* value * a(i,j,k) * b(t) of a piece; comp: the velocity component of a per-component table */
+static double bc_value(const orc_problem* p, const orc_bc* e, int t, int i, int j, int k, int comp) {
   const lbmx_bc_desc* d = &e->d;
   double s, tt, shape = 1.;
   tangential(p, e->nrm, i, j, k, &s, &tt);
-  switch (d->profile) {
+  if (e->prof) {
+    long long n = ((long long)(k - e->tbox.z0) * (e->tbox.y1 - e->tbox.y0 + 1) + (j - e->tbox.y0)) *
+                      (e->tbox.x1 - e->tbox.x0 + 1) + (i - e->tbox.x0);
+    shape = e->prof[n * e->ncomp + (e->ncomp > 1 ? comp : 0)];
+  } else switch (d->profile) {
     case LBMX_PROFILE_PARABOLIC: {
       double R = d->radius[0], x = s - d->center[0];
       shape = (R * R - x * x) / (R * R);
@@ -975,7 +1026,9 @@ static double bc_value(const orc_problem* p, const orc_bc* e, int t, int i, int 
     default: break;
   }
   double tf = 1.;
-  if (d->time_mode == LBMX_TIME_SINE)
+  if (e->tfac)
+    tf = e->tfac[t < e->nt ? t : e->nt - 1];
+  else if (d->time_mode == LBMX_TIME_SINE)
     tf = d->t_offset + d->t_amp * sin(2. * ORC_PI * t / d->t_period + d->t_phase);
   return d->value * shape * tf;
 }
@@ -1013,8 +1066,7 @@ static void bc_node(const orc_problem* p, const orc_bc* e, int t, int i, int j, 
   switch (e->type) {
     case LBMX_BC_BOUNCEBACK: bounceback_node(p, n, f); break;
     case LBMX_BC_ZOUHE_VELOCITY: {
-      double v = bc_value(p, e, t, i, j, k);
-      double ux = v * e->d.dir[0], uy = v * e->d.dir[1];
+      double ux = bc_value(p, e, t, i, j, k, 0) * e->d.dir[0], uy = bc_value(p, e, t, i, j, k, 1) * e->d.dir[1];
       if (n[0] == -1 && inc) { /* IncompressibleZouHeVelocityWest2d (ZouHe2d.hh:29-44) */
         f[1] = 0.5 * (f[6] - f[2]) + f[5] + ux / 6. - 0.5 * uy;
         f[7] = 0.5 * (f[2] - f[6]) + f[3] + ux / 6. + 0.5 * uy;
@@ -1035,7 +1087,7 @@ static void bc_node(const orc_problem* p, const orc_bc* e, int t, int i, int j, 
       }
     } break;
     case LBMX_BC_ZOUHE_PRESSURE: {
-      double r0 = bc_value(p, e, t, i, j, k);
+      double r0 = bc_value(p, e, t, i, j, k, 0);
       if (n[0] == 1) { /* east: ZouHe2d.hh:252-270 (inc), standard variant */
         double s = f[0] + f[2] + f[6] + 2. * (f[1] + f[7] + f[8]);
         double u0 = inc ? -r0 + s : -1. + s / r0;
@@ -1067,8 +1119,8 @@ static void bc_node(const orc_problem* p, const orc_bc* e, int t, int i, int j, 
       }
     } break;
     case LBMX_BC_HECHT_VELOCITY: { /* {Inc,Std}HechtVelocityWest3d (Hecht3d.hh:26-97) */
-      double v = bc_value(p, e, t, i, j, k);
-      double ux = v * e->d.dir[0], uy = v * e->d.dir[1], uz = v * e->d.dir[2];
+      double ux = bc_value(p, e, t, i, j, k, 0) * e->d.dir[0], uy = bc_value(p, e, t, i, j, k, 1) * e->d.dir[1],
+             uz = bc_value(p, e, t, i, j, k, 2) * e->d.dir[2];
       static const int known[9] = {0, 1, 2, 4, 5, 10, 11, 13, 14};
       static const int outg[5] = {3, 6, 8, 16, 18};
       double rho = inc ? 1. : (sum_idx(f, known, 9) + 2. * sum_idx(f, outg, 5)) / (1. - ux);
@@ -1081,7 +1133,7 @@ static void bc_node(const orc_problem* p, const orc_bc* e, int t, int i, int j, 
       f[17] = f[8] + rho * (ux + uy) / 6. - Nyx;
     } break;
     case LBMX_BC_HECHT_PRESSURE: { /* {Inc,Std}HechtPressureEast3d (Hecht3d.hh:129-204) */
-      double r0 = bc_value(p, e, t, i, j, k);
+      double r0 = bc_value(p, e, t, i, j, k, 0);
       static const int known[9] = {0, 1, 2, 4, 5, 10, 11, 13, 14};
       static const int outg[5] = {7, 9, 12, 15, 17};
       double s = sum_idx(f, known, 9) + 2. * sum_idx(f, outg, 5);
@@ -1114,7 +1166,7 @@ static void bc_node(const orc_problem* p, const orc_bc* e, int t, int i, int j, 
     } break;
     case LBMX_BC_THERMAL_DIRICHLET: { /* ThermalZouHeWest2d<.,D2Q5> (thermalZouHe2d.hh:62-93) */
       double* g = f + p->Qf;
-      double T = bc_value(p, e, t, i, j, k);
+      double T = bc_value(p, e, t, i, j, k, 0);
       int unk = 0;
       for (int q = 1; q < 5; ++q)
         if (D2Q5_EX[q] * n[0] + D2Q5_EY[q] * n[1] < 0) unk = q;

@@ -22,6 +22,10 @@ typedef struct orc_problem orc_problem;
 orc_problem* orc_create(const lbmx_desc* d);
 void orc_destroy(orc_problem* p);
 int orc_add_boundary(orc_problem* p, const lbmx_box* box, const lbmx_bc_desc* bc);
+/* the same with value tables (lbmx_add_boundary_table): prof (ncomp per node of box) and tfac
+   (b(t), t = 0..nt-1) may be NULL */
+int orc_add_boundary_table(orc_problem* p, const lbmx_box* box, const lbmx_bc_desc* bc, const double* prof,
+                           long long nprof, int ncomp, const double* tfac, int nt);
 int orc_num_boundaries(const orc_problem* p);
 /* piece k: type, normal (3 ints), box (6 ints) */
 int orc_boundary_info(const orc_problem* p, int k, int* type, int* nrm, int* box);
@@ -29,6 +33,10 @@ int orc_set_design(orc_problem* p, const double* gamma);
 int orc_set_objective(orc_problem* p, const lbmx_objective* obj);
 int orc_num_dof(const orc_problem* p);
 void orc_set_threads(int n);
+/* on != 0: the cascaded reverse collision is the reference's ReverseIncompressibleCascaded2d as
+   coded (src/adjointDynamics/reverseCascaded2d.hh:27-133), which is not the transpose of the
+   forward cascaded collision; for reporting the reference adjoint's deviation only */
+void orc_set_reference_reverse(orc_problem* p, int on);
 
 /* ---- node-level primitives (for unit tests) ---- */
 void orc_lattice(const orc_problem* p, int* Q, int* ex, int* ey, int* ez, int* opp, double* w);

@@ -38,12 +38,14 @@ def library():
         "orc_create": ([P(lbmx.Desc)], vp),
         "orc_destroy": ([vp], None),
         "orc_add_boundary": ([vp, P(lbmx.Box), P(lbmx.BCDesc)], C.c_int),
+        "orc_add_boundary_table": ([vp, P(lbmx.Box), P(lbmx.BCDesc), dp, C.c_longlong, C.c_int, dp, C.c_int], C.c_int),
         "orc_num_boundaries": ([vp], C.c_int),
         "orc_boundary_info": ([vp, C.c_int, ip, ip, ip], C.c_int),
         "orc_set_design": ([vp, dp], C.c_int),
         "orc_set_objective": ([vp, P(lbmx.Objective)], C.c_int),
         "orc_num_dof": ([vp], C.c_int),
         "orc_set_threads": ([C.c_int], None),
+        "orc_set_reference_reverse": ([vp, C.c_int], None),
         "orc_lattice": ([vp, ip, ip, ip, ip, ip, dp], None),
         "orc_collide_node": ([vp, dp, dp, dp, C.c_double], None),
         "orc_reverse_collide_node": ([vp, dp, C.c_double, dp, dp, dp], None),
@@ -107,6 +109,10 @@ class Oracle:
         except Exception:
             pass
 
+    def set_reference_reverse(self, on=True):
+        """Cascaded reverse collision as the reference codes it (not the true transpose)."""
+        self.lib.orc_set_reference_reverse(self.h, 1 if on else 0)
+
     def lattice(self):
         Q = C.c_int()
         ex = (C.c_int * 24)(); ey = (C.c_int * 24)(); ez = (C.c_int * 24)(); op = (C.c_int * 24)()
@@ -120,6 +126,15 @@ class Oracle:
         code = self.lib.orc_add_boundary(self.h, C.byref(box), C.byref(bc))
         if code:
             raise lbmx.LbmxError(code, "oracle add_boundary")
+
+    def add_boundary_table(self, box, bc, profile=None, ncomp=1, time_factors=None):
+        prof = None if profile is None else np.ascontiguousarray(profile, dtype=np.float64).ravel()
+        tf = None if time_factors is None else np.ascontiguousarray(time_factors, dtype=np.float64).ravel()
+        code = self.lib.orc_add_boundary_table(self.h, C.byref(box), C.byref(bc),
+                                               None if prof is None else _d(prof), 0 if prof is None else prof.size,
+                                               ncomp, None if tf is None else _d(tf), 0 if tf is None else tf.size)
+        if code:
+            raise lbmx.LbmxError(code, "oracle add_boundary_table")
 
     def boundaries(self):
         out = []
