@@ -134,7 +134,8 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement):
     b = lbmx.Batch(members)
     b.checkpoint_config(slots, mode)
     rows, tune_ms = b.autotune(10)
-    log(f"ensemble: {len(members)} members, {nodes} nodes, autotuned tile rows {rows} "
+    paired = b.tile()[1]
+    log(f"ensemble: {len(members)} members, {nodes} nodes, autotuned tile rows {rows}{' paired' if paired else ''} "
         f"(ms per fwd+adj sweep for rows 1/2/4: {', '.join(f'{v:.3f}' for v in tune_ms)})")
     for _ in range(warmup):
         b.gradient(steps, want_sens=False)
@@ -160,14 +161,18 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement):
     Js = [m.gradient(steps)[0] for m in members]
     ts = time.perf_counter() - t0
     log(f"ensemble: sequential single-instance gradients {ts:.2f} s")
-    jerr = max(max(abs(a - c) / abs(c) for a, c in zip(Jb, Js)), max(abs(a - c) / abs(c) for a, c in zip(Jg, Js)))
+    Js = np.array(Js)
+    if not (np.all(np.isfinite(Js)) and np.all(np.isfinite(Jb)) and np.all(np.isfinite(Jg))):
+        raise RuntimeError(f"ensemble objective not finite: batched {Jb}, single {Js}")
+    jerr = float(max(np.max(np.abs(Jb - Js) / np.abs(Js)), np.max(np.abs(Jg - Js) / np.abs(Js))))
     dev = b.device_bytes()
     b.close()
     for m in members:
         m.close()
     return {"members": len(members), "nodes": nodes, "time_steps": steps, "checkpoints": slots,
             "placement": placement,
-            "tile_rows": rows, "autotune_ms_rows_1_2_4": [round(v, 4) for v in tune_ms],
+            "tile_rows": rows, "tile_forward_paired": paired,
+            "autotune_ms_rows_1_2_4": [round(v, 4) for v in tune_ms],
             "batched_s": round(tb, 3), "batched_graph_s": round(tg, 3), "sequential_s": round(ts, 3),
             "batched_mlups": round(nodes * steps / tb / 1e6, 1),
             "batched_graph_mlups": round(nodes * steps / tg / 1e6, 1),
@@ -325,33 +330,234 @@ def read_traffic(key):
     return None
 
 
-def cpu_baseline(workloads, lbmx, nx=96, ny=48, nz=48, horizon=30):
-    """The CPU oracle (port of the reference's algorithm) on a bounded sample of the workload,
-    on the host's cores (up to 16) and on one core (a smaller sample)."""
+# The workloads of the reference's benchmark configurations besides the headline duct:
+# (name, lattice, collision, grid, tau or omega, time steps, checkpoint slots, setup, initial state).
+def other_workloads(lbmx, workloads):
+    def ch256(obj, nx, ny, steps):
+        workloads.channel2d(obj, nx, ny, time_const=1.0 / steps)
+
+    def ch1024(obj, nx, ny, steps):
+        workloads.modulated_channel(obj, nx, ny, steps=steps)
+
+    def th2048(obj, nx, ny, steps):
+        workloads.thermal_channel2d(obj, nx, ny, time_const=1.0 / steps)
+
+    bgk = dict(lattice=lbmx.D2Q9, collision=lbmx.BGK, design=lbmx.DESIGN_BRINKMAN, omega=1.0 / 0.8, interp_q=0.1,
+               alpha_max=0.1)
+    th = dict(lattice=lbmx.D2Q9_D2Q5, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN, omega=1.25, interp_q=0.1,
+              alpha_max=0.1, thermal_diffusivity=0.02, ramp_p=1.0, ramp_ck=0.1)
+    return [
+        {"name": "channel_256x128_bgk_brinkman_forward", "grid": (256, 128), "desc": bgk, "steps": 2000, "slots": None,
+         "setup": ch256, "init": {}, "Q": 9},
+        {"name": "channel_1024x512_bgk_brinkman_modulated", "grid": (1024, 512), "desc": bgk, "steps": 5000,
+         "slots": 64, "setup": ch1024, "init": {}, "Q": 9},
+        {"name": "thermal_2048x1024_mrt", "grid": (2048, 1024), "desc": th, "steps": 4000, "slots": 64,
+         "setup": th2048, "init": {"u0": (0.03, 0.0, 0.0), "T0": 0.5}, "Q": 14},
+    ]
+
+
+def run_other_workloads(lbmx, workloads, mode, sweeps=50):
+    """Forward and adjoint sweep rates (HIP events) and the whole pass of the other benchmark
+    workloads, FP64 and FP16-shifted forms. The 256 x 128 channel is a forward run; the others
+    are design gradients with checkpoints. Their grids are small enough to be partly
+    L2-resident, so their GB/s are algorithmic bytes over time, not HBM traffic."""
+    out = {}
+    for w in other_workloads(lbmx, workloads):
+        nx, ny = w["grid"]
+        N = nx * ny
+        rec = {}
+        for form, prec in (("fp64", lbmx.FP64), ("fp16", lbmx.FP16_SHIFTED)):
+            s = lbmx.Solver(lbmx.make_desc(nx, ny, precision=prec, **w["desc"]))
+            w["setup"](s, nx, ny, w["steps"])
+            s.set_design(workloads.design_field(nx, ny, 1, seed=0))
+            bf, ba = bytes_per_lu(form, Q=w["Q"])
+            s.init_equilibrium(**w["init"])
+            s.time_sweeps(0, 5)
+            fms = s.time_sweeps(0, sweeps) / sweeps
+            r = {"fwd_sweep_ms": round(fms, 5), "mlups_fwd_kernel": round(N / (fms * 1e3), 1),
+                 "fwd_gbs": round(bf * N / (fms * 1e-3) / 1e9, 1), "fwd_bytes_per_lu": bf}
+            if w["slots"] is None:
+                s.init_equilibrium(**w["init"])
+                t0 = time.perf_counter()
+                J = s.forward(w["steps"])
+                dt = time.perf_counter() - t0
+                r.update({"forward_run_s": round(dt, 4), "mlups_forward_run": round(N * w["steps"] / dt / 1e6, 1),
+                          "objective": J})
+            else:
+                s.time_sweeps(1, 3)
+                ams = s.time_sweeps(1, sweeps) / sweeps
+                r.update({"adj_sweep_ms": round(ams, 5), "mlups_adj_kernel": round(N / (ams * 1e3), 1),
+                          "adj_gbs": round(ba * N / (ams * 1e-3) / 1e9, 1), "adj_bytes_per_lu": ba})
+                s.checkpoint_config(w["slots"], mode)
+                s.gradient(w["steps"], want_sens=False, **w["init"])  # warm-up
+                t0 = time.perf_counter()
+                J, g = s.gradient(w["steps"], **w["init"])
+                dt = time.perf_counter() - t0
+                st = s.stats()
+                r.update({"gradient_s": round(dt, 4), "mlups_fwd_adj": round(N * w["steps"] / dt / 1e6, 1),
+                          "mlups_forward_phase": round(N * w["steps"] / (st["forward_phase_ms"] / 1e3) / 1e6, 1),
+                          "mlups_reverse_phase": round(N * w["steps"] / (st["reverse_phase_ms"] / 1e3) / 1e6, 1),
+                          "objective": J, "_sens": g})
+            s.close()
+            rec[form] = r
+            log(f"{w['name']} {form}: forward sweep {fms:.4f} ms"
+                + (f", adjoint sweep {r['adj_sweep_ms']:.4f} ms, gradient {r['gradient_s']:.3f} s"
+                   if "gradient_s" in r else f", {w['steps']}-step run {r['forward_run_s']:.3f} s"))
+        if "_sens" in rec["fp64"]:
+            g64, g16 = rec["fp64"].pop("_sens"), rec["fp16"].pop("_sens")
+            J64, J16 = rec["fp64"]["objective"], rec["fp16"]["objective"]
+            rec["fp16"]["vs_fp64"] = {"objective_rel_err": abs(J16 - J64) / abs(J64),
+                                      "gradient_rel_err_max_norm": float(np.max(np.abs(g16 - g64)) / np.max(np.abs(g64))),
+                                      "mlups_ratio": round(rec["fp16"]["mlups_fwd_adj"] / rec["fp64"]["mlups_fwd_adj"], 3)}
+        out[w["name"]] = {"nodes": N, "time_steps": w["steps"], "checkpoints": w["slots"], **rec}
+    return out
+
+
+def _host_cores():
+    """(logical CPUs, physical cores per socket, sockets) from /proc/cpuinfo."""
+    logical, cores, sockets = 0, None, set()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "cpu cores" and cores is None:
+                    cores = int(v)
+                elif k == "physical id":
+                    sockets.add(v.strip())
+    except OSError:
+        pass
+    return logical, cores, max(1, len(sockets))
+
+
+def reference_adjoint_report(lbmx, workloads, oracle, threads, steps=3000, ncells=8):
+    """The reference's own adjoint of the cascaded collision, run as it is coded (the CPU
+    oracle's as-coded variant of ReverseIncompressibleCascaded2d), on one member of the robust
+    ensemble at its full horizon (256 x 128, eta 0.5, PBB<cascaded> at tau 0.6, FP64), against
+    the engine's gradient (the true transpose) and central differences of J on the GPU."""
+    nx, ny = 256, 128
+    d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9, collision=lbmx.CASCADED, design=lbmx.DESIGN_PBB, omega=1.0 / 0.6,
+                       interp_q=0.1, alpha_max=1.0)
+    g = workloads.heaviside_project(workloads.ensemble_candidate(nx, ny, 1000), 8.0, 0.5)
+    s = lbmx.Solver(d)
+    workloads.channel2d(s, nx, ny, time_const=1.0 / steps)
+    s.set_design(g)
+    s.checkpoint_config(64, lbmx.CHECKPOINT_BINOMIAL)
+    J, sens = s.gradient(steps)
+    o = oracle.Oracle(d, threads=threads)
+    workloads.channel2d(o, nx, ny, time_const=1.0 / steps)
+    o.set_design(g)
+    o.set_reference_reverse(True)
+    t0 = time.perf_counter()
+    _, sref, _ = o.gradient(steps)
+    t_cpu = time.perf_counter() - t0
+    del o
+    scale = float(np.max(np.abs(sens)))
+    cells = np.concatenate([np.argsort(-np.abs(sref - sens))[:ncells // 2],
+                            np.random.default_rng(0).choice(g.size, ncells - ncells // 2, replace=False)])
+    h = 1e-5
+    fd = []
+    for c in cells:
+        v = []
+        for sgn in (1, -1):
+            gg = g.copy()
+            gg[c] += sgn * h
+            s.set_design(gg)
+            s.init_equilibrium()
+            v.append(s.forward(steps))
+        fd.append((v[0] - v[1]) / (2 * h))
+    s.close()
+    fd = np.array(fd)
+    r = {"member": "256x128 channel, eta 0.5, PBB<IncompressibleCascaded2d> tau 0.6, FP64", "steps": steps,
+         "reference_vs_true_rel_max": float(np.max(np.abs(sref - sens)) / scale),
+         "true_vs_central_fd_rel_max": float(np.max(np.abs(sens[cells] - fd)) / scale),
+         "reference_vs_central_fd_rel_max": float(np.max(np.abs(sref[cells] - fd)) / scale),
+         "fd_cells": int(len(cells)), "oracle_gradient_s": round(t_cpu, 2)}
+    log(f"reference adjoint on an ensemble member: vs true {r['reference_vs_true_rel_max']:.2e}; vs central FD: "
+        f"true {r['true_vs_central_fd_rel_max']:.1e}, reference {r['reference_vs_central_fd_rel_max']:.1e}")
+    return r
+
+
+def cpu_baseline(workloads, lbmx, horizon_head=3000, slots_head=48, mode_name="binomial"):
+    """The CPU oracle (the reference's algorithm restated, FP64, OpenMP over nodes) on the
+    host's cores, on every benchmark workload at its full grid and a reduced horizon: forward
+    MLUPS from forward steps, reverse-phase MLUPS from the reverse steps of a gradient whose
+    states are all kept, and the design-gradient rate of the full horizon with the engine's
+    checkpoint schedule (same slots and placement, its recomputation counted from the schedule's
+    host replay), modelled from those measured per-step times."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: E402  (baseline leg only)
 
-    def run(threads, gx, gy, gz):
-        d = lbmx.make_desc(gx, gy, gz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
-                           omega=1.6, interp_q=0.1, alpha_max=0.1)
-        o = oracle.Oracle(d, threads=threads)
-        workloads.duct3d(o, gx, gy, gz, u_in=0.04, time_const=1.0 / horizon)
-        o.set_design(workloads.design_field(gx, gy, gz, seed=0))
-        t0 = time.perf_counter()
-        o.gradient(horizon)
-        dt = time.perf_counter() - t0
-        return gx * gy * gz * horizon / dt / 1e6, dt
-
+    logical, cores_per_socket, sockets = _host_cores()
+    # the host CPU share that one GPU of the box comes with is 16 threads
     threads = min(16, os.cpu_count() or 1)
-    v, dt = run(threads, nx, ny, nz)
-    log(f"cpu baseline: {v:.3f} MLUPS forward+adjoint on {threads} threads ({dt:.1f} s)")
-    v1, dt1 = run(1, nx // 2, ny // 2, nz // 2)
-    log(f"cpu baseline: {v1:.3f} MLUPS forward+adjoint on 1 thread ({dt1:.1f} s)")
-    return {"value": v, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"D3Q19 MRT Brinkman duct {nx}x{ny}x{nz}, {horizon} steps forward+adjoint "
-                      f"(every state stored), FP64, OpenMP over nodes",
-            "single_core": {"value": v1, "sample": f"{nx // 2}x{ny // 2}x{nz // 2}, {horizon} steps"},
-            "cpu": _cpu_model()}
+    mode = {"binomial": lbmx.CHECKPOINT_BINOMIAL, "dynamic": lbmx.CHECKPOINT_DYNAMIC,
+            "static": lbmx.CHECKPOINT_STATIC}[mode_name]
+
+    def measure(desc, grid, setup, steps_full, slots, init, kf, kg, threads_used):
+        nx, ny, nz = grid
+        N = nx * ny * nz
+        o = oracle.Oracle(desc, threads=threads_used)
+        setup(o, nx, ny, nz, steps_full)
+        o.set_design(workloads.design_field(nx, ny, nz, seed=0))
+        o.init_equilibrium(**init)
+        t0 = time.perf_counter()
+        o.forward(kf)
+        tf = (time.perf_counter() - t0) / kf
+        r = {"forward_mlups": N / tf / 1e6, "sample": f"{nx}x{ny}x{nz}: {kf} forward steps"}
+        if slots is not None:
+            t0 = time.perf_counter()
+            o.gradient(kg, **init)
+            tg = time.perf_counter() - t0
+            ta = max(tg / kg - tf, 1e-12)
+            rec, _ = lbmx.checkpoint_simulate(slots, steps_full, mode)
+            t_pass = tf * (steps_full + rec) + ta * steps_full
+            r.update({"reverse_step_mlups": N / ta / 1e6,
+                      "reverse_phase_mlups": N * steps_full / (tf * rec + ta * steps_full) / 1e6,
+                      "gradient_mlups_modelled": N * steps_full / t_pass / 1e6,
+                      "recompute_sweeps": rec, "gradient_s_modelled": t_pass,
+                      "sample": r["sample"] + f", a {kg}-step gradient with every state stored; the "
+                                f"{steps_full}-step pass with {slots} slots ({mode_name}) modelled"})
+        return r
+
+    def d3(gx, gy, gz):
+        return lbmx.make_desc(gx, gy, gz, lattice=lbmx.D3Q19, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
+                              omega=1.6, interp_q=0.1, alpha_max=0.1)
+
+    def duct(o, nx, ny, nz, steps):
+        workloads.duct3d(o, nx, ny, nz, u_in=0.04, time_const=1.0 / steps)
+
+    per = {}
+    t_all = time.perf_counter()
+    head = measure(d3(384, 192, 192), (384, 192, 192), duct, horizon_head, slots_head, {}, 3, 3, threads)
+    per["duct_384x192x192_mrt_brinkman"] = head
+    log(f"cpu baseline duct: forward {head['forward_mlups']:.2f} MLUPS, reverse steps {head['reverse_step_mlups']:.2f} "
+        f"MLUPS, modelled gradient {head['gradient_mlups_modelled']:.2f} MLUPS on {threads} threads")
+    sizes = {"channel_256x128_bgk_brinkman_forward": (200, None), "channel_1024x512_bgk_brinkman_modulated": (20, 10),
+             "thermal_2048x1024_mrt": (6, 4)}
+    for w in other_workloads(lbmx, workloads):
+        nx, ny = w["grid"]
+        kf, kg = sizes[w["name"]]
+        r = measure(lbmx.make_desc(nx, ny, **w["desc"]), (nx, ny, 1),
+                    lambda o, a, b, c, st, w=w: w["setup"](o, a, b, st), w["steps"], w["slots"], w["init"], kf,
+                    kg or 1, threads)
+        per[w["name"]] = r
+        log(f"cpu baseline {w['name']}: {', '.join(f'{k} {v:.2f}' for k, v in r.items() if k.endswith('mlups') or k.endswith('modelled'))}")
+    ref_adj = reference_adjoint_report(lbmx, workloads, oracle, threads)
+    one = measure(d3(96, 48, 48), (96, 48, 48), duct, horizon_head, slots_head, {}, 4, 4, 1)
+    log(f"cpu baseline duct on 1 thread (96x48x48): modelled gradient {one['gradient_mlups_modelled']:.3f} MLUPS")
+    return {"value": head["gradient_mlups_modelled"], "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"D3Q19 MRT Brinkman duct 384x192x192, FP64: 3 forward steps and a 3-step gradient "
+                      f"(every state stored) timed; the {horizon_head}-step pass with {slots_head} checkpoints "
+                      f"({mode_name} placement, recomputation counted) modelled from those per-step times",
+            "recompute": True, "modelled": True,
+            "single_core": {"value": one["gradient_mlups_modelled"], "sample": "96x48x48 duct, same model, 1 thread"},
+            "workloads": per, "reference_adjoint": ref_adj, "cpu": _cpu_model(), "host_logical_cpus": logical,
+            "host_cores_per_socket": cores_per_socket, "host_sockets": sockets,
+            "threads_note": "16 threads: the host CPU share that comes with one GPU of the box",
+            "wall_s": round(time.perf_counter() - t_all, 1)}
 
 
 def _cpu_model():
@@ -385,6 +591,8 @@ def main():
     ap.add_argument("--ensemble-slots", type=int, default=64)
     ap.add_argument("--no-design-pipeline", action="store_true")
     ap.add_argument("--no-mma", action="store_true")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="skip the 256x128, 1024x512 and 2048x1024 workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but "
@@ -467,8 +675,10 @@ def main():
     out["roofline"]["plane_copy_gbs"] = round(pc, 1)
     out["roofline"]["frac_of_plane_copy"] = round(head["fwd_gbs"] / pc, 4)
     log(f"19-plane copy {pc:.1f} GB/s; forward sweep at {head['fwd_gbs'] / pc:.3f} of it")
+    if not args.no_other_workloads:
+        out["other_workloads"] = run_other_workloads(lbmx, workloads, mode)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(workloads, lbmx)
+        out["cpu_baseline"] = cpu_baseline(workloads, lbmx, args.horizon, args.slots, args.placement)
     print(json.dumps(out), flush=True)
 
 
