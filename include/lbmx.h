@@ -500,6 +500,19 @@ lbmx_status lbmx_mma_design_change_device(lbmx_mma* h, const double* xnew, doubl
    milliseconds of the last update (any pointer may be NULL). */
 lbmx_status lbmx_mma_get_state(lbmx_mma* h, double* L, double* U, double* lam, int* newton_iters, double* ms);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the design change
+   of lbmx_mma_design_change_device, max_i |xnew[i] - xold[i]| (MMA::DesignChange, MMA.cc:389-414),
+   with the previous design passed in instead of an MMA handle. xnew, xold (n values) and ch (one
+   value, the result) are HIP device pointers; the work is enqueued on hip_stream (NULL: the
+   legacy default stream) and the call returns without waiting for it. It allocates nothing: the
+   reduction's per-block partials live in a library-static scratch array of 2048 doubles (16 KB)
+   on each device, kept until the process ends. Calls that overlap on one device race on that
+   array. n must lie in 1..LBMX_NOALLOC_MAX_N (else LBMX_ERR_ARG). */
+#define LBMX_NOALLOC_MAX_N (1LL << 40)
+lbmx_status lbmx_design_change_noalloc(const double* xnew, const double* xold, long long n, double* ch,
+                                       void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -243,6 +244,34 @@ struct lbmx_mma {
   std::unique_ptr<lbmx::Mma> o;
 };
 
+// Scratch of lbmx_design_change_noalloc: the per-block partial maxima of the change reduction
+// (mma_blocks() launches at most kNoAllocPartials blocks). Each device holds its own copy for
+// the life of the process; its address on the current device comes from hipGetSymbolAddress
+// (the array's name is a host-side stand-in, never a device address).
+static constexpr int kNoAllocPartials = 2048;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // per device, filled on the first call there
+
+double* scratch_on_current_device() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw lbmx::Error{LBMX_ERR_DEVICE, std::string("hipGetSymbolAddress of the scratch array failed: ") +
+                                            hipGetErrorString(e)};
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 using lbmx::Error;
 using lbmx::guarded;
 
@@ -325,6 +354,21 @@ lbmx_status lbmx_mma_design_change_device(lbmx_mma* h, const double* xnew, doubl
   CHECK_MMA(h);
   if (!ch) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   return guarded([&] { *ch = h->o->design_change(xnew, true); });
+}
+
+lbmx_status lbmx_design_change_noalloc(const double* xnew, const double* xold, long long n, double* ch,
+                                       void* hip_stream) {
+  if (!xnew || !xold || !ch) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  if (n < 1 || n > LBMX_NOALLOC_MAX_N) {
+    lbmx::g_last_error = "design length " + std::to_string(n) + " outside 1.." + std::to_string(LBMX_NOALLOC_MAX_N);
+    return LBMX_ERR_ARG;
+  }
+  return guarded([&] {
+    static_assert(kNoAllocPartials >= 2048, "the scratch must hold mma_blocks() partials");
+    if (lbmx::mma_blocks(n) > kNoAllocPartials) throw Error{LBMX_ERR_ARG, "too many reduction blocks"};
+    double* part = scratch_on_current_device();
+    HIP_OK(lbmx::launch_mma_change(xnew, xold, n, part, ch, static_cast<hipStream_t>(hip_stream)));
+  });
 }
 
 lbmx_status lbmx_mma_get_state(lbmx_mma* h, double* L, double* U, double* lam, int* newton_iters, double* ms) {

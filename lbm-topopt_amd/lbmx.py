@@ -171,6 +171,7 @@ def library():
         "lbmx_batch_checkpoint_config": [C.c_void_p, C.c_int],
         "lbmx_batch_set_tile_rows": [C.c_void_p, C.c_int],
         "lbmx_batch_set_tile": [C.c_void_p, C.c_int, C.c_int],
+        "lbmx_design_change_noalloc": [dp, dp, C.c_longlong, dp, C.c_void_p],
         "lbmx_batch_get_tile": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "lbmx_batch_set_graph": [C.c_void_p, C.c_int],
         "lbmx_batch_autotune": [C.c_void_p, C.c_int, P(C.c_int), dp],
@@ -244,7 +245,8 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_filter_solver_sensitivities_device", "lbmx_mma_update_device", "lbmx_mma_design_change_device",
             "lbmx_dmda_global_to_natural", "lbmx_dmda_natural_to_global", "lbmx_checkpoint_mode",
             "lbmx_checkpoint_simulate_mode", "lbmx_batch_checkpoint_mode",
-            "lbmx_batch_set_tile", "lbmx_batch_get_tile", "lbmx_add_boundary_table"]
+            "lbmx_batch_set_tile", "lbmx_batch_get_tile", "lbmx_add_boundary_table",
+            "lbmx_design_change_noalloc"]
 
 
 def checkpoint_simulate(nslots, nsteps, mode=CHECKPOINT_DYNAMIC):
