@@ -356,13 +356,15 @@ def other_workloads(lbmx, workloads):
     ]
 
 
-def run_other_workloads(lbmx, workloads, mode, sweeps=50):
+def run_other_workloads(lbmx, workloads, mode, sweeps=50, only=None):
     """Forward and adjoint sweep rates (HIP events) and the whole pass of the other benchmark
     workloads, FP64 and FP16-shifted forms. The 256 x 128 channel is a forward run; the others
     are design gradients with checkpoints. Their grids are small enough to be partly
     L2-resident, so their GB/s are algorithmic bytes over time, not HBM traffic."""
     out = {}
     for w in other_workloads(lbmx, workloads):
+        if only and w["name"] not in only:
+            continue
         nx, ny = w["grid"]
         N = nx * ny
         rec = {}
@@ -593,6 +595,9 @@ def main():
     ap.add_argument("--no-mma", action="store_true")
     ap.add_argument("--no-other-workloads", action="store_true",
                     help="skip the 256x128, 1024x512 and 2048x1024 workloads")
+    ap.add_argument("--only-other-workloads", type=str, default="",
+                    help="comma-separated names: run only these of the other workloads and print their record "
+                         "(for kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but "
@@ -600,10 +605,14 @@ def main():
         sys.exit(2)
     import lbmx
     import workloads
-    nx, ny, nz = (int(v) for v in args.grid.split("x"))
-    forms = [f for f in args.forms.split(",") if f]
     placements = {"binomial": lbmx.CHECKPOINT_BINOMIAL, "dynamic": lbmx.CHECKPOINT_DYNAMIC,
                   "static": lbmx.CHECKPOINT_STATIC}
+    if args.only_other_workloads:
+        only = set(args.only_other_workloads.split(","))
+        print(json.dumps(run_other_workloads(lbmx, workloads, placements[args.placement], only=only)), flush=True)
+        return
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    forms = [f for f in args.forms.split(",") if f]
     mode = placements[args.placement]
     # the headline form also runs the reference's own (dynamic) placement once, or the binomial one
     cmp_name = "dynamic" if args.placement != "dynamic" else "binomial"
