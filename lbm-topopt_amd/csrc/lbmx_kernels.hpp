@@ -245,6 +245,48 @@ __device__ __forceinline__ unsigned long long bc_pieces(unsigned long long m) {
   return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
+// The boundary pieces of a wave's row segment (descriptors and this step's time factors),
+// copied into LDS by the wave itself as soon as its row word has arrived, while its population
+// loads are still in flight. Its boundary lanes then read them from LDS instead of through a
+// chain of scalar and global loads that would start only after the populations. A wave reads
+// back only the slots it wrote itself, so no workgroup barrier is needed; two waves of a
+// workgroup may write the same slot, with the same words. Dynamic LDS of the launch:
+// pieces_lds_bytes(nbc).
+extern __shared__ __align__(16) unsigned char pieces_lds[];
+
+__host__ __device__ constexpr size_t pieces_lds_bytes(int nbc) {
+  return (size_t)(nbc > 0 ? nbc : 0) * (sizeof(DevBC) + sizeof(double));
+}
+__device__ __forceinline__ const DevBC* staged_pieces() { return reinterpret_cast<const DevBC*>(pieces_lds); }
+__device__ __forceinline__ const double* staged_tfac(int nbc) {
+  return reinterpret_cast<const double*>(pieces_lds + (size_t)nbc * sizeof(DevBC));
+}
+
+// Called by every lane of the wave with the same mask.
+__device__ __forceinline__ void stage_wave_pieces(const DevBC* __restrict__ bcs, const double* __restrict__ tfac, int nbc,
+                                                  unsigned long long mask) {
+  constexpr int W = (int)(sizeof(DevBC) / 4);
+  static_assert(sizeof(DevBC) % 4 == 0 && W < kTileX, "a descriptor is copied one 32-bit word per lane");
+  const int lane = (int)threadIdx.x;
+  const unsigned int* s = reinterpret_cast<const unsigned int*>(bcs);
+  unsigned int* d = reinterpret_cast<unsigned int*>(pieces_lds);
+  double* tf = reinterpret_cast<double*>(pieces_lds + (size_t)nbc * sizeof(DevBC));
+  for (unsigned long long m = mask; m != 0; m &= m - 1) {
+    const int b = __builtin_ctzll(m);
+    if (lane < W) d[b * W + lane] = s[b * W + lane];
+    else if (lane == W) tf[b] = tfac[b];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// The wave's row segment (x0 .. x0+63 of row y, slice z) holds a boundary node.
+template <class L>
+__device__ __forceinline__ bool segment_on_boundary(int x0, int y, int z, int nx, int ny, int nz) {
+  bool b = x0 == 0 || x0 + kTileX >= nx || y == 0 || y == ny - 1;
+  if constexpr (L::D == 3) b = b || z == 0 || z == nz - 1;
+  return b;
+}
+
 template <class L, class P>
 __device__ __forceinline__ void apply_bcs(typename P::T (&f)[L::Q], const DevBC* __restrict__ bcs,
                                           unsigned long long pieces, const double* tfac, int x, int y, int z,
@@ -403,20 +445,26 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   const bool active = x < nx && y < ny;
   const bool objblk = a.obj.acc != nullptr && a.mode == 0 &&
                       tile_meets_terms<TY>(a.obj, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
+  const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x - (int)threadIdx.x, y, z, nx, ny, nz);
   double objv = 0.0;
+  const unsigned int off = active ? layout_off(a.ls, x, y, z) : 0u;
+  const int n = (z * ny + y) * nx + x;
+  const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
+  unsigned long long rowword = 0;
+  T f[L::Q];
   if (active) {
-    const unsigned int off = layout_off(a.ls, x, y, z);
-    const int n = (z * ny + y) * nx + x;
-    const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
-    const unsigned long long rowword = bc_row_word(a.bcrows, nx, ny, x, y, z);
-    T f[L::Q];
+    rowword = bc_row_word(a.bcrows, nx, ny, x, y, z);
     if (a.mode == 2) {
       const double* s = static_cast<const double*>(a.src);
       static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
     } else {
       pull_node<L, P, 2>(a.src, a.ls, off, f);
-      if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(rowword), tfac, x, y, z, a.src, a.ls);
     }
+  }
+  const unsigned long long pieces = bseg ? bc_pieces(rowword) : 0ull;
+  if (bseg) stage_wave_pieces(a.bcs, tfac, a.nbc, pieces);
+  if (active) {
+    if (bnd && a.mode != 2) apply_bcs<L, P>(f, staged_pieces(), pieces, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
       static_for<0, L::Q>([&](auto qc) { out[(long long)n * L::Q + decltype(qc)::value] = double(f[decltype(qc)::value]); });
@@ -495,22 +543,30 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_init(FwdArgs a, const double
 }
 
 // One adjoint sweep over node (x,y,z); tfac: time factor per boundary piece at step a.t.
-template <class L, int COLL, int DES, class P>
+template <class L, int COLL, int DES, class P, int TY>
 __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, int x, int y, int z) {
   using T = typename P::T;
   using LS = typename P::LS;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-  if (x >= nx || y >= ny) return;
+  const bool active = x < nx && y < ny;
+  const bool bseg = a.nbc > 0 && segment_on_boundary<L>(x - (int)threadIdx.x, y, z, nx, ny, nz);
   const int n = (z * ny + y) * nx + x;
-  const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
-  const unsigned long long rowword = bc_row_word(a.bcrows, nx, ny, x, y, z);
+  const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
+  unsigned long long rowword = 0;
   T f[L::Q];
-  if (a.fwd_aos) {
-    static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(a.fwd_aos[(long long)n * L::Q + decltype(qc)::value]); });
-  } else {
-    pull_node<L, P, 1>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
-    if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(rowword), tfac, x, y, z, a.fwd_src, a.ls);
+  if (active) {
+    rowword = bc_row_word(a.bcrows, nx, ny, x, y, z);
+    if (a.fwd_aos) {
+      static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(a.fwd_aos[(long long)n * L::Q + decltype(qc)::value]); });
+    } else {
+      pull_node<L, P, 1>(a.fwd_src, a.ls, layout_off(a.ls, x, y, z), f);
+    }
   }
+  const unsigned long long pieces = bseg ? bc_pieces(rowword) : 0ull;
+  if (bseg) stage_wave_pieces(a.bcs, tfac, a.nbc, pieces);
+  if (!active) return;
+  const DevBC* const sbcs = staged_pieces();
+  if (bnd && !a.fwd_aos) apply_bcs<L, P>(f, sbcs, pieces, staged_tfac(a.nbc), x, y, z, a.fwd_src, a.ls);
   const unsigned int offl = layout_off(a.ll, x, y, z) * (unsigned int)sizeof(LS);
   // The adjoint field between sweeps is m_t, the boundary-corrected adjoint of s_t before the
   // transposed stream, stored at its own node. The transposed stream is gathered here:
@@ -544,7 +600,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   T dg = 0;
   reverse_collide<L, COLL, DES, T>(f, gam, a.m, lam, dg, srcA, srcB, srcAT);
   __builtin_nontemporal_store(__builtin_nontemporal_load(a.sens + n) + double(dg), a.sens + n);
-  if (bnd) apply_bcs_adj<L, T>(lam, a.bcs, bc_pieces(rowword), x, y, z, a.nside);
+  if (bnd) apply_bcs_adj<L, T>(lam, sbcs, pieces, x, y, z, a.nside);
   // m_{t-1} at the own node (whole rows); the next adjoint sweep gathers its transposed stream
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
@@ -556,7 +612,7 @@ template <class L, int COLL, int DES, class P>
 __global__ __launch_bounds__(kTileX* kTileY) void k_adj(AdjArgs a) {
   int x, y, z, tile;
   tile_coords(a.g, x, y, z, tile);
-  adj_node<L, COLL, DES, P>(a, a.tfac, x, y, z);
+  adj_node<L, COLL, DES, P, kTileY>(a, a.tfac, x, y, z);
 }
 
 // ---- batched sweeps: one launch covers every instance of a batch ------------------------
@@ -631,7 +687,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   a.obj = I.obj;
   a.obj.acc = nullptr;
   if (!st.with_obj) a.obj.nterm = 0;
-  adj_node<L, COLL, DES, P>(a, st.tfac + (long long)i * st.tf_stride, x, y, z);
+  adj_node<L, COLL, DES, P, TY>(a, st.tfac + (long long)i * st.tf_stride, x, y, z);
 }
 
 // Batched forward sweep with two nodes per thread: a tile of 2H rows is swept by H waves, each
@@ -641,13 +697,13 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
 // which already has three fields in flight per node, keeps one node per thread: paired, it
 // was slower.)
 template <class L, int COLL, int DES, class P>
-__device__ __forceinline__ double fwd_finish(const FwdArgs& a, const double* tfac, typename P::T (&f)[L::Q],
-                                             typename P::T gam, unsigned long long rowword, unsigned int off, int x,
-                                             int y, int z, bool objblk) {
+__device__ __forceinline__ double fwd_finish(const FwdArgs& a, typename P::T (&f)[L::Q], typename P::T gam,
+                                             unsigned long long pieces, unsigned int off, int x, int y, int z,
+                                             bool objblk) {
   using T = typename P::T;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
-  if (bnd) apply_bcs<L, P>(f, a.bcs, bc_pieces(rowword), tfac, x, y, z, a.src, a.ls);
+  if (bnd) apply_bcs<L, P>(f, staged_pieces(), pieces, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
   double objv = 0.0;
   if (objblk && in_box(a.obj.bbox, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
   T rho, j[3];
@@ -680,9 +736,14 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
     pull_node<L, P, 2>(a.src, a.ls, off2, f2);
     g2 = T(gp[(z * ny + y2) * nx + x]);
   }
+  const int x0 = x - (int)threadIdx.x;
+  const bool bseg1 = a.nbc > 0 && segment_on_boundary<L>(x0, y, z, nx, ny, a.g.nz);
+  const bool bseg2 = a.nbc > 0 && segment_on_boundary<L>(x0, y2, z, nx, ny, a.g.nz);
+  const unsigned long long p1 = bseg1 ? bc_pieces(rw1) : 0ull, p2 = bseg2 ? bc_pieces(rw2) : 0ull;
+  if (bseg1 || bseg2) stage_wave_pieces(a.bcs, tfac, a.nbc, p1 | p2);
   double obj1 = 0.0, obj2 = 0.0;
-  if (act1) obj1 = fwd_finish<L, COLL, DES, P>(a, tfac, f1, g1, rw1, off1, x, y, z, objblk);
-  if (act2) obj2 = fwd_finish<L, COLL, DES, P>(a, tfac, f2, g2, rw2, off2, x, y2, z, objblk);
+  if (act1) obj1 = fwd_finish<L, COLL, DES, P>(a, f1, g1, p1, off1, x, y, z, objblk);
+  if (act2) obj2 = fwd_finish<L, COLL, DES, P>(a, f2, g2, p2, off2, x, y2, z, objblk);
   if (objblk) {
     double sum = block_sum_rows2<H>(obj1, obj2);
     if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += sum;
@@ -714,7 +775,8 @@ __global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
 // ---- host-side launchers ----
 template <class L, int COLL, int DES, class P>
 hipError_t launch_fwd_t(const FwdArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_fwd<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), 0, s, a);
+  hipLaunchKernelGGL((k_fwd<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), pieces_lds_bytes(a.nbc), s,
+                     a);
   return hipGetLastError();
 }
 
@@ -728,7 +790,8 @@ hipError_t launch_init_t(const FwdArgs& a, const double* macros, const double* u
 
 template <class L, int COLL, int DES, class P>
 hipError_t launch_adj_t(const AdjArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_adj<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), 0, s, a);
+  hipLaunchKernelGGL((k_adj<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), pieces_lds_bytes(a.nbc), s,
+                     a);
   return hipGetLastError();
 }
 
@@ -749,19 +812,20 @@ hipError_t launch_neumann_fix_t(const DevBC& e, void* lam, const Layout& ll, con
 // thread (TY waves). Both forward forms cover the same tiles with the same node arithmetic.
 template <class L, int COLL, int DES, class P, int TY>
 hipError_t launch_batch_t(int kind, bool pair, const BatchStep& st, hipStream_t s) {
+  const size_t lds = pieces_lds_bytes(st.tf_stride);  // tf_stride: the most pieces of any member
   if (kind == 0) {
     if constexpr (TY == 1) {
-      hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, 1>), dim3(st.nblocks), dim3(kTileX, 1, 1), 0, s, st);
+      hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, 1>), dim3(st.nblocks), dim3(kTileX, 1, 1), lds, s, st);
     } else {
       static_assert(TY % 2 == 0, "paired tiles have an even number of rows");
       if (pair)
-        hipLaunchKernelGGL((k_fwd_batch_pair<L, COLL, DES, P, TY / 2>), dim3(st.nblocks), dim3(kTileX, TY / 2, 1), 0,
+        hipLaunchKernelGGL((k_fwd_batch_pair<L, COLL, DES, P, TY / 2>), dim3(st.nblocks), dim3(kTileX, TY / 2, 1), lds,
                            s, st);
       else
-        hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);
+        hipLaunchKernelGGL((k_fwd_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), lds, s, st);
     }
   } else {
-    hipLaunchKernelGGL((k_adj_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), 0, s, st);
+    hipLaunchKernelGGL((k_adj_batch<L, COLL, DES, P, TY>), dim3(st.nblocks), dim3(kTileX, TY, 1), lds, s, st);
   }
   return hipGetLastError();
 }
