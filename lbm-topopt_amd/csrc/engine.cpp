@@ -234,6 +234,8 @@ struct Engine {
   size_t state_bytes() const { return (size_t)ls.plane * Q * prec_state_bytes(d.precision); }
   size_t lam_bytes() const { return (size_t)ll.plane * Q * prec_lam_bytes(d.precision); }
   int grid_blocks() const { return geom().nblocks; }
+  // 64-node row segments: one objective accumulator each (the sweeps' per-wave sums)
+  size_t nseg() const { return (size_t)((d.nx + kTileX - 1) / kTileX) * d.ny * d.nz; }
 
   // ---- boundary pieces: NewLBSolver2d::addBoundaryCondition (src/latticeBoltzmann/NewLBSolver2d.cc:106-155),
   //      getBoxBoundaries (src/core/geometry2d.cc:31-53, src/core/geometry3d.cc:36-85) ----
@@ -457,6 +459,8 @@ struct Engine {
         bb[2 * k + 1] = std::max(bb[2 * k + 1], tv[m].box[2 * k + 1]);
       }
     for (int k = 0; k < 6; ++k) a.bbox[k] = bb[k];
+    for (int m = 0; m < a.nterm; ++m)
+      for (int k = 0; k < 6; ++k) a.tbox[m][k] = tv[m].box[k];
     obj = a;
     terms = tv;
     terms_dirty = true;
@@ -683,7 +687,7 @@ struct Engine {
     touched = true;
     upload_static();
     ensure_pool(2);
-    acc_dev.ensure((size_t)grid_blocks() * sizeof(double));
+    acc_dev.ensure(nseg() * sizeof(double));
     red_dev.ensure(sizeof(double));
   }
 
@@ -716,10 +720,10 @@ struct Engine {
     if (state_kind < 0) throw Error{LBMX_ERR_STATE, "state not initialised (call lbmx_init_equilibrium)"};
   }
 
-  void zero_acc() { zero_async(acc_dev.p, (size_t)grid_blocks() * sizeof(double), stream); }
+  void zero_acc() { zero_async(acc_dev.p, nseg() * sizeof(double), stream); }
 
   double reduce_acc() {
-    std::vector<double> h(grid_blocks());
+    std::vector<double> h(nseg());
     HIP_OK(hipMemcpyAsync(h.data(), acc_dev.p, h.size() * 8, hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
     double s = 0;
@@ -1148,7 +1152,7 @@ struct Batch {
       e.stream = stream;
       e.prepare_static();
       const Geometry g = geom_for(e);
-      e.acc_dev.ensure((size_t)std::max(g.nblocks, e.grid_blocks()) * sizeof(double));
+      e.acc_dev.ensure(e.nseg() * sizeof(double));
       e.sens_dev.ensure((size_t)e.N * 8);
       BatchInst I{};
       I.g = g;
@@ -1238,12 +1242,12 @@ struct Batch {
 
   void zero_accs() {
     for (size_t i = 0; i < m.size(); ++i)
-      if (inst[i].obj.acc) zero_async(inst[i].obj.acc, (size_t)inst[i].g.nblocks * sizeof(double), stream);
+      if (inst[i].obj.acc) zero_async(inst[i].obj.acc, m[i]->nseg() * sizeof(double), stream);
   }
 
   double reduce_acc(size_t i) {
     if (!inst[i].obj.acc) return 0.0;
-    std::vector<double> h(inst[i].g.nblocks);
+    std::vector<double> h(m[i]->nseg());
     HIP_OK(hipMemcpyAsync(h.data(), inst[i].obj.acc, h.size() * 8, hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
     double s = 0;

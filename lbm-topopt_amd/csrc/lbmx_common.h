@@ -84,9 +84,10 @@ struct ObjArgs {
   int nterm;
   int bbox[6];        // union of term boxes (skip test)
   int pad;
+  int tbox[kMaxTerms][6];  // the term boxes by value: the box tests need no memory loads
   double time_const;
   const DevTerm* terms;  // device array of nterm terms
-  double* acc;           // per-workgroup running sums (FP64), null = off
+  double* acc;           // running sums per 64-node row segment (FP64), null = off
 };
 
 // Forward sweep: dst = C(B_t(S(src))) ; objective of B_t(S(src)) accumulated.

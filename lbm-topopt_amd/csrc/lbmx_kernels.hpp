@@ -376,53 +376,37 @@ __global__ __launch_bounds__(64) void k_neumann_fix(void* lam, Layout ll, DevBC 
   }
 }
 
-// TY: rows of the workgroup tile (kTileX x TY threads, one wave per row)
-template <int TY>
-__device__ __forceinline__ bool tile_meets(const int* bb, int x0, int y0, int z) {
-  int x1 = x0 + kTileX - 1, y1 = y0 + TY - 1;
-  return !(x1 < bb[0] || x0 > bb[1] || y1 < bb[2] || y0 > bb[3] || z < bb[4] || z > bb[5]);
+// The objective of a sweep is summed per 64-node row segment: the wave of a segment that
+// meets a functional term reduces its nodes' values across the wave and adds the sum to the
+// segment's accumulator (one writer per segment and sweep, sweeps in stream order), and the host
+// adds the accumulators in segment order. The result does not depend on the workgroup tile, and
+// no workgroup barrier is needed.
+__device__ __forceinline__ bool segment_meets_box(const int* bb, int x0, int y, int z) {
+  return !(x0 + kTileX - 1 < bb[0] || x0 > bb[1] || y < bb[2] || y > bb[3] || z < bb[4] || z > bb[5]);
 }
-
-// the workgroup tile holds nodes of some functional term (uniform over the workgroup)
-template <int TY>
-__device__ __forceinline__ bool tile_meets_terms(const ObjArgs& o, int x0, int y0, int z) {
-  if (!tile_meets<TY>(o.bbox, x0, y0, z)) return false;
-  for (int m = 0; m < o.nterm; ++m)
-    if (tile_meets<TY>(o.terms[m].box, x0, y0, z)) return true;
-  return false;
+// (the term loops are unrolled over kMaxTerms: a runtime index into the by-value box array
+// would place the argument structure in scratch memory)
+__device__ __forceinline__ bool segment_meets_terms(const ObjArgs& o, int x0, int y, int z) {
+  if (o.nterm <= 0 || !segment_meets_box(o.bbox, x0, y, z)) return false;
+  bool meets = false;
+  static_for<0, kMaxTerms>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    meets = meets || (m < o.nterm && segment_meets_box(o.tbox[m], x0, y, z));
+  });
+  return meets;
 }
-
-template <int TY>
-__device__ __forceinline__ double block_sum(double v) {
-  __shared__ double part[TY];
+__device__ __forceinline__ bool in_some_term(const ObjArgs& o, int x, int y, int z) {
+  bool in = false;
+  static_for<0, kMaxTerms>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    in = in || (m < o.nterm && in_box(o.tbox[m], x, y, z));
+  });
+  return in;
+}
+// wave sum of v, then lane 0 adds it to acc[seg]; every lane of the wave calls it
+__device__ __forceinline__ void add_segment_sum(double* acc, long long seg, double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if (threadIdx.x == 0) part[threadIdx.y] = v;
-  __syncthreads();
-  double s = 0;
-  if (threadIdx.x == 0 && threadIdx.y == 0)
-    for (int w = 0; w < TY; ++w) s += part[w];
-  return s;
-}
-
-// The paired form of block_sum: wave w holds rows w and w + H of a 2H-row tile. Each row is
-// reduced across its wave on its own and the 2H row sums are added in row order, which is the
-// order block_sum<2H> gives one node per thread: the tile's partial is bitwise the same.
-template <int H>
-__device__ __forceinline__ double block_sum_rows2(double v1, double v2) {
-  __shared__ double part[2 * H];
-  for (int off = 32; off > 0; off >>= 1) {
-    v1 += __shfl_down(v1, off, 64);
-    v2 += __shfl_down(v2, off, 64);
-  }
-  if (threadIdx.x == 0) {
-    part[threadIdx.y] = v1;
-    part[threadIdx.y + H] = v2;
-  }
-  __syncthreads();
-  double s = 0;
-  if (threadIdx.x == 0 && threadIdx.y == 0)
-    for (int w = 0; w < 2 * H; ++w) s += part[w];
-  return s;
+  if (threadIdx.x == 0) acc[seg] += v;
 }
 
 template <class L, int COLL, class T>
@@ -430,9 +414,11 @@ __device__ __forceinline__ T node_objective(const ObjArgs& o, const T (&f)[L::Q]
   T rho, j[3];
   flow_macros<L>(f, rho, j);
   T acc = 0;
-  for (int m = 0; m < o.nterm; ++m)
-    if (in_box(o.terms[m].box, x, y, z))
+  static_for<0, kMaxTerms>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m < o.nterm && in_box(o.tbox[m], x, y, z))
       acc += T(o.terms[m].weight) * term_value<L, COLL, T>(o.terms[m].kind, f, rho, j);
+  });
   return acc * T(o.time_const);
 }
 
@@ -443,9 +429,9 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   using T = typename P::T;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool active = x < nx && y < ny;
-  const bool objblk = a.obj.acc != nullptr && a.mode == 0 &&
-                      tile_meets_terms<TY>(a.obj, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
-  const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x - (int)threadIdx.x, y, z, nx, ny, nz);
+  const int x0 = x - (int)threadIdx.x;
+  const bool objseg = a.obj.acc != nullptr && a.mode == 0 && y < ny && segment_meets_terms(a.obj, x0, y, z);
+  const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x0, y, z, nx, ny, nz);
   double objv = 0.0;
   const unsigned int off = active ? layout_off(a.ls, x, y, z) : 0u;
   const int n = (z * ny + y) * nx + x;
@@ -469,17 +455,14 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
       double* out = static_cast<double*>(a.aos_out);
       static_for<0, L::Q>([&](auto qc) { out[(long long)n * L::Q + decltype(qc)::value] = double(f[decltype(qc)::value]); });
     } else {
-      if (objblk && in_box(a.obj.bbox, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
+      if (objseg && in_some_term(a.obj, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
       const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
       T rho, j[3];
       collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
       store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
     }
   }
-  if (objblk) {
-    double s = block_sum<TY>(objv);
-    if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += s;
-  }
+  if (objseg) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), objv);
 }
 
 template <class L, int COLL, int DES, class P>
@@ -550,6 +533,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool active = x < nx && y < ny;
   const bool bseg = a.nbc > 0 && segment_on_boundary<L>(x - (int)threadIdx.x, y, z, nx, ny, nz);
+  const bool objseg = a.obj.nterm > 0 && segment_meets_terms(a.obj, x - (int)threadIdx.x, y, z);
   const int n = (z * ny + y) * nx + x;
   const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
   unsigned long long rowword = 0;
@@ -586,15 +570,17 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
     });
   }
   T srcA = 0, srcB[3] = {0, 0, 0}, srcAT = 0;
-  if (a.obj.nterm > 0 && in_box(a.obj.bbox, x, y, z)) {
+  if (objseg && in_some_term(a.obj, x, y, z)) {
     T rho, j[3];
     flow_macros<L>(f, rho, j);
     T temp = 0;
     if constexpr (L::QT > 0) temp = f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
-    for (int m = 0; m < a.obj.nterm; ++m)
-      if (in_box(a.obj.terms[m].box, x, y, z))
+    static_for<0, kMaxTerms>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      if (m < a.obj.nterm && in_box(a.obj.tbox[m], x, y, z))
         term_source<T>(a.obj.terms[m].kind, rho, j, temp, T(a.obj.time_const * a.obj.terms[m].weight), srcA, srcB,
                        srcAT);
+    });
   }
   const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
   T dg = 0;
@@ -699,13 +685,13 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
 template <class L, int COLL, int DES, class P>
 __device__ __forceinline__ double fwd_finish(const FwdArgs& a, typename P::T (&f)[L::Q], typename P::T gam,
                                              unsigned long long pieces, unsigned int off, int x, int y, int z,
-                                             bool objblk) {
+                                             bool objseg) {
   using T = typename P::T;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
   if (bnd) apply_bcs<L, P>(f, staged_pieces(), pieces, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
   double objv = 0.0;
-  if (objblk && in_box(a.obj.bbox, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
+  if (objseg && in_some_term(a.obj, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
   T rho, j[3];
   collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
   store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
@@ -719,7 +705,9 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   const int nx = a.g.nx, ny = a.g.ny;
   const int y2 = y + H;
   const bool act1 = x < nx && y < ny, act2 = x < nx && y2 < ny;
-  const bool objblk = a.obj.acc != nullptr && tile_meets_terms<2 * H>(a.obj, x - (int)threadIdx.x, y - (int)threadIdx.y, z);
+  const int x0 = x - (int)threadIdx.x;
+  const bool objseg1 = a.obj.acc != nullptr && y < ny && segment_meets_terms(a.obj, x0, y, z);
+  const bool objseg2 = a.obj.acc != nullptr && y2 < ny && segment_meets_terms(a.obj, x0, y2, z);
   const typename P::G* gp = static_cast<const typename P::G*>(a.gamma);
   T f1[L::Q], f2[L::Q], g1 = 0, g2 = 0;
   unsigned long long rw1 = 0, rw2 = 0;
@@ -736,18 +724,15 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
     pull_node<L, P, 2>(a.src, a.ls, off2, f2);
     g2 = T(gp[(z * ny + y2) * nx + x]);
   }
-  const int x0 = x - (int)threadIdx.x;
   const bool bseg1 = a.nbc > 0 && segment_on_boundary<L>(x0, y, z, nx, ny, a.g.nz);
   const bool bseg2 = a.nbc > 0 && segment_on_boundary<L>(x0, y2, z, nx, ny, a.g.nz);
   const unsigned long long p1 = bseg1 ? bc_pieces(rw1) : 0ull, p2 = bseg2 ? bc_pieces(rw2) : 0ull;
   if (bseg1 || bseg2) stage_wave_pieces(a.bcs, tfac, a.nbc, p1 | p2);
   double obj1 = 0.0, obj2 = 0.0;
-  if (act1) obj1 = fwd_finish<L, COLL, DES, P>(a, f1, g1, p1, off1, x, y, z, objblk);
-  if (act2) obj2 = fwd_finish<L, COLL, DES, P>(a, f2, g2, p2, off2, x, y2, z, objblk);
-  if (objblk) {
-    double sum = block_sum_rows2<H>(obj1, obj2);
-    if (threadIdx.x == 0 && threadIdx.y == 0) a.obj.acc[tile] += sum;
-  }
+  if (act1) obj1 = fwd_finish<L, COLL, DES, P>(a, f1, g1, p1, off1, x, y, z, objseg1);
+  if (act2) obj2 = fwd_finish<L, COLL, DES, P>(a, f2, g2, p2, off2, x, y2, z, objseg2);
+  if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1);
+  if (objseg2) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y2, z), obj2);
 }
 
 template <class L, int COLL, int DES, class P, int H>
