@@ -152,10 +152,15 @@ def test_thermal_channel_2048x1024_forms_full_size(lbmx_mod):
     k_s/k_f = 10 (ramp C_k = 0.1), hot inlet T = 1, cold walls T = 0, outlet heat flux; the
     BF16- and FP16-shifted storage forms against the FP64 form. The run starts from a moving,
     uniformly warm state (u = 0.03, T = 0.5): from rest, heat from the inlet does not reach
-    the outlet 2047 cells away within 4000 steps and the objective would be zero."""
+    the outlet 2047 cells away within 4000 steps and the objective would be zero.
+
+    From that start the outlet heat flux comes almost all from the first ~1000 steps, so J over
+    4000 steps is a poorly conditioned quantity: even the FP32 form is only good to ~1e-4 in
+    the gradient. The 16-bit forms' error comes from the stochastic rounding of the flow
+    planes (storing the temperature planes in FP32 leaves it unchanged)."""
     nx, ny, steps = 2048, 1024, 4000
     out = {}
-    for prec in (lbmx.FP64, lbmx.FP16_SHIFTED, lbmx.BF16_SHIFTED):
+    for prec in (lbmx.FP64, lbmx.FP32, lbmx.FP16_SHIFTED, lbmx.BF16_SHIFTED):
         d = lbmx.make_desc(nx, ny, lattice=lbmx.D2Q9_D2Q5, collision=lbmx.MRT, design=lbmx.DESIGN_BRINKMAN,
                            precision=prec, omega=1.25, interp_q=0.1, alpha_max=0.1, thermal_diffusivity=0.02,
                            ramp_p=1.0, ramp_ck=0.1)
@@ -167,7 +172,8 @@ def test_thermal_channel_2048x1024_forms_full_size(lbmx_mod):
         s.close()
     J64, g64 = out[lbmx.FP64]
     assert np.isfinite(J64) and abs(J64) > 0
-    for prec, (tj, tg) in ((lbmx.FP16_SHIFTED, (1e-3, 1e-3)), (lbmx.BF16_SHIFTED, (1e-2, 1e-2))):
+    for prec, (tj, tg) in ((lbmx.FP32, (2e-4, 3e-4)), (lbmx.FP16_SHIFTED, (1e-3, 1e-3)),
+                           (lbmx.BF16_SHIFTED, (1e-2, 1e-2))):
         J, g = out[prec]
         ej = abs(J - J64) / abs(J64)
         eg = np.max(np.abs(g - g64)) / np.max(np.abs(g64))
