@@ -9,6 +9,8 @@
 // a by (1 - 2 imult). The macros are rho = f_0 + sum s_d, j = sum c_d a_d. The moments of the
 // Guo source w_q[3(c-v).F + 9(c.v)(c.F)] are contractions with compile-time tables.
 #pragma once
+#include <type_traits>
+
 #include "lbmx_device.hpp"
 
 namespace lbmx {
@@ -17,6 +19,12 @@ enum : int { D_NONE = LBMX_DESIGN_NONE, D_PBB = LBMX_DESIGN_PBB, D_BRINK = LBMX_
              D_DARCY = LBMX_DESIGN_DARCY };
 
 #define LBMX_BAS BasisOf<L::QF>::B
+
+// Multiply-adds are fused only within one statement in the functions marked with this, so that
+// their rounding does not depend on the kernel they are inlined into (cross-statement fusion
+// follows the basic blocks and the uses of each product there). Instances of one operator
+// in different kernels (single lattice, batch, tile shapes) then give bitwise equal states.
+#define LBMX_CONTRACT_IN_STATEMENT _Pragma("clang fp contract(on)")
 
 template <class T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
 // a model parameter in the arithmetic type (FP32 forms read the copy rounded on the host)
@@ -77,13 +85,16 @@ __device__ __forceinline__ void pair_macros(const Pairs<L, T>& p, T& rho, T (&j)
   rho = r; j[0] = jx; j[1] = jy; j[2] = jz;
 }
 
+constexpr double abs_c(double x) { return x < 0 ? -x : x; }
+
+// Row k carries a Brinkman source unless its polynomial P_k(v) = 3 v.G1_k + 9 v^T G2_k v
+// - 3 |v|^2 G0_k vanishes identically (the density row: the Guo source conserves mass).
 template <class L, int k>
 constexpr bool row_has_src() {
-  if (LBMX_BAS.G0[k] != 0) return true;
   for (int a = 0; a < 3; ++a) {
-    if (LBMX_BAS.G1[k][a] != 0) return true;
+    if (abs_c(LBMX_BAS.G1[k][a]) > 1e-12) return true;
     for (int b = 0; b < 3; ++b)
-      if (LBMX_BAS.G2[k][a][b] != 0) return true;
+      if (abs_c(9.0 * LBMX_BAS.G2[k][a][b] - (a == b ? 3.0 * LBMX_BAS.G0[k] : 0.0)) > 1e-12) return true;
   }
   return false;
 }
@@ -151,6 +162,7 @@ __device__ __forceinline__ void pair_row_update(Pairs<L, T>& x, T c) {
 // m_eq of row k (src/dynamics/incompressibleMRT2d.hh:62-67, src/dynamics/MRT3d.hh:92-109)
 template <class L, int k, class T>
 __device__ __forceinline__ T meq_row(T rho, const T (&v)[3], T inv_rho_s) {
+  LBMX_CONTRACT_IN_STATEMENT
   T m = 0;
   if constexpr (LBMX_BAS.a[k] != 0) {
     constexpr T kc = T(LBMX_BAS.a[k]);
@@ -180,6 +192,7 @@ __device__ __forceinline__ T meq_row(T rho, const T (&v)[3], T inv_rho_s) {
 // P_k(v) = 3 v.G1_k + 9 v^T G2_k v - 3 |v|^2 G0_k ; source moment = -alpha P_k
 template <class L, int k, class T>
 __device__ __forceinline__ T src_row(const T (&v)[3], T u2) {
+  LBMX_CONTRACT_IN_STATEMENT
   T p = 0;
   if constexpr (LBMX_BAS.G0[k] != 0) {
     constexpr T kc = T(3.0 * LBMX_BAS.G0[k]);
@@ -201,6 +214,125 @@ __device__ __forceinline__ T src_row(const T (&v)[3], T u2) {
   });
   return p;
 }
+
+// ---- D3Q19 moments grouped by lattice symmetry --------------------------------------------
+// In pair space the D3Q19 basis rows (d'Humieres et al. 2002) read the pairs through a few
+// shared sums: the three axis pairs (A_x, A_y, A_z) and, for each coordinate plane, the sum
+// and difference of its two diagonal pairs (D, E); the odd rows read, per axis a, the axis
+// pair (K_a) and the parts of the diagonal pairs in the two planes through a (H_a). Building
+// the rows from these sums, and the back transform from the matching combinations of the
+// row updates, takes about half the operations of the row-by-row form. Pair d of the
+// lattice order: 1 -z, 2 -y, 3 -x, 4/5 the yz plane (c_y c_z = +1/-1), 6/7 xz, 8/9 xy.
+namespace d3q19 {
+constexpr bool pair_order_ok() {
+  using L = LatD3Q19;
+  return L::ez[1] == -1 && L::ey[2] == -1 && L::ex[3] == -1 && L::ey[4] * L::ez[4] == 1 &&
+         L::ey[5] * L::ez[5] == -1 && L::ex[4] == 0 && L::ex[5] == 0 && L::ex[6] * L::ez[6] == 1 &&
+         L::ex[7] * L::ez[7] == -1 && L::ey[6] == 0 && L::ey[7] == 0 && L::ex[8] * L::ey[8] == 1 &&
+         L::ex[9] * L::ey[9] == -1 && L::ez[8] == 0 && L::ez[9] == 0 && L::ex[8] == -1 && L::ex[9] == 1 &&
+         L::ex[6] == -1 && L::ex[7] == 1 && L::ey[4] == -1 && L::ey[5] == 1 && L::ez[4] == -1 &&
+         L::ez[6] == -1 && L::ey[8] == -1 && L::ey[9] == -1 && L::ez[5] == -1 && L::ez[7] == -1;
+}
+static_assert(pair_order_ok(), "grouped D3Q19 transform assumes the lattice's pair order");
+// the coefficients used below, against the basis table (row k, pair d; d = 0 is f_0)
+constexpr bool rows_ok() {
+  // even rows 4, 5, 9..15 at f_0, the three axis pairs (z, y, x) and the six diagonal pairs
+  constexpr int ev[9] = {4, 5, 9, 10, 11, 12, 13, 14, 15};
+  constexpr double c[9][10] = {
+      {-30, -11, -11, -11, 8, 8, 8, 8, 8, 8}, {12, -4, -4, -4, 1, 1, 1, 1, 1, 1},
+      {0, -1, -1, 2, -2, -2, 1, 1, 1, 1},     {0, 2, 2, -4, -2, -2, 1, 1, 1, 1},
+      {0, -1, 1, 0, 0, 0, -1, -1, 1, 1},      {0, 2, -2, 0, 0, 0, -1, -1, 1, 1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 1, -1},        {0, 0, 0, 0, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 1, -1, 0, 0}};
+  for (int i = 0; i < 9; ++i)
+    for (int d = 0; d < 10; ++d)
+      if (kB19.M[ev[i]][d] != c[i][d] || kB19.par[ev[i]] != 1) return false;
+  // odd rows 1, 2, 3, 6, 7, 8, 16, 17, 18 at the nine pairs
+  constexpr int od[9] = {1, 2, 3, 6, 7, 8, 16, 17, 18};
+  constexpr double o[9][9] = {
+      {0, 0, -1, 0, 0, -1, 1, -1, 1},  {0, -1, 0, -1, 1, 0, 0, -1, -1}, {-1, 0, 0, -1, -1, -1, -1, 0, 0},
+      {0, 0, 4, 0, 0, -1, 1, -1, 1},   {0, 4, 0, -1, 1, 0, 0, -1, -1},  {4, 0, 0, -1, -1, -1, -1, 0, 0},
+      {0, 0, 0, 0, 0, 1, -1, -1, 1},   {0, 0, 0, -1, 1, 0, 0, 1, 1},    {0, 0, 0, 1, 1, -1, -1, 0, 0}};
+  for (int i = 0; i < 9; ++i)
+    for (int d = 0; d < 9; ++d)
+      if (kB19.M[od[i]][d + 1] != o[i][d] || kB19.par[od[i]] != -1) return false;
+  return kB19.par[0] == 1;
+}
+static_assert(rows_ok(), "grouped D3Q19 transform does not match the basis table");
+
+// density and momentum from the pairs
+template <class T>
+__device__ __forceinline__ void macros(const Pairs<LatD3Q19, T>& p, T& rho, T (&j)[3]) {
+  LBMX_CONTRACT_IN_STATEMENT
+  rho = p.f0 + ((p.s[1] + p.s[2]) + p.s[3]) + (((p.s[4] + p.s[5]) + (p.s[6] + p.s[7])) + (p.s[8] + p.s[9]));
+  j[0] = ((p.a[9] - p.a[8]) + (p.a[7] - p.a[6])) - p.a[3];
+  j[1] = (p.a[5] - p.a[4]) - (p.a[8] + p.a[9]) - p.a[2];
+  j[2] = -((p.a[6] + p.a[7]) + (p.a[4] + p.a[5])) - p.a[1];
+}
+
+// The relaxation in two independent halves: the odd rows read and update only a, the even
+// rows only f_0 and s. row(k, m_k) returns C_k = 2 dm_k / |M_k|^2 for row k with moment m_k
+// (the moment argument is ignored for the momentum rows, which carry the source alone).
+template <class T, class Row>
+__device__ __forceinline__ void relax_odd(Pairs<LatD3Q19, T>& p, Row&& row) {
+  LBMX_CONTRACT_IN_STATEMENT
+  const T Hx_xy = p.a[9] - p.a[8], Hx_xz = p.a[7] - p.a[6];
+  const T Hy_xy = -(p.a[8] + p.a[9]), Hy_yz = p.a[5] - p.a[4];
+  const T Hz_xz = -(p.a[6] + p.a[7]), Hz_yz = -(p.a[4] + p.a[5]);
+  const T Hx = Hx_xy + Hx_xz, Hy = Hy_xy + Hy_yz, Hz = Hz_xz + Hz_yz;
+  const T C1 = row(IC<1>{}, T(0)), C2 = row(IC<2>{}, T(0)), C3 = row(IC<3>{}, T(0));
+  const T C6 = row(IC<6>{}, Hx + T(4) * p.a[3]);
+  const T C7 = row(IC<7>{}, Hy + T(4) * p.a[2]);
+  const T C8 = row(IC<8>{}, Hz + T(4) * p.a[1]);
+  const T C16 = row(IC<16>{}, Hx_xy - Hx_xz);
+  const T C17 = row(IC<17>{}, Hy_yz - Hy_xy);
+  const T C18 = row(IC<18>{}, Hz_xz - Hz_yz);
+  // p.a -= M^T C over the odd rows
+  const T Gx = C1 + C6, Gy = C2 + C7, Gz = C3 + C8;
+  const T Xy = Gx + C16, Xz = Gx - C16, Yx = Gy - C17, Yz = Gy + C17, Zx = Gz + C18, Zy = Gz - C18;
+  p.a[3] += C1 - T(4) * C6;
+  p.a[2] += C2 - T(4) * C7;
+  p.a[1] += C3 - T(4) * C8;
+  p.a[8] += Xy + Yx;
+  p.a[9] -= Xy - Yx;
+  p.a[6] += Xz + Zx;
+  p.a[7] -= Xz - Zx;
+  p.a[4] += Yz + Zy;
+  p.a[5] -= Yz - Zy;
+}
+
+template <class T, class Row>
+__device__ __forceinline__ void relax_even(Pairs<LatD3Q19, T>& p, Row&& row) {
+  LBMX_CONTRACT_IN_STATEMENT
+  const T Az = p.s[1], Ay = p.s[2], Ax = p.s[3];
+  const T Dyz = p.s[4] + p.s[5], Dxz = p.s[6] + p.s[7], Dxy = p.s[8] + p.s[9];
+  const T SA = Ax + Ay + Az, SD = Dxy + Dxz + Dyz;
+  const T t1 = T(3) * Ax - SA, t2 = SD - T(3) * Dyz, t3 = Ay - Az, t4 = Dxy - Dxz;
+  const T C4 = row(IC<4>{}, T(8) * SD - T(11) * SA - T(30) * p.f0);
+  const T C5 = row(IC<5>{}, SD - T(4) * SA + T(12) * p.f0);
+  const T C9 = row(IC<9>{}, t1 + t2);
+  const T C10 = row(IC<10>{}, t2 - T(2) * t1);
+  const T C11 = row(IC<11>{}, t3 + t4);
+  const T C12 = row(IC<12>{}, t4 - T(2) * t3);
+  const T C13 = row(IC<13>{}, p.s[8] - p.s[9]);
+  const T C14 = row(IC<14>{}, p.s[4] - p.s[5]);
+  const T C15 = row(IC<15>{}, p.s[6] - p.s[7]);
+  // f_0 -= M^T C / 2 and s -= M^T C over the even rows
+  p.f0 -= T(6) * C5 - T(15) * C4;
+  const T ua = -(T(11) * C4 + T(4) * C5), P = C9 - T(2) * C10, W = C11 - T(2) * C12;
+  p.s[3] -= ua + T(2) * P;
+  p.s[2] -= (ua - P) + W;
+  p.s[1] -= (ua - P) - W;
+  const T ud = T(8) * C4 + C5, R = C9 + C10, V = C11 + C12;
+  const T bxy = (ud + R) + V, bxz = (ud + R) - V, byz = ud - T(2) * R;
+  p.s[8] -= bxy + C13;
+  p.s[9] -= bxy - C13;
+  p.s[6] -= bxz + C15;
+  p.s[7] -= bxz - C15;
+  p.s[4] -= byz + C14;
+  p.s[5] -= byz - C14;
+}
+}  // namespace d3q19
 
 // ---- forward (pair space) ---------------------------------------------------------------
 template <class L, int COLL, int DES, class T>
@@ -265,6 +397,42 @@ __device__ __forceinline__ void base_relax(Pairs<L, T>& p, T rho, const T (&v)[3
       }
     });
   }
+}
+
+// D3Q19 MRT relaxation with the grouped transforms: the row values of base_relax, each row
+// update applied at once (the rows are orthogonal, so the simultaneous update equals the
+// row-by-row one).
+template <int COLL, int DES, class T>
+__device__ __forceinline__ void relax_d3q19(Pairs<LatD3Q19, T>& p, T rho, const T (&v)[3], T omega, T alpha) {
+  LBMX_CONTRACT_IN_STATEMENT
+  using L = LatD3Q19;
+  constexpr bool inc = CollTraits<COLL>::inc;
+  static_assert(CollTraits<COLL>::mrt, "MRT operators only");
+  static_assert(LBMX_BAS.rate[0] == 0 && !row_has_src<L, 0>(), "the density row is neither relaxed nor forced");
+  const T u2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const T inv_rho_s = inc ? T(1) : recip<T>(rho);
+  T vm[3];
+  static_for<0, 3>([&](auto ac) {
+    constexpr int a = decltype(ac)::value;
+    vm[a] = inc ? v[a] : rho * v[a];
+  });
+  auto row = [&](auto kc, T mk) -> T {
+    constexpr int k = decltype(kc)::value;
+    constexpr T two_inv_n2 = T(2.0 * LBMX_BAS.invn2[k]);
+    if constexpr (LBMX_BAS.rate[k] != 0) {
+      constexpr T s_k = T(LBMX_BAS.s[k]);
+      const T sk = LBMX_BAS.rate[k] == 1 ? omega : s_k;
+      T val = sk * (mk - meq_row<L, k>(rho, vm, inv_rho_s));
+      if constexpr (DES == D_BRINK && row_has_src<L, k>()) val += (T(1) - T(0.5) * sk) * alpha * src_row<L, k>(v, u2);
+      return val * two_inv_n2;
+    } else if constexpr (DES == D_BRINK && row_has_src<L, k>()) {
+      return alpha * src_row<L, k>(v, u2) * two_inv_n2;
+    } else {
+      return T(0);
+    }
+  };
+  d3q19::relax_odd<T>(p, row);
+  d3q19::relax_even<T>(p, row);
 }
 
 // thermal part: ThermalObstacleCollision<ThermalMRT2d<D2Q5_TMRT>,Ramp>
@@ -533,12 +701,17 @@ template <class L, int COLL, int DES, class T>
 __device__ __forceinline__ void collide_moment(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
   Pairs<L, T> p;
   to_pairs<L>(f, p);
-  pair_macros<L>(p, rho, j);
+  // (FP32 arithmetic only: the FP64 sweeps are bound by HBM, and the grouped form holds more
+  // values live, which would cost the FP64 kernels occupancy)
+  constexpr bool grouped = std::is_same_v<L, LatD3Q19> && CollTraits<COLL>::mrt && sizeof(T) == 4;
+  if constexpr (grouped) d3q19::macros(p, rho, j);
+  else pair_macros<L>(p, rho, j);
   const T omega = mpar<T>(mp.omega, mp.omega_f);
   T alpha = 0, beta, v[3];
   if constexpr (DES == D_BRINK) alpha = mpar<T>(mp.alpha_max, mp.alpha_max_f) * bp_val<T>(mpar<T>(mp.interp_q, mp.interp_q_f), gamma);
   eq_velocity<L, COLL, DES>(rho, j, alpha, v, beta);
-  base_relax<L, COLL, DES>(p, rho, v, omega, alpha);
+  if constexpr (grouped) relax_d3q19<COLL, DES>(p, rho, v, omega, alpha);
+  else base_relax<L, COLL, DES>(p, rho, v, omega, alpha);
   if constexpr (DES == D_PBB) {
     // PartialBouncebackCollision (src/dynamics/partialBouncebackCollision.hh:53-65): the
     // pair blend keeps f_d + f_d' and scales f_d - f_d' by 1 - 2 imult

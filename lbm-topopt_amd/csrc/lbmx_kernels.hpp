@@ -403,8 +403,17 @@ __device__ __forceinline__ bool in_some_term(const ObjArgs& o, int x, int y, int
   });
   return in;
 }
-// wave sum of v, then lane 0 adds it to acc[seg]; every lane of the wave calls it
-__device__ __forceinline__ void add_segment_sum(double* acc, long long seg, double v) {
+// Wave sum of v over the lanes whose node lies in a term (`contrib`; v is 0 elsewhere), added
+// to acc[seg] by one lane; every lane of the wave calls it. A segment crossed by a face term
+// has one such lane, which adds its value itself: the tree sum would return that value
+// unchanged, since the other lanes add exact zeros.
+__device__ __forceinline__ void add_segment_sum(double* acc, long long seg, double v, bool contrib) {
+  const unsigned long long who = __ballot(contrib);
+  if (who == 0ull) return;
+  if ((who & (who - 1)) == 0ull) {
+    if (contrib) acc[seg] += v;
+    return;
+  }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if (threadIdx.x == 0) acc[seg] += v;
 }
@@ -433,6 +442,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   const bool objseg = a.obj.acc != nullptr && a.mode == 0 && y < ny && segment_meets_terms(a.obj, x0, y, z);
   const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x0, y, z, nx, ny, nz);
   double objv = 0.0;
+  bool objnode = false;
   const unsigned int off = active ? layout_off(a.ls, x, y, z) : 0u;
   const int n = (z * ny + y) * nx + x;
   const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
@@ -455,14 +465,18 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
       double* out = static_cast<double*>(a.aos_out);
       static_for<0, L::Q>([&](auto qc) { out[(long long)n * L::Q + decltype(qc)::value] = double(f[decltype(qc)::value]); });
     } else {
-      if (objseg && in_some_term(a.obj, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
-      const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
-      T rho, j[3];
-      collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
-      store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
+      objnode = objseg && in_some_term(a.obj, x, y, z);
+      if (objnode) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
     }
   }
-  if (objseg) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), objv);
+  // the segment's sum is added before the collision, so that nothing of it stays live there
+  if (objseg) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), objv, objnode);
+  if (active && a.mode != 1) {
+    const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
+    T rho, j[3];
+    collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+    store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
+  }
 }
 
 template <class L, int COLL, int DES, class P>
@@ -682,20 +696,14 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
 // occupancy; each node's arithmetic is that of fwd_node, in the same order. (The adjoint,
 // which already has three fields in flight per node, keeps one node per thread: paired, it
 // was slower.)
+// Boundary pieces and the functional of one node of a pair: returns the node's objective term
+// (0 off the functional's terms)
 template <class L, int COLL, int DES, class P>
-__device__ __forceinline__ double fwd_finish(const FwdArgs& a, typename P::T (&f)[L::Q], typename P::T gam,
-                                             unsigned long long pieces, unsigned int off, int x, int y, int z,
-                                             bool objseg) {
+__device__ __forceinline__ double fwd_bc_obj(const FwdArgs& a, typename P::T (&f)[L::Q], unsigned long long pieces,
+                                             int x, int y, int z, bool bnd, bool objnode) {
   using T = typename P::T;
-  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-  const bool bnd = on_boundary<L>(x, y, z, nx, ny, nz);
   if (bnd) apply_bcs<L, P>(f, staged_pieces(), pieces, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
-  double objv = 0.0;
-  if (objseg && in_some_term(a.obj, x, y, z)) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
-  T rho, j[3];
-  collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
-  store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
-  return objv;
+  return objnode ? double(node_objective<L, COLL, T>(a.obj, f, x, y, z)) : 0.0;
 }
 
 // Rows y and y + H of the node's tile (2H rows, H waves).
@@ -729,10 +737,24 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   const unsigned long long p1 = bseg1 ? bc_pieces(rw1) : 0ull, p2 = bseg2 ? bc_pieces(rw2) : 0ull;
   if (bseg1 || bseg2) stage_wave_pieces(a.bcs, tfac, a.nbc, p1 | p2);
   double obj1 = 0.0, obj2 = 0.0;
-  if (act1) obj1 = fwd_finish<L, COLL, DES, P>(a, f1, g1, p1, off1, x, y, z, objseg1);
-  if (act2) obj2 = fwd_finish<L, COLL, DES, P>(a, f2, g2, p2, off2, x, y2, z, objseg2);
-  if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1);
-  if (objseg2) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y2, z), obj2);
+  const bool on1 = act1 && objseg1 && in_some_term(a.obj, x, y, z);
+  const bool on2 = act2 && objseg2 && in_some_term(a.obj, x, y2, z);
+  const bool bnd1 = act1 && on_boundary<L>(x, y, z, nx, ny, a.g.nz);
+  const bool bnd2 = act2 && on_boundary<L>(x, y2, z, nx, ny, a.g.nz);
+  if (act1) obj1 = fwd_bc_obj<L, COLL, DES, P>(a, f1, p1, x, y, z, bnd1, on1);
+  if (act2) obj2 = fwd_bc_obj<L, COLL, DES, P>(a, f2, p2, x, y2, z, bnd2, on2);
+  // the segment sums are added before the collisions, so that nothing of them stays live there
+  if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1, on1);
+  if (objseg2) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y2, z), obj2, on2);
+  T rho, j[3];
+  if (act1) {
+    collide<L, COLL, DES, T>(f1, g1, a.m, rho, j);
+    store_node<L, P>(a.dst, a.ls, off1, f1, bnd1, x, y, z, nx, ny, a.g.nz, a.t);
+  }
+  if (act2) {
+    collide<L, COLL, DES, T>(f2, g2, a.m, rho, j);
+    store_node<L, P>(a.dst, a.ls, off2, f2, bnd2, x, y2, z, nx, ny, a.g.nz, a.t);
+  }
 }
 
 template <class L, int COLL, int DES, class P, int H>
@@ -758,10 +780,17 @@ __global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
 }
 
 // ---- host-side launchers ----
+// The FP64 forward sweeps run best at 4 waves per SIMD: the 384 x 192 x 192 D3Q19 sweep takes
+// 858 us there against 882 us at the 7 its registers allow (3 is slower again), and the 2D
+// sweeps are unchanged. Each workgroup (one wave per SIMD) reserves a quarter of the CU's
+// 160 KB of LDS, which caps the CU at 4 workgroups.
+constexpr size_t kFwdF64Lds = 40 * 1024;
+
 template <class L, int COLL, int DES, class P>
 hipError_t launch_fwd_t(const FwdArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_fwd<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), pieces_lds_bytes(a.nbc), s,
-                     a);
+  size_t lds = pieces_lds_bytes(a.nbc);
+  if constexpr (sizeof(typename P::T) == 8) lds = lds > kFwdF64Lds ? lds : kFwdF64Lds;
+  hipLaunchKernelGGL((k_fwd<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), lds, s, a);
   return hipGetLastError();
 }
 
