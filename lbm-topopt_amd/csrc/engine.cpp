@@ -18,6 +18,7 @@
 #include "host_common.hpp"
 #include "lbmx.h"
 #include "lbmx_common.h"
+#include "lbmx_device.hpp"
 #include "lbmx_launch.h"
 
 namespace lbmx {
@@ -134,7 +135,6 @@ struct Engine {
   std::vector<std::shared_ptr<DeviceBuffer>> bc_profiles;  // per-node tables of the pieces (shared by split pieces)
   std::vector<std::shared_ptr<std::vector<double>>> bc_time;  // per-piece time-factor tables (null: none)
   std::vector<DevTerm> terms;       // functional terms
-  bool terms_dirty = true;
   Layout ls{}, ll{};                // padded layouts of state and adjoint fields
   std::vector<double> gamma_host;   // design field
   bool gamma_dirty = true;          // host copy newer than the device copy
@@ -144,7 +144,7 @@ struct Engine {
 
   // device
   DeviceBuffer gamma64_dev;  // FP64 copy of a design set on the device, for the FP32-design forms
-  DeviceBuffer gamma_dev, bc_dev, term_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
+  DeviceBuffer gamma_dev, bc_dev, acc_dev, red_dev, sens_dev, lam_a, lam_b, raw_aos;
   DeviceBuffer nside_dev;  // Neumann pieces: adjoint transfer coefficients of the last adjoint sweep
   DeviceBuffer bcrow_dev;  // boundary pieces meeting each 64-node row segment (bc_row_index)
   int nside_count = 0;     // doubles in nside_dev
@@ -188,12 +188,23 @@ struct Engine {
     N = (long long)d.nx * d.ny * d.nz;
     if (N >= (1LL << 31)) throw Error{LBMX_ERR_SUP, "grids of 2^31 nodes or more are not supported"};
     gamma_host.assign(N, 1.0);
-    ls = make_layout(d.nx, d.ny, d.nz, D, (int)prec_state_bytes(d.precision));
-    ll = make_layout(d.nx, d.ny, d.nz, D, (int)prec_lam_bytes(d.precision));
+    ls = layout_for((int)prec_state_bytes(d.precision));
+    ll = layout_for((int)prec_lam_bytes(d.precision));
     if ((unsigned long long)ls.plane * prec_state_bytes(d.precision) >= (1ULL << 32) ||
         (unsigned long long)ll.plane * prec_lam_bytes(d.precision) >= (1ULL << 32))
       throw Error{LBMX_ERR_SUP, "a population plane must stay below 4 GiB (32-bit lane offsets)"};
     obj.nterm = 0;
+  }
+
+  // layout of a field of this lattice with the given element size
+  Layout layout_for(int elem_bytes) const {
+    switch (d.lattice) {
+      case LBMX_D2Q9: return make_layout(d.nx, d.ny, d.nz, D, elem_bytes, Q, LatD2Q9::ex, LatD2Q9::ey, LatD2Q9::ez);
+      case LBMX_D3Q19:
+        return make_layout(d.nx, d.ny, d.nz, D, elem_bytes, Q, LatD3Q19::ex, LatD3Q19::ey, LatD3Q19::ez);
+      default:
+        return make_layout(d.nx, d.ny, d.nz, D, elem_bytes, Q, LatD2Q9D2Q5::ex, LatD2Q9D2Q5::ey, LatD2Q9D2Q5::ez);
+    }
   }
 
   ~Engine() {
@@ -452,18 +463,13 @@ struct Engine {
         break;
       default: throw Error{LBMX_ERR_ARG, "unknown objective"};
     }
-    int bb[6] = {INT32_MAX, -1, INT32_MAX, -1, INT32_MAX, -1};
-    for (int m = 0; m < a.nterm; ++m)
-      for (int k = 0; k < 3; ++k) {
-        bb[2 * k] = std::min(bb[2 * k], tv[m].box[2 * k]);
-        bb[2 * k + 1] = std::max(bb[2 * k + 1], tv[m].box[2 * k + 1]);
-      }
-    for (int k = 0; k < 6; ++k) a.bbox[k] = bb[k];
-    for (int m = 0; m < a.nterm; ++m)
+    for (int m = 0; m < a.nterm; ++m) {
       for (int k = 0; k < 6; ++k) a.tbox[m][k] = tv[m].box[k];
+      a.tkind[m] = tv[m].kind;
+      a.tweight[m] = tv[m].weight;
+    }
     obj = a;
     terms = tv;
-    terms_dirty = true;
     obj_set = true;
   }
 
@@ -512,13 +518,6 @@ struct Engine {
         HIP_OK(hipStreamSynchronize(stream));
       }
       gamma_dirty = false;
-    }
-    if (terms_dirty) {
-      term_dev.ensure(std::max<size_t>(1, terms.size()) * sizeof(DevTerm));
-      if (!terms.empty())
-        HIP_OK(hipMemcpyAsync(term_dev.p, terms.data(), terms.size() * sizeof(DevTerm), hipMemcpyHostToDevice, stream));
-      HIP_OK(hipStreamSynchronize(stream));
-      terms_dirty = false;
     }
     if (bc_dirty) {
       nside_dev.ensure(std::max<size_t>(1, (size_t)nside_count) * sizeof(double));
@@ -589,7 +588,6 @@ struct Engine {
     a.ls = ls;
     a.m = params();
     a.obj = obj;
-    a.obj.terms = static_cast<const DevTerm*>(term_dev.p);
     a.obj.acc = (with_obj && obj_set && obj.nterm > 0) ? static_cast<double*>(acc_dev.p) : nullptr;
     return a;
   }
@@ -671,7 +669,6 @@ struct Engine {
     a.ll = ll;
     a.m = params();
     a.obj = oa;
-    a.obj.terms = static_cast<const DevTerm*>(term_dev.p);
     a.obj.acc = nullptr;
     return a;
   }
@@ -1029,7 +1026,7 @@ struct Engine {
   }
 
   size_t device_bytes() const {
-    size_t b = gamma64_dev.bytes + gamma_dev.bytes + bc_dev.bytes + term_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
+    size_t b = gamma64_dev.bytes + gamma_dev.bytes + bc_dev.bytes + acc_dev.bytes + red_dev.bytes + sens_dev.bytes + lam_a.bytes +
                lam_b.bytes + raw_aos.bytes + nside_dev.bytes + bcrow_dev.bytes;
     for (auto& p : pool) b += p->bytes;
     return b;
@@ -1123,7 +1120,7 @@ struct Batch {
     for (auto& p : pool) ptr(p->p);
     ptr(lam_a.p); ptr(lam_b.p); ptr(inst_dev.p); ptr(blk_dev.p); ptr(tfac_dev.p);
     for (size_t i = 0; i < m.size(); ++i) {
-      ptr(inst[i].obj.acc); ptr(inst[i].sens); ptr(inst[i].gamma); ptr(inst[i].bcs); ptr(inst[i].obj.terms);
+      ptr(inst[i].obj.acc); ptr(inst[i].sens); ptr(inst[i].gamma); ptr(inst[i].bcs);
       k.push_back(inst[i].state_off);
       k.push_back(inst[i].lam_off);
     }
@@ -1161,7 +1158,6 @@ struct Batch {
       I.m = e.params();
       I.obj = e.obj;
       if (!e.obj_set) I.obj.nterm = 0;
-      I.obj.terms = static_cast<const DevTerm*>(e.term_dev.p);
       I.obj.acc = I.obj.nterm > 0 ? static_cast<double*>(e.acc_dev.p) : nullptr;
       I.gamma = e.gamma_dev.p;
       I.sens = static_cast<double*>(e.sens_dev.p);

@@ -6,6 +6,7 @@
 namespace lbmx {
 
 constexpr int kMaxTerms = 4;
+constexpr int kMaxQ = 19;   // populations per node (D3Q19)
 constexpr int kMaxBC = 64;  // boundary pieces per lattice
 
 // Boundary pieces by row segment: bit b of bcrows[bc_row_index(...)] is set when piece b
@@ -67,10 +68,15 @@ struct ModelParams {
 // lives at element  q*plane + ((z+oz)*py + y+1)*px + ax + x ; rows are 128-byte aligned
 // (ax = 128 bytes / element size) and a one-cell halo surrounds the domain. The halo holds
 // the mid-way bounce-back reflections written by the sweep that produced the field.
+// The byte offsets of each plane's bases are computed once on the host (a sweep's waves would
+// otherwise each rebuild 38-57 64-bit addresses on the scalar unit): the own base q*plane, the
+// pull base (q*plane - shift_q), read at node x - c_q, and the gather base (q*plane + shift_q),
+// read at node x + c_q, with shift_q = c_qx + c_qy px + c_qz px py, times the element size.
 struct Layout {
   int px, py;        // padded row length (elements), padded rows per slice
   int ax, oz;        // x offset of interior x = 0; z offset (1 in 3D, 0 in 2D)
   long long plane;   // elements per population plane
+  long long own_b[kMaxQ], pull_b[kMaxQ], gath_b[kMaxQ];
 };
 
 struct Geometry {
@@ -82,11 +88,10 @@ struct Geometry {
 
 struct ObjArgs {
   int nterm;
-  int bbox[6];        // union of term boxes (skip test)
-  int pad;
-  int tbox[kMaxTerms][6];  // the term boxes by value: the box tests need no memory loads
+  int tbox[kMaxTerms][6];  // the terms by value (boxes, kinds, weights): no memory loads
+  int tkind[kMaxTerms];
+  double tweight[kMaxTerms];
   double time_const;
-  const DevTerm* terms;  // device array of nterm terms
   double* acc;           // running sums per 64-node row segment (FP64), null = off
 };
 
@@ -163,7 +168,9 @@ struct BatchStep {
   int with_obj;           // forward: accumulate the functional; adjoint: add its source
 };
 
-inline Layout make_layout(int nx, int ny, int nz, int dim, int elem_bytes) {
+// Q directions c_q = (ex[q], ey[q], ez[q]) of the field's lattice
+inline Layout make_layout(int nx, int ny, int nz, int dim, int elem_bytes, int Q, const int* ex, const int* ey,
+                          const int* ez) {
   Layout l{};
   l.ax = 128 / elem_bytes;
   int rowlen = ((nx + 1 + l.ax - 1) / l.ax) * l.ax;
@@ -172,6 +179,12 @@ inline Layout make_layout(int nx, int ny, int nz, int dim, int elem_bytes) {
   l.oz = dim == 3 ? 1 : 0;
   long long pz = dim == 3 ? nz + 2 : 1;
   l.plane = (long long)l.px * l.py * pz;
+  for (int q = 0; q < Q && q < kMaxQ; ++q) {
+    const long long sh = (long long)ex[q] + (long long)ey[q] * l.px + (long long)ez[q] * l.px * l.py;
+    l.own_b[q] = (long long)q * l.plane * elem_bytes;
+    l.pull_b[q] = ((long long)q * l.plane - sh) * elem_bytes;
+    l.gath_b[q] = ((long long)q * l.plane + sh) * elem_bytes;
+  }
   return l;
 }
 

@@ -377,8 +377,9 @@ __device__ __forceinline__ T bc_value(const DevBC& b, int D, T tf, int x, int y,
   return T(b.value) * shape * tf;
 }
 
+// (bitwise, not short-circuit: all six bounds are read at once rather than one after another)
 __device__ __forceinline__ bool in_box(const int* bx, int x, int y, int z) {
-  return x >= bx[0] && x <= bx[1] && y >= bx[2] && y <= bx[3] && z >= bx[4] && z <= bx[5];
+  return (x >= bx[0]) & (x <= bx[1]) & (y >= bx[2]) & (y <= bx[3]) & (z >= bx[4]) & (z <= bx[5]);
 }
 
 // on-grid bounce-back: q takes opp(q) when c_q.n <= 0 (incoming or tangential), per

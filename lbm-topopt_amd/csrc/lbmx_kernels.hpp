@@ -22,11 +22,6 @@
 
 namespace lbmx {
 
-template <class L, int q>
-__device__ __forceinline__ long long shift_of(const Layout& l) {
-  return (long long)L::ex[q] + (long long)L::ey[q] * l.px + (long long)L::ez[q] * l.px * l.py;
-}
-
 template <class S>
 __device__ __forceinline__ S ld(const char* base, unsigned int off_bytes) {
   return *reinterpret_cast<const S*>(base + off_bytes);
@@ -44,27 +39,30 @@ __device__ __forceinline__ void st_nt(char* base, unsigned int off_bytes, S v) {
   __builtin_nontemporal_store(v, reinterpret_cast<S*>(base + off_bytes));
 }
 
+// Plane bases from the layout's byte offsets (built on the host for the field's element size,
+// which S names here: sizeof(S) of the field's storage type)
 // pull address base of plane q: start + (q*plane - shift_q) elements
 template <class L, int q, class S>
 __device__ __forceinline__ const char* pull_base(const void* start, const Layout& l) {
-  return static_cast<const char*>(start) + ((long long)q * l.plane - shift_of<L, q>(l)) * (long long)sizeof(S);
+  static_assert(q < kMaxQ, "plane index");
+  return static_cast<const char*>(start) + l.pull_b[q];
 }
 template <class L, int q, class S>
 __device__ __forceinline__ char* pull_base_w(void* start, const Layout& l) {
-  return static_cast<char*>(start) + ((long long)q * l.plane - shift_of<L, q>(l)) * (long long)sizeof(S);
+  return static_cast<char*>(start) + l.pull_b[q];
 }
 // gather address base of plane q: start + (q*plane + shift_q) elements (node x + c_q)
 template <class L, int q, class S>
 __device__ __forceinline__ const char* gather_base(const void* start, const Layout& l) {
-  return static_cast<const char*>(start) + ((long long)q * l.plane + shift_of<L, q>(l)) * (long long)sizeof(S);
+  return static_cast<const char*>(start) + l.gath_b[q];
 }
 template <int q, class S>
 __device__ __forceinline__ char* own_base_w(void* start, const Layout& l) {
-  return static_cast<char*>(start) + (long long)q * l.plane * (long long)sizeof(S);
+  return static_cast<char*>(start) + l.own_b[q];
 }
 template <int q, class S>
 __device__ __forceinline__ const char* own_base(const void* start, const Layout& l) {
-  return static_cast<const char*>(start) + (long long)q * l.plane * (long long)sizeof(S);
+  return static_cast<const char*>(start) + l.own_b[q];
 }
 
 __device__ __forceinline__ unsigned int layout_off(const Layout& l, int x, int y, int z) {
@@ -381,17 +379,27 @@ __global__ __launch_bounds__(64) void k_neumann_fix(void* lam, Layout ll, DevBC 
 // segment's accumulator (one writer per segment and sweep, sweeps in stream order), and the host
 // adds the accumulators in segment order. The result does not depend on the workgroup tile, and
 // no workgroup barrier is needed.
+// The tests below combine with bitwise, not short-circuit, operators: every wave evaluates them
+// before its pulls, and a short-circuit chain would read the term boxes from the argument
+// segment one after another, each read waiting for the previous test. Their coordinates are
+// the wave's (segment_coords), so that they run on the scalar unit.
+// first x, y and z of the calling wave's row segment, in scalar registers (a wave covers one
+// 64-node row segment)
+__device__ __forceinline__ void segment_coords(int x, int y, int z, int& x0, int& ys, int& zs) {
+  x0 = __builtin_amdgcn_readfirstlane(x - (int)threadIdx.x);
+  ys = __builtin_amdgcn_readfirstlane(y);
+  zs = __builtin_amdgcn_readfirstlane(z);
+}
 __device__ __forceinline__ bool segment_meets_box(const int* bb, int x0, int y, int z) {
-  return !(x0 + kTileX - 1 < bb[0] || x0 > bb[1] || y < bb[2] || y > bb[3] || z < bb[4] || z > bb[5]);
+  return (x0 + kTileX - 1 >= bb[0]) & (x0 <= bb[1]) & (y >= bb[2]) & (y <= bb[3]) & (z >= bb[4]) & (z <= bb[5]);
 }
 // (the term loops are unrolled over kMaxTerms: a runtime index into the by-value box array
 // would place the argument structure in scratch memory)
 __device__ __forceinline__ bool segment_meets_terms(const ObjArgs& o, int x0, int y, int z) {
-  if (o.nterm <= 0 || !segment_meets_box(o.bbox, x0, y, z)) return false;
   bool meets = false;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    meets = meets || (m < o.nterm && segment_meets_box(o.tbox[m], x0, y, z));
+    meets = meets | ((m < o.nterm) & segment_meets_box(o.tbox[m], x0, y, z));
   });
   return meets;
 }
@@ -399,23 +407,25 @@ __device__ __forceinline__ bool in_some_term(const ObjArgs& o, int x, int y, int
   bool in = false;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    in = in || (m < o.nterm && in_box(o.tbox[m], x, y, z));
+    in = in | ((m < o.nterm) & in_box(o.tbox[m], x, y, z));
   });
   return in;
 }
 // Wave sum of v over the lanes whose node lies in a term (`contrib`; v is 0 elsewhere), added
-// to acc[seg] by one lane; every lane of the wave calls it. A segment crossed by a face term
-// has one such lane, which adds its value itself: the tree sum would return that value
-// unchanged, since the other lanes add exact zeros.
+// to acc[seg] by one lane. A segment crossed by a face term has one such lane, which adds its
+// value itself: the tree sum would return that value unchanged, since the other lanes add
+// exact zeros. The addition is an atomic without return, so the wave does not wait for the
+// accumulator's read; each segment has one adder per sweep and sweeps run in stream order, so
+// the additions into acc[seg] happen in sweep order. Every lane of the wave calls it.
 __device__ __forceinline__ void add_segment_sum(double* acc, long long seg, double v, bool contrib) {
   const unsigned long long who = __ballot(contrib);
   if (who == 0ull) return;
   if ((who & (who - 1)) == 0ull) {
-    if (contrib) acc[seg] += v;
+    if (contrib) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if (threadIdx.x == 0) acc[seg] += v;
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class L, int COLL, class T>
@@ -425,8 +435,8 @@ __device__ __forceinline__ T node_objective(const ObjArgs& o, const T (&f)[L::Q]
   T acc = 0;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    if (m < o.nterm && in_box(o.tbox[m], x, y, z))
-      acc += T(o.terms[m].weight) * term_value<L, COLL, T>(o.terms[m].kind, f, rho, j);
+    if ((m < o.nterm) & in_box(o.tbox[m], x, y, z))
+      acc += T(o.tweight[m]) * term_value<L, COLL, T>(o.tkind[m], f, rho, j);
   });
   return acc * T(o.time_const);
 }
@@ -438,11 +448,12 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   using T = typename P::T;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool active = x < nx && y < ny;
-  const int x0 = x - (int)threadIdx.x;
-  const bool objseg = a.obj.acc != nullptr && a.mode == 0 && y < ny && segment_meets_terms(a.obj, x0, y, z);
-  const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x0, y, z, nx, ny, nz);
+  int x0, ys, zs;
+  segment_coords(x, y, z, x0, ys, zs);
+  const bool objseg = a.obj.acc != nullptr && a.mode == 0 && (ys < ny) & segment_meets_terms(a.obj, x0, ys, zs);
+  const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x0, ys, zs, nx, ny, nz);
   double objv = 0.0;
-  bool objnode = false;
+  const bool objnode = objseg & active & in_some_term(a.obj, x, y, z);
   const unsigned int off = active ? layout_off(a.ls, x, y, z) : 0u;
   const int n = (z * ny + y) * nx + x;
   const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
@@ -464,9 +475,8 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
       static_for<0, L::Q>([&](auto qc) { out[(long long)n * L::Q + decltype(qc)::value] = double(f[decltype(qc)::value]); });
-    } else {
-      objnode = objseg && in_some_term(a.obj, x, y, z);
-      if (objnode) objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
+    } else if (objnode) {
+      objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
     }
   }
   // the segment's sum is added before the collision, so that nothing of it stays live there
@@ -546,8 +556,10 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
   using LS = typename P::LS;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const bool active = x < nx && y < ny;
-  const bool bseg = a.nbc > 0 && segment_on_boundary<L>(x - (int)threadIdx.x, y, z, nx, ny, nz);
-  const bool objseg = a.obj.nterm > 0 && segment_meets_terms(a.obj, x - (int)threadIdx.x, y, z);
+  int x0, ys, zs;
+  segment_coords(x, y, z, x0, ys, zs);
+  const bool bseg = a.nbc > 0 && segment_on_boundary<L>(x0, ys, zs, nx, ny, nz);
+  const bool objseg = a.obj.nterm > 0 && segment_meets_terms(a.obj, x0, ys, zs);
   const int n = (z * ny + y) * nx + x;
   const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
   unsigned long long rowword = 0;
@@ -584,15 +596,15 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
     });
   }
   T srcA = 0, srcB[3] = {0, 0, 0}, srcAT = 0;
-  if (objseg && in_some_term(a.obj, x, y, z)) {
+  if (objseg & in_some_term(a.obj, x, y, z)) {
     T rho, j[3];
     flow_macros<L>(f, rho, j);
     T temp = 0;
     if constexpr (L::QT > 0) temp = f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
     static_for<0, kMaxTerms>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      if (m < a.obj.nterm && in_box(a.obj.tbox[m], x, y, z))
-        term_source<T>(a.obj.terms[m].kind, rho, j, temp, T(a.obj.time_const * a.obj.terms[m].weight), srcA, srcB,
+      if ((m < a.obj.nterm) & in_box(a.obj.tbox[m], x, y, z))
+        term_source<T>(a.obj.tkind[m], rho, j, temp, T(a.obj.time_const * a.obj.tweight[m]), srcA, srcB,
                        srcAT);
     });
   }
@@ -713,9 +725,13 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   const int nx = a.g.nx, ny = a.g.ny;
   const int y2 = y + H;
   const bool act1 = x < nx && y < ny, act2 = x < nx && y2 < ny;
-  const int x0 = x - (int)threadIdx.x;
-  const bool objseg1 = a.obj.acc != nullptr && y < ny && segment_meets_terms(a.obj, x0, y, z);
-  const bool objseg2 = a.obj.acc != nullptr && y2 < ny && segment_meets_terms(a.obj, x0, y2, z);
+  int x0, ys, zs;
+  segment_coords(x, y, z, x0, ys, zs);
+  const int ys2 = ys + H;
+  const bool objseg1 = a.obj.acc != nullptr && (ys < ny) & segment_meets_terms(a.obj, x0, ys, zs);
+  const bool objseg2 = a.obj.acc != nullptr && (ys2 < ny) & segment_meets_terms(a.obj, x0, ys2, zs);
+  const bool on1 = act1 & objseg1 & in_some_term(a.obj, x, y, z);
+  const bool on2 = act2 & objseg2 & in_some_term(a.obj, x, y2, z);
   const typename P::G* gp = static_cast<const typename P::G*>(a.gamma);
   T f1[L::Q], f2[L::Q], g1 = 0, g2 = 0;
   unsigned long long rw1 = 0, rw2 = 0;
@@ -732,13 +748,11 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
     pull_node<L, P, 2>(a.src, a.ls, off2, f2);
     g2 = T(gp[(z * ny + y2) * nx + x]);
   }
-  const bool bseg1 = a.nbc > 0 && segment_on_boundary<L>(x0, y, z, nx, ny, a.g.nz);
-  const bool bseg2 = a.nbc > 0 && segment_on_boundary<L>(x0, y2, z, nx, ny, a.g.nz);
+  const bool bseg1 = a.nbc > 0 && segment_on_boundary<L>(x0, ys, zs, nx, ny, a.g.nz);
+  const bool bseg2 = a.nbc > 0 && segment_on_boundary<L>(x0, ys2, zs, nx, ny, a.g.nz);
   const unsigned long long p1 = bseg1 ? bc_pieces(rw1) : 0ull, p2 = bseg2 ? bc_pieces(rw2) : 0ull;
   if (bseg1 || bseg2) stage_wave_pieces(a.bcs, tfac, a.nbc, p1 | p2);
   double obj1 = 0.0, obj2 = 0.0;
-  const bool on1 = act1 && objseg1 && in_some_term(a.obj, x, y, z);
-  const bool on2 = act2 && objseg2 && in_some_term(a.obj, x, y2, z);
   const bool bnd1 = act1 && on_boundary<L>(x, y, z, nx, ny, a.g.nz);
   const bool bnd2 = act2 && on_boundary<L>(x, y2, z, nx, ny, a.g.nz);
   if (act1) obj1 = fwd_bc_obj<L, COLL, DES, P>(a, f1, p1, x, y, z, bnd1, on1);
