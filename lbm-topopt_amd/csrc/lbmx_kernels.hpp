@@ -453,7 +453,8 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   const bool objseg = a.obj.acc != nullptr && a.mode == 0 && (ys < ny) & segment_meets_terms(a.obj, x0, ys, zs);
   const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x0, ys, zs, nx, ny, nz);
   double objv = 0.0;
-  const bool objnode = objseg & active & in_some_term(a.obj, x, y, z);
+  // (objseg is the wave's: the per-node test is skipped by whole waves)
+  const bool objnode = objseg && (active & in_some_term(a.obj, x, y, z));
   const unsigned int off = active ? layout_off(a.ls, x, y, z) : 0u;
   const int n = (z * ny + y) * nx + x;
   const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
@@ -596,7 +597,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
     });
   }
   T srcA = 0, srcB[3] = {0, 0, 0}, srcAT = 0;
-  if (objseg & in_some_term(a.obj, x, y, z)) {
+  if (objseg && in_some_term(a.obj, x, y, z)) {
     T rho, j[3];
     flow_macros<L>(f, rho, j);
     T temp = 0;
@@ -638,12 +639,25 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return xcd < r ? xcd * (qd + 1) + k : r * (qd + 1) + (xcd - r) * qd + k;
 }
 
+// The member table is written by the host before the batch runs and is read-only for every
+// kernel, so it is read through the constant address space: its fields are then scalar
+// loads wherever the compiler places them (a read of the global table after the sweep's first
+// store could otherwise be a vector load, as the store might alias it).
+using ConstInst = const __attribute__((address_space(4))) BatchInst;
+// a structure of the member table, copied out of the constant address space
+template <class T>
+__device__ __forceinline__ T from_const(const __attribute__((address_space(4))) T& src) {
+  T v;
+  __builtin_memcpy(&v, (const T*)&src, sizeof(T));
+  return v;
+}
+
 template <int TY>
-__device__ __forceinline__ const BatchInst& batch_locate(const BatchStep& st, int& inst, int& x, int& y, int& z,
-                                                         int& tile) {
+__device__ __forceinline__ ConstInst& batch_locate(const BatchStep& st, int& inst, int& x, int& y, int& z,
+                                                   int& tile) {
   const int b = xcd_remap(blockIdx.x, st.nblocks);
   inst = st.blk_inst[b];
-  const BatchInst& I = st.inst[inst];
+  ConstInst& I = *((ConstInst*)(st.inst) + inst);
   tile = b - I.blk_begin;
   const int bx = tile % I.g.nblk_x;
   const int rest = tile / I.g.nblk_x;
@@ -657,7 +671,7 @@ __device__ __forceinline__ const BatchInst& batch_locate(const BatchStep& st, in
 template <class L, int COLL, int DES, class P, int TY>
 __global__ __launch_bounds__(kTileX* TY) void k_fwd_batch(BatchStep st) {
   int i, x, y, z, tile;
-  const BatchInst& I = batch_locate<TY>(st, i, x, y, z, tile);
+  ConstInst& I = batch_locate<TY>(st, i, x, y, z, tile);
   FwdArgs a;
   a.src = static_cast<const char*>(st.src) + I.state_off;
   a.dst = static_cast<char*>(st.dst) + I.state_off;
@@ -668,10 +682,10 @@ __global__ __launch_bounds__(kTileX* TY) void k_fwd_batch(BatchStep st) {
   a.t = st.t;
   a.mode = 0;
   a.aos_out = nullptr;
-  a.g = I.g;
-  a.ls = I.ls;
-  a.m = I.m;
-  a.obj = I.obj;
+  a.g = from_const(I.g);
+  a.ls = from_const(I.ls);
+  a.m = from_const(I.m);
+  a.obj = from_const(I.obj);
   if (!st.with_obj) a.obj.acc = nullptr;
   fwd_node<L, COLL, DES, P, TY>(a, st.tfac + (long long)i * st.tf_stride, x, y, z, tile);
 }
@@ -679,7 +693,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_fwd_batch(BatchStep st) {
 template <class L, int COLL, int DES, class P, int TY>
 __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   int i, x, y, z, tile;
-  const BatchInst& I = batch_locate<TY>(st, i, x, y, z, tile);
+  ConstInst& I = batch_locate<TY>(st, i, x, y, z, tile);
   AdjArgs a;
   a.fwd_src = static_cast<const char*>(st.src) + I.state_off;
   a.fwd_aos = nullptr;
@@ -692,11 +706,11 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   a.bcrows = I.bcrows;
   a.nbc = I.nbc;
   a.t = st.t;
-  a.g = I.g;
-  a.ls = I.ls;
-  a.ll = I.ll;
-  a.m = I.m;
-  a.obj = I.obj;
+  a.g = from_const(I.g);
+  a.ls = from_const(I.ls);
+  a.ll = from_const(I.ll);
+  a.m = from_const(I.m);
+  a.obj = from_const(I.obj);
   a.obj.acc = nullptr;
   if (!st.with_obj) a.obj.nterm = 0;
   adj_node<L, COLL, DES, P, TY>(a, st.tfac + (long long)i * st.tf_stride, x, y, z);
@@ -730,8 +744,8 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   const int ys2 = ys + H;
   const bool objseg1 = a.obj.acc != nullptr && (ys < ny) & segment_meets_terms(a.obj, x0, ys, zs);
   const bool objseg2 = a.obj.acc != nullptr && (ys2 < ny) & segment_meets_terms(a.obj, x0, ys2, zs);
-  const bool on1 = act1 & objseg1 & in_some_term(a.obj, x, y, z);
-  const bool on2 = act2 & objseg2 & in_some_term(a.obj, x, y2, z);
+  const bool on1 = objseg1 && (act1 & in_some_term(a.obj, x, y, z));
+  const bool on2 = objseg2 && (act2 & in_some_term(a.obj, x, y2, z));
   const typename P::G* gp = static_cast<const typename P::G*>(a.gamma);
   T f1[L::Q], f2[L::Q], g1 = 0, g2 = 0;
   unsigned long long rw1 = 0, rw2 = 0;
@@ -774,7 +788,7 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
 template <class L, int COLL, int DES, class P, int H>
 __global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
   int i, x, y, z, tile;
-  const BatchInst& I = batch_locate<2 * H>(st, i, x, y, z, tile);
+  ConstInst& I = batch_locate<2 * H>(st, i, x, y, z, tile);
   FwdArgs a;
   a.src = static_cast<const char*>(st.src) + I.state_off;
   a.dst = static_cast<char*>(st.dst) + I.state_off;
@@ -785,10 +799,10 @@ __global__ __launch_bounds__(kTileX* H) void k_fwd_batch_pair(BatchStep st) {
   a.t = st.t;
   a.mode = 0;
   a.aos_out = nullptr;
-  a.g = I.g;
-  a.ls = I.ls;
-  a.m = I.m;
-  a.obj = I.obj;
+  a.g = from_const(I.g);
+  a.ls = from_const(I.ls);
+  a.m = from_const(I.m);
+  a.obj = from_const(I.obj);
   if (!st.with_obj) a.obj.acc = nullptr;
   fwd_pair<L, COLL, DES, P, H>(a, st.tfac + (long long)i * st.tf_stride, x, y, z, tile);
 }
