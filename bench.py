@@ -121,7 +121,7 @@ def run_form(lbmx, workloads, form, nx, ny, nz, horizon, slots, steps, warmup, s
     return res
 
 
-def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement):
+def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement, more_slots=None):
     """Robust ensemble (24 members: 3 projection realisations x 8 candidates on uneven grids,
     PBB<IncompressibleCascaded2d> at tau = 0.6, FP32): batched gradient vs the same 24
     gradients as sequential single-instance runs. Aggregate MLUPS = total nodes x steps / s."""
@@ -166,6 +166,22 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement):
         raise RuntimeError(f"ensemble objective not finite: batched {Jb}, single {Js}")
     jerr = float(max(np.max(np.abs(Jb - Js) / np.abs(Js)), np.max(np.abs(Jg - Js) / np.abs(Js))))
     dev = b.device_bytes()
+    # the same batched pass with more of HBM given to checkpoints (fewer recomputed sweeps)
+    more = {}
+    if more_slots and more_slots > slots:
+        b.checkpoint_config(more_slots, mode)
+        b.gradient(steps, want_sens=False)
+        t0 = time.perf_counter()
+        Jm, _ = b.gradient(steps)
+        tm = time.perf_counter() - t0
+        stm = b.stats()
+        if not np.array_equal(Jm, Jb):
+            raise RuntimeError("ensemble objective depends on the checkpoint count")
+        more = {"checkpoints": more_slots, "batched_s": round(tm, 3),
+                "batched_mlups": round(nodes * steps / tm / 1e6, 1),
+                "recompute_sweeps": int(stm.get("recompute_sweeps", -1)),
+                "batch_device_gb": round(b.device_bytes() / 1e9, 2)}
+        log(f"ensemble: batched gradient with {more_slots} checkpoints {tm:.2f} s")
     b.close()
     for m in members:
         m.close()
@@ -178,7 +194,7 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement):
             "batched_graph_mlups": round(nodes * steps / tg / 1e6, 1),
             "sequential_mlups": round(nodes * steps / ts / 1e6, 1),
             "speedup": round(ts / tb, 3), "max_rel_J_diff_batched_vs_single": jerr,
-            "batch_device_gb": round(dev / 1e9, 2)}
+            "batch_device_gb": round(dev / 1e9, 2), "more_checkpoints": more}
 
 
 def cone_taps(R, dims):
@@ -591,6 +607,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=3000,
                     help="time steps of the batched robust-ensemble measurement (0: skip it)")
     ap.add_argument("--ensemble-slots", type=int, default=64)
+    ap.add_argument("--ensemble-more-slots", type=int, default=96,
+                    help="also time the batched pass with this many checkpoints (0: skip)")
     ap.add_argument("--no-design-pipeline", action="store_true")
     ap.add_argument("--no-mma", action="store_true")
     ap.add_argument("--no-other-workloads", action="store_true",
@@ -672,7 +690,7 @@ def main():
         out["fp32_over_fp64"] = round(res["fp32"]["mlups_fwd_adj"] / res["fp64"]["mlups_fwd_adj"], 3)
     if args.ensemble_steps > 0:
         out["ensemble"] = run_ensemble(lbmx, workloads, args.ensemble_steps, args.ensemble_slots, min(args.warmup, 1),
-                                       mode, args.placement)
+                                       mode, args.placement, args.ensemble_more_slots)
     if not args.no_design_pipeline:
         out["design_pipeline"] = run_design_pipeline(lbmx, cpu=not args.no_cpu_baseline)
     if not args.no_mma:

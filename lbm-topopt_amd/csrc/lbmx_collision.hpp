@@ -20,11 +20,6 @@ enum : int { D_NONE = LBMX_DESIGN_NONE, D_PBB = LBMX_DESIGN_PBB, D_BRINK = LBMX_
 
 #define LBMX_BAS BasisOf<L::QF>::B
 
-// Multiply-adds are fused only within one statement in the functions marked with this, so that
-// their rounding does not depend on the kernel they are inlined into (cross-statement fusion
-// follows the basic blocks and the uses of each product there). Instances of one operator
-// in different kernels (single lattice, batch, tile shapes) then give bitwise equal states.
-#define LBMX_CONTRACT_IN_STATEMENT _Pragma("clang fp contract(on)")
 
 template <class T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
 // a model parameter in the arithmetic type (FP32 forms read the copy rounded on the host)

@@ -11,6 +11,13 @@
 
 namespace lbmx {
 
+// Multiply-adds are fused only within one statement in the functions marked with this, so that
+// their rounding does not depend on the kernel they are inlined into (cross-statement fusion
+// follows the basic blocks and the uses of each product there). Instances of one operator
+// or functional in different kernels (single lattice, batch, tile shapes) then give bitwise
+// equal results.
+#define LBMX_CONTRACT_IN_STATEMENT _Pragma("clang fp contract(on)")
+
 // ---------------------------------------------------------------------------------------
 // Lattices (velocity order of src/latticeBoltzmann/lattices2d.hh:16-84 and
 // lattices3d.hh:34-53: direction q and q+HALF are opposite).
@@ -742,20 +749,24 @@ template <int COLL> struct CollTraits {
 // ---------------------------------------------------------------------------------------
 // Functional terms: value and gradient w.r.t. the node's pre-collision populations.
 // ---------------------------------------------------------------------------------------
-template <class L, int COLL, class T>
-__device__ __forceinline__ T term_value(int kind, const T (&f)[L::Q], T rho, const T (&j)[3]) {
+// value of a term from the node's pre-collision density, momentum and temperature (the sum of
+// the D2Q5 populations; 0 on the flow lattices)
+template <int COLL, class T>
+__device__ __forceinline__ T term_value(int kind, T rho, const T (&j)[3], T temp) {
+  LBMX_CONTRACT_IN_STATEMENT
   constexpr bool inc = CollTraits<COLL>::inc;
   switch (kind) {
     case 0: return rho;
     case 1: return inc ? j[0] : j[0] / rho;
     case 2: return (rho / T(3) + T(0.5) * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2])) * j[0];
-    case 3: {
-      T temp = 0;
-      if constexpr (L::QT > 0) temp = f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
-      return j[0] * temp;
-    }
+    case 3: return j[0] * temp;
   }
   return T(0);
+}
+template <class L, class T>
+__device__ __forceinline__ T node_temperature(const T (&f)[L::Q]) {
+  if constexpr (L::QT > 0) return f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
+  else return T(0);
 }
 
 // Gradient of a term in the form A + c_q.B on the flow populations (+ AT on the thermal ones).

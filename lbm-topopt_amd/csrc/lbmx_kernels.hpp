@@ -418,6 +418,7 @@ __device__ __forceinline__ bool in_some_term(const ObjArgs& o, int x, int y, int
 // accumulator's read; each segment has one adder per sweep and sweeps run in stream order, so
 // the additions into acc[seg] happen in sweep order. Every lane of the wave calls it.
 __device__ __forceinline__ void add_segment_sum(double* acc, long long seg, double v, bool contrib) {
+  LBMX_CONTRACT_IN_STATEMENT
   const unsigned long long who = __ballot(contrib);
   if (who == 0ull) return;
   if ((who & (who - 1)) == 0ull) {
@@ -428,15 +429,15 @@ __device__ __forceinline__ void add_segment_sum(double* acc, long long seg, doub
   if (threadIdx.x == 0) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <class L, int COLL, class T>
-__device__ __forceinline__ T node_objective(const ObjArgs& o, const T (&f)[L::Q], int x, int y, int z) {
-  T rho, j[3];
-  flow_macros<L>(f, rho, j);
+// The node's functional value from its pre-collision macros (the collision returns rho and
+// j), evaluated after the node's stores so that it is off the sweep's critical path.
+template <int COLL, class T>
+__device__ __forceinline__ T node_objective(const ObjArgs& o, T rho, const T (&j)[3], T temp, int x, int y, int z) {
+  LBMX_CONTRACT_IN_STATEMENT
   T acc = 0;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    if ((m < o.nterm) & in_box(o.tbox[m], x, y, z))
-      acc += T(o.tweight[m]) * term_value<L, COLL, T>(o.tkind[m], f, rho, j);
+    if ((m < o.nterm) & in_box(o.tbox[m], x, y, z)) acc += T(o.tweight[m]) * term_value<COLL, T>(o.tkind[m], rho, j, temp);
   });
   return acc * T(o.time_const);
 }
@@ -476,18 +477,17 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
     if (a.mode == 1) {
       double* out = static_cast<double*>(a.aos_out);
       static_for<0, L::Q>([&](auto qc) { out[(long long)n * L::Q + decltype(qc)::value] = double(f[decltype(qc)::value]); });
-    } else if (objnode) {
-      objv = double(node_objective<L, COLL, T>(a.obj, f, x, y, z));
     }
   }
-  // the segment's sum is added before the collision, so that nothing of it stays live there
-  if (objseg) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), objv, objnode);
   if (active && a.mode != 1) {
     const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
+    const T temp = objnode ? node_temperature<L>(f) : T(0);
     T rho, j[3];
     collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+    if (objnode) objv = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
     store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
   }
+  if (objseg) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), objv, objnode);
 }
 
 template <class L, int COLL, int DES, class P>
@@ -722,16 +722,6 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
 // occupancy; each node's arithmetic is that of fwd_node, in the same order. (The adjoint,
 // which already has three fields in flight per node, keeps one node per thread: paired, it
 // was slower.)
-// Boundary pieces and the functional of one node of a pair: returns the node's objective term
-// (0 off the functional's terms)
-template <class L, int COLL, int DES, class P>
-__device__ __forceinline__ double fwd_bc_obj(const FwdArgs& a, typename P::T (&f)[L::Q], unsigned long long pieces,
-                                             int x, int y, int z, bool bnd, bool objnode) {
-  using T = typename P::T;
-  if (bnd) apply_bcs<L, P>(f, staged_pieces(), pieces, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
-  return objnode ? double(node_objective<L, COLL, T>(a.obj, f, x, y, z)) : 0.0;
-}
-
 // Rows y and y + H of the node's tile (2H rows, H waves).
 template <class L, int COLL, int DES, class P, int H>
 __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, int x, int y, int z, int tile) {
@@ -769,20 +759,23 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   double obj1 = 0.0, obj2 = 0.0;
   const bool bnd1 = act1 && on_boundary<L>(x, y, z, nx, ny, a.g.nz);
   const bool bnd2 = act2 && on_boundary<L>(x, y2, z, nx, ny, a.g.nz);
-  if (act1) obj1 = fwd_bc_obj<L, COLL, DES, P>(a, f1, p1, x, y, z, bnd1, on1);
-  if (act2) obj2 = fwd_bc_obj<L, COLL, DES, P>(a, f2, p2, x, y2, z, bnd2, on2);
-  // the segment sums are added before the collisions, so that nothing of them stays live there
-  if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1, on1);
-  if (objseg2) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y2, z), obj2, on2);
+  if (bnd1) apply_bcs<L, P>(f1, staged_pieces(), p1, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
+  if (bnd2) apply_bcs<L, P>(f2, staged_pieces(), p2, staged_tfac(a.nbc), x, y2, z, a.src, a.ls);
   T rho, j[3];
   if (act1) {
+    const T temp = on1 ? node_temperature<L>(f1) : T(0);
     collide<L, COLL, DES, T>(f1, g1, a.m, rho, j);
     store_node<L, P>(a.dst, a.ls, off1, f1, bnd1, x, y, z, nx, ny, a.g.nz, a.t);
+    if (on1) obj1 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
   }
   if (act2) {
+    const T temp = on2 ? node_temperature<L>(f2) : T(0);
     collide<L, COLL, DES, T>(f2, g2, a.m, rho, j);
     store_node<L, P>(a.dst, a.ls, off2, f2, bnd2, x, y2, z, nx, ny, a.g.nz, a.t);
+    if (on2) obj2 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y2, z));
   }
+  if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1, on1);
+  if (objseg2) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y2, z), obj2, on2);
 }
 
 template <class L, int COLL, int DES, class P, int H>
