@@ -174,12 +174,11 @@ def run_ensemble(lbmx, workloads, steps, slots, warmup, mode, placement, more_sl
         t0 = time.perf_counter()
         Jm, _ = b.gradient(steps)
         tm = time.perf_counter() - t0
-        stm = b.stats()
         if not np.array_equal(Jm, Jb):
             raise RuntimeError("ensemble objective depends on the checkpoint count")
         more = {"checkpoints": more_slots, "batched_s": round(tm, 3),
                 "batched_mlups": round(nodes * steps / tm / 1e6, 1),
-                "recompute_sweeps": int(stm.get("recompute_sweeps", -1)),
+                "recompute_sweeps": int(lbmx.checkpoint_simulate(more_slots, steps, mode)[0]),
                 "batch_device_gb": round(b.device_bytes() / 1e9, 2)}
         log(f"ensemble: batched gradient with {more_slots} checkpoints {tm:.2f} s")
     b.close()
