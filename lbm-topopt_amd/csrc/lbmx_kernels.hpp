@@ -430,7 +430,7 @@ __device__ __forceinline__ void add_segment_sum(double* acc, long long seg, doub
 }
 
 // The node's functional value from its pre-collision macros (the collision returns rho and
-// j), evaluated after the node's stores so that it is off the sweep's critical path.
+// j, so they are not summed again); the sweeps add the segment's sum after the node's stores.
 template <int COLL, class T>
 __device__ __forceinline__ T node_objective(const ObjArgs& o, T rho, const T (&j)[3], T temp, int x, int y, int z) {
   LBMX_CONTRACT_IN_STATEMENT
@@ -765,14 +765,14 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   if (act1) {
     const T temp = on1 ? node_temperature<L>(f1) : T(0);
     collide<L, COLL, DES, T>(f1, g1, a.m, rho, j);
-    store_node<L, P>(a.dst, a.ls, off1, f1, bnd1, x, y, z, nx, ny, a.g.nz, a.t);
     if (on1) obj1 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
+    store_node<L, P>(a.dst, a.ls, off1, f1, bnd1, x, y, z, nx, ny, a.g.nz, a.t);
   }
   if (act2) {
     const T temp = on2 ? node_temperature<L>(f2) : T(0);
     collide<L, COLL, DES, T>(f2, g2, a.m, rho, j);
-    store_node<L, P>(a.dst, a.ls, off2, f2, bnd2, x, y2, z, nx, ny, a.g.nz, a.t);
     if (on2) obj2 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y2, z));
+    store_node<L, P>(a.dst, a.ls, off2, f2, bnd2, x, y2, z, nx, ny, a.g.nz, a.t);
   }
   if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1, on1);
   if (objseg2) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y2, z), obj2, on2);

@@ -172,8 +172,9 @@ def test_thermal_channel_2048x1024_forms_full_size(lbmx_mod):
         s.close()
     J64, g64 = out[lbmx.FP64]
     assert np.isfinite(J64) and abs(J64) > 0
-    for prec, (tj, tg) in ((lbmx.FP32, (2e-4, 3e-4)), (lbmx.FP16_SHIFTED, (1e-3, 1e-3)),
-                           (lbmx.BF16_SHIFTED, (1e-2, 1e-2))):
+    # measured: FP32 J 7.3e-5, gradient 1.0e-4; FP16 7.3e-4, 5.9e-4; BF16 1.8e-3, 2.7e-3 to 3.4e-3
+    for prec, (tj, tg) in ((lbmx.FP32, (1.5e-4, 2e-4)), (lbmx.FP16_SHIFTED, (9e-4, 8e-4)),
+                           (lbmx.BF16_SHIFTED, (2.5e-3, 4.5e-3))):
         J, g = out[prec]
         ej = abs(J - J64) / abs(J64)
         eg = np.max(np.abs(g - g64)) / np.max(np.abs(g64))
