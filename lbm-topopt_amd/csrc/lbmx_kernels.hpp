@@ -461,8 +461,11 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
   unsigned long long rowword = 0;
   T f[L::Q];
+  T gam = T(0);
   if (active) {
+    // the row word and the design value are issued with the pulls, so their latency overlaps
     rowword = bc_row_word(a.bcrows, nx, ny, x, y, z);
+    gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
     if (a.mode == 2) {
       const double* s = static_cast<const double*>(a.src);
       static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(s[(long long)n * L::Q + decltype(qc)::value]); });
@@ -480,7 +483,6 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
     }
   }
   if (active && a.mode != 1) {
-    const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
     const T temp = objnode ? node_temperature<L>(f) : T(0);
     T rho, j[3];
     collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
