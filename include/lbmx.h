@@ -166,6 +166,28 @@ typedef struct {
   lbmx_box box_b;
 } lbmx_objective;
 
+/* ---- linear box functionals: the extension point of LBMacroFunctional ----
+   LBMacroFunctional (src/functionals/LBMacroFunctional.hh:6-14) lets an application supply any
+   functional through evaluate / adjointCollideSource. This engine takes the functionals of the
+   form J_t = timeConst * sum_k weight_k * sum_{x in box_k} m_k(x) over up to 4 terms, with the
+   per-node quantities below; the adjoint source is their exact gradient. Every shipped
+   functional is of this form, e.g. DomainTemperature2d (src/functionals/domainTemperature2d.hh:
+   weight 1/area on LBMX_TERM_TEMPERATURE over the domain box). */
+typedef enum {
+  LBMX_TERM_DENSITY = 0,              /* rho                                                   */
+  LBMX_TERM_VELOCITY_X = 1,           /* u_x (= j_x in the incompressible models)              */
+  LBMX_TERM_TOTAL_PRESSURE_FLUX = 2,  /* (rho/3 + |u|^2/2) u_x                                 */
+  LBMX_TERM_HEAT_FLUX_X = 3,          /* u_x T (thermal lattice)                               */
+  LBMX_TERM_TEMPERATURE = 4           /* T = sum of the D2Q5 populations (thermal lattice)     */
+} lbmx_term_kind;
+
+typedef struct {
+  int kind;                 /* lbmx_term_kind                                            */
+  int reserved0;
+  double weight;
+  lbmx_box box;             /* nodes summed over (inclusive, inside the grid)            */
+} lbmx_term;
+
 typedef struct lbmx_solver lbmx_solver;
 
 /* ---- lifetime ---- */
@@ -206,6 +228,10 @@ lbmx_status lbmx_set_design(lbmx_solver* s, const double* gamma, size_t n);
 lbmx_status lbmx_get_design(lbmx_solver* s, double* gamma, size_t n);
 /* The functional later used by lbmx_forward/lbmx_gradient. */
 lbmx_status lbmx_set_objective(lbmx_solver* s, const lbmx_objective* obj);
+/* The same from explicit terms (LBMacroFunctional, src/functionals/LBMacroFunctional.hh:6-14,
+   for the linear box functionals described at lbmx_term): 0 < nterms <= 4; thermal kinds need
+   the thermal lattice (LBMX_ERR_SUP otherwise). Replaces the functional set before. */
+lbmx_status lbmx_set_objective_terms(lbmx_solver* s, const lbmx_term* terms, int nterms, double time_const);
 
 /* ---- forward ---- */
 /* LBSolver::initializeAtEquilibrium (src/latticeBoltzmann/LBSolver.cc:234-249) with uniform

@@ -463,6 +463,36 @@ struct Engine {
         break;
       default: throw Error{LBMX_ERR_ARG, "unknown objective"};
     }
+    install_terms(a, tv);
+  }
+
+  // explicit terms (lbmx_set_objective_terms): the same representation
+  void set_objective_terms(const lbmx_term* t, int nterms, double time_const) {
+    if (nterms <= 0 || nterms > kMaxTerms) throw Error{LBMX_ERR_ARG, "1 to 4 functional terms"};
+    ObjArgs a{};
+    a.time_const = time_const;
+    std::vector<DevTerm> tv;
+    for (int m = 0; m < nterms; ++m) {
+      const lbmx_box& b = t[m].box;
+      if (t[m].kind < LBMX_TERM_DENSITY || t[m].kind > LBMX_TERM_TEMPERATURE)
+        throw Error{LBMX_ERR_ARG, "unknown functional term"};
+      if ((t[m].kind == LBMX_TERM_HEAT_FLUX_X || t[m].kind == LBMX_TERM_TEMPERATURE) && d.lattice != LBMX_D2Q9_D2Q5)
+        throw Error{LBMX_ERR_SUP, "temperature terms need the thermal lattice"};
+      if (!std::isfinite(t[m].weight)) throw Error{LBMX_ERR_ARG, "functional term weight not finite"};
+      if (b.x1 < b.x0 || b.y1 < b.y0 || b.z1 < b.z0) throw Error{LBMX_ERR_ARG, "empty objective box"};
+      if (b.x0 < 0 || b.y0 < 0 || b.z0 < 0 || b.x1 >= d.nx || b.y1 >= d.ny || b.z1 >= d.nz)
+        throw Error{LBMX_ERR_ARG, "objective box outside the grid"};
+      DevTerm dt{};
+      dt.box[0] = b.x0; dt.box[1] = b.x1; dt.box[2] = b.y0; dt.box[3] = b.y1; dt.box[4] = b.z0; dt.box[5] = b.z1;
+      dt.weight = t[m].weight;
+      dt.kind = t[m].kind;
+      tv.push_back(dt);
+    }
+    a.nterm = nterms;
+    install_terms(a, tv);
+  }
+
+  void install_terms(ObjArgs& a, const std::vector<DevTerm>& tv) {
     for (int m = 0; m < a.nterm; ++m) {
       for (int k = 0; k < 6; ++k) a.tbox[m][k] = tv[m].box[k];
       a.tkind[m] = tv[m].kind;
@@ -1530,6 +1560,11 @@ lbmx_status lbmx_set_objective(lbmx_solver* s, const lbmx_objective* obj) {
   CHECK_HANDLE(s);
   if (!obj) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
   return guarded([&] { s->e->set_objective(*obj); });
+}
+lbmx_status lbmx_set_objective_terms(lbmx_solver* s, const lbmx_term* terms, int nterms, double time_const) {
+  CHECK_HANDLE(s);
+  if (!terms) { lbmx::g_last_error = "null argument"; return LBMX_ERR_ARG; }
+  return guarded([&] { s->e->set_objective_terms(terms, nterms, time_const); });
 }
 
 lbmx_status lbmx_init_equilibrium(lbmx_solver* s, double rho0, const double u0[3], double T0) {

@@ -760,6 +760,7 @@ __device__ __forceinline__ T term_value(int kind, T rho, const T (&j)[3], T temp
     case 1: return inc ? j[0] : j[0] / rho;
     case 2: return (rho / T(3) + T(0.5) * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2])) * j[0];
     case 3: return j[0] * temp;
+    case 4: return temp;
   }
   return T(0);
 }
@@ -786,30 +787,9 @@ __device__ __forceinline__ void term_source(int kind, T rho, const T (&j)[3], T 
       B[0] += wgt * temp;
       AT += wgt * j[0];
       break;
+    case 4: AT += wgt; break;
   }
 }
 
-template <class L, class T>
-__device__ __forceinline__ void term_grad(int kind, const T (&f)[L::Q], T rho, const T (&j)[3], T wgt,
-                                          T (&g)[L::Q]) {
-  T temp = 0;
-  if constexpr (L::QT > 0) temp = f[L::QF] + f[L::QF + 1] + f[L::QF + 2] + f[L::QF + 3] + f[L::QF + 4];
-  const T pt = rho / T(3) + T(0.5) * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
-  static_for<0, L::QF>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr T cx = T(L::ex[q]);
-    T add = 0;
-    if (kind == 0) add = wgt;
-    else if (kind == 1) add = wgt * cx;
-    else if (kind == 2) add = wgt * (j[0] / T(3) + cdot<L, q>(j) * j[0] + pt * cx);
-    else if (kind == 3) add = wgt * cx * temp;
-    g[q] += add;
-  });
-  if constexpr (L::QT > 0) {
-    if (kind == 3) {
-      static_for<L::QF, L::Q>([&](auto qc) { g[decltype(qc)::value] += wgt * j[0]; });
-    }
-  }
-}
 
 }  // namespace lbmx

@@ -19,6 +19,7 @@
 #include <memory>
 #include <algorithm>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "lbmx.h"
@@ -267,16 +268,26 @@ inline NewBoundaryDescriptor2d ThermalZouHeNeumannDescriptor() {
 }
 
 // ---- macroscopic functionals (src/functionals/LBMacroFunctional.hh:6-14) ----
+// One of the engine's functionals (lbmx_objective_kind with its boxes), or a linear box
+// functional given by its terms (lbmx_term): the form an application-defined functional takes
+// here in place of an LBMacroFunctional subclass.
 class LBMacroFunctional {
  public:
   LBMacroFunctional(int kind, lbmx_box a, lbmx_box b = lbmx_box{0, -1, 0, -1, 0, -1}) : kind_(kind), a_(a), b_(b) {}
+  explicit LBMacroFunctional(std::vector<lbmx_term> terms) : kind_(-1), a_{}, b_{}, terms_(std::move(terms)) {}
   lbmx_objective objective(PetscScalar timeConst) const {
     lbmx_objective o{};
-    o.kind = kind_;
+    o.kind = kind_ < 0 ? LBMX_OBJ_NONE : kind_;
     o.time_const = timeConst;
     o.box_a = a_;
     o.box_b = b_;
     return o;
+  }
+  // sets this functional on a solver (lbmx_set_objective or lbmx_set_objective_terms)
+  lbmx_status install(lbmx_solver* h, PetscScalar timeConst) const {
+    if (kind_ < 0) return lbmx_set_objective_terms(h, terms_.data(), (int)terms_.size(), timeConst);
+    const lbmx_objective o = objective(timeConst);
+    return lbmx_set_objective(h, &o);
   }
   // J_t of a state from its macros (rho, ux, uy, uz, T per node, natural order), the value the
   // engine's forward sweep accumulates for the same state
@@ -295,6 +306,21 @@ class LBMacroFunctional {
       return (m[0] / 3.0 + 0.5 * (m[1] * m[1] + m[2] * m[2] + m[3] * m[3])) * m[1];
     };
     auto heat = [](const double* m) { return m[1] * m[4]; };
+    auto temperature = [](const double* m) { return m[4]; };
+    if (kind_ < 0) {
+      double J = 0.0;
+      for (const lbmx_term& t : terms_) {
+        switch (t.kind) {
+          case LBMX_TERM_DENSITY: J += t.weight * sum(t.box, rho); break;
+          case LBMX_TERM_VELOCITY_X: J += t.weight * sum(t.box, ux); break;
+          case LBMX_TERM_TOTAL_PRESSURE_FLUX: J += t.weight * sum(t.box, power); break;
+          case LBMX_TERM_HEAT_FLUX_X: J += t.weight * sum(t.box, heat); break;
+          case LBMX_TERM_TEMPERATURE: J += t.weight * sum(t.box, temperature); break;
+          default: break;
+        }
+      }
+      return J;
+    }
     switch (kind_) {
       case LBMX_OBJ_PRESSURE_DROP: return sum(a_, rho) / count(a_) - sum(b_, rho) / count(b_);
       case LBMX_OBJ_OUTFLOW: return -sum(a_, ux) / count(a_);
@@ -308,6 +334,7 @@ class LBMacroFunctional {
  private:
   int kind_;
   lbmx_box a_, b_;
+  std::vector<lbmx_term> terms_;
 };
 inline LBMacroFunctional PressureDrop2d(const Box2d& in, const Box2d& out) {  // pressureDrop2d.hh:14-52
   return LBMacroFunctional(LBMX_OBJ_PRESSURE_DROP, in.raw(), out.raw());
@@ -326,6 +353,13 @@ inline LBMacroFunctional DissipatedPower2d(const Box2d& in, const Box2d& out) {
 }
 inline LBMacroFunctional ThermalOutflowEast2d(const Box2d& out) {  // temperatureFlow2d.hh:18-54
   return LBMacroFunctional(LBMX_OBJ_HEAT_FLUX, out.raw());
+}
+// DomainTemperature2d (src/functionals/domainTemperature2d.hh): the mean temperature over the
+// domain box, as one explicit term
+inline LBMacroFunctional DomainTemperature2d(const Box2d& domain) {
+  const lbmx_box b = domain.raw();
+  const double area = (double)(b.x1 - b.x0 + 1) * (b.y1 - b.y0 + 1);
+  return LBMacroFunctional(std::vector<lbmx_term>{lbmx_term{LBMX_TERM_TEMPERATURE, 0, 1.0 / area, b}});
 }
 
 // ---- binary design files (src/core/binaryIO.cc:3-34): PETSc's binary Vec format ----
@@ -520,8 +554,7 @@ class AdjointLBSolver {
   // computeCollideSource(objective, timeConst, f, material) (adjointLBSolver.cc:83-102)
   PetscErrorCode computeCollideSource(const LBMacroFunctional& obj, PetscScalar timeConst) {
     if (!solver_) return LBMX_ERR_STATE;
-    const lbmx_objective o = obj.objective(timeConst);
-    PetscErrorCode e = lbmx_set_objective(solver_->handle(), &o);
+    PetscErrorCode e = obj.install(solver_->handle(), timeConst);
     if (!e) source_ = true;
     return e;
   }
@@ -749,16 +782,14 @@ class UnsteadyLBTopOpt {
   }
   // J only (UnsteadyLBTopOpt2d::computeObjective)
   PetscErrorCode computeObjective() {
-    const lbmx_objective o = obj_.objective(tc_);
-    LBM_CHKERRQ(lbmx_set_objective(solver_.handle(), &o));
+    LBM_CHKERRQ(obj_.install(solver_.handle(), tc_));
     LBM_CHKERRQ(solver_.initializeAtEquilibrium());
     return lbmx_forward(solver_.handle(), nt_, &J_);
   }
   // forward with checkpoints, reverse loop with recomputation, dJ/dgamma (and its filtered
   // form when a filter is attached: LBTopOpt2d's filter->filterSensitivities)
   PetscErrorCode forwardAndAdjoint() {
-    const lbmx_objective o = obj_.objective(tc_);
-    LBM_CHKERRQ(lbmx_set_objective(solver_.handle(), &o));
+    LBM_CHKERRQ(obj_.install(solver_.handle(), tc_));
     const double u0[3] = {0, 0, 0};
     LBM_CHKERRQ(solver_.makeSourceAdjoint(&adj_));
     return lbmx_gradient(solver_.handle(), nt_, 1.0, u0, 0.0, &J_, nullptr, 0);

@@ -24,6 +24,7 @@ BC_THERMAL_NEUMANN = 8
 PROFILE_UNIFORM, PROFILE_PARABOLIC, PROFILE_PARABOLIC_DISK, PROFILE_PARABOLIC_DUCT, PROFILE_TABLE = 0, 1, 2, 3, 4
 TIME_CONSTANT, TIME_SINE, TIME_TABLE = 0, 1, 2
 OBJ_NONE, OBJ_PRESSURE_DROP, OBJ_OUTFLOW, OBJ_DISSIPATED_POWER, OBJ_HEAT_FLUX, OBJ_THROUGHFLOW = 0, 1, 2, 3, 4, 5
+TERM_DENSITY, TERM_VELOCITY_X, TERM_TOTAL_PRESSURE_FLUX, TERM_HEAT_FLUX_X, TERM_TEMPERATURE = 0, 1, 2, 3, 4
 
 LBMX_OK, ERR_ARG, ERR_SUP, ERR_MEM, ERR_STATE, ERR_DEVICE = 0, 62, 56, 55, 73, 76
 CHECKPOINT_DYNAMIC, CHECKPOINT_STATIC, CHECKPOINT_BINOMIAL = 0, 1, 2
@@ -55,6 +56,10 @@ class BCDesc(C.Structure):
 class Objective(C.Structure):
     _fields_ = [("kind", C.c_int), ("reserved0", C.c_int), ("time_const", C.c_double),
                 ("box_a", Box), ("box_b", Box)]
+
+
+class Term(C.Structure):
+    _fields_ = [("kind", C.c_int), ("reserved0", C.c_int), ("weight", C.c_double), ("box", Box)]
 
 
 class FilterDesc(C.Structure):
@@ -113,6 +118,14 @@ def make_objective(kind, box_a, box_b=None, time_const=1.0):
     return o
 
 
+def make_terms(terms):
+    """(kind, weight, box) tuples as a Term array."""
+    arr = (Term * max(1, len(terms)))()
+    for i, (kind, weight, box) in enumerate(terms):
+        arr[i].kind, arr[i].weight, arr[i].box = kind, weight, box
+    return arr
+
+
 _lib = None
 
 
@@ -144,6 +157,7 @@ def library():
         "lbmx_set_design": [C.c_void_p, dp, C.c_size_t],
         "lbmx_get_design": [C.c_void_p, dp, C.c_size_t],
         "lbmx_set_objective": [C.c_void_p, P(Objective)],
+        "lbmx_set_objective_terms": [C.c_void_p, P(Term), C.c_int, C.c_double],
         "lbmx_init_equilibrium": [C.c_void_p, C.c_double, dp, C.c_double],
         "lbmx_init_equilibrium_field": [C.c_void_p, dp, C.c_size_t],
         "lbmx_forward": [C.c_void_p, C.c_int, dp],
@@ -226,7 +240,8 @@ def library():
 
 
 EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error", "lbmx_add_boundary",
-            "lbmx_set_design", "lbmx_get_design", "lbmx_set_objective", "lbmx_init_equilibrium",
+            "lbmx_set_design", "lbmx_get_design", "lbmx_set_objective", "lbmx_set_objective_terms",
+            "lbmx_init_equilibrium",
             "lbmx_init_equilibrium_field", "lbmx_forward", "lbmx_get_state", "lbmx_set_state",
             "lbmx_get_macros", "lbmx_get_timestep", "lbmx_checkpoint_config", "lbmx_gradient",
             "lbmx_get_sensitivity", "lbmx_get_adjoint", "lbmx_get_stats", "lbmx_device_bytes",
@@ -401,6 +416,11 @@ class Solver:
 
     def set_objective(self, obj):
         _check(self.lib.lbmx_set_objective(self.h, C.byref(obj)))
+
+    def set_objective_terms(self, terms, time_const=1.0):
+        """terms: [(kind, weight, box), ...] (lbmx_set_objective_terms)."""
+        arr = make_terms(terms)
+        _check(self.lib.lbmx_set_objective_terms(self.h, arr, len(terms), time_const))
 
     def init_equilibrium(self, rho0=1.0, u0=(0.0, 0.0, 0.0), T0=0.0):
         u = np.asarray(u0, dtype=np.float64)

@@ -104,7 +104,7 @@ typedef struct {
 typedef struct {
   lbmx_box box;
   double weight;
-  int kind; /* 0 rho, 1 ux, 2 total pressure flux x, 3 ux*T */
+  int kind; /* 0 rho, 1 ux, 2 total pressure flux x, 3 ux*T, 4 T */
 } orc_term;
 
 struct orc_problem {
@@ -361,6 +361,21 @@ int orc_set_objective(orc_problem* p, const lbmx_objective* o) {
       break;
     default: return LBMX_ERR_ARG;
   }
+  return 0;
+}
+
+/* Explicit terms of a linear box functional (the LBMacroFunctional interface,
+   src/functionals/LBMacroFunctional.hh:6-14), e.g. DomainTemperature2d
+   (src/functionals/domainTemperature2d.hh: timeConst / area * sum_box T). */
+int orc_set_objective_terms(orc_problem* p, const lbmx_term* t, int nterms, double time_const) {
+  if (nterms <= 0 || nterms > NTERM_MAX) return LBMX_ERR_ARG;
+  for (int m = 0; m < nterms; ++m) {
+    if (t[m].kind < 0 || t[m].kind > 4) return LBMX_ERR_ARG;
+    if ((t[m].kind == 3 || t[m].kind == 4) && p->Qt == 0) return LBMX_ERR_SUP;
+  }
+  p->time_const = time_const;
+  for (int m = 0; m < nterms; ++m) p->terms[m] = (orc_term){t[m].box, t[m].weight, t[m].kind};
+  p->nterm = nterms;
   return 0;
 }
 
@@ -1300,6 +1315,11 @@ static double term_value(const orc_problem* p, int kind, const double* f) {
       for (int q = 0; q < p->Qt; ++q) T += f[p->Qf + q];
       return j[0] * T;
     }
+    case 4: {
+      double T = 0;
+      for (int q = 0; q < p->Qt; ++q) T += f[p->Qf + q];
+      return T;
+    }
   }
   return 0;
 }
@@ -1326,6 +1346,8 @@ static void term_grad(const orc_problem* p, int kind, const double* f, double wg
   }
   if (kind == 3)
     for (int q = 0; q < p->Qt; ++q) g[p->Qf + q] += wgt * j[0];
+  if (kind == 4)
+    for (int q = 0; q < p->Qt; ++q) g[p->Qf + q] += wgt;
 }
 
 double orc_objective(const orc_problem* p, const double* f, int t) {

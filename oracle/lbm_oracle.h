@@ -31,6 +31,7 @@ int orc_num_boundaries(const orc_problem* p);
 int orc_boundary_info(const orc_problem* p, int k, int* type, int* nrm, int* box);
 int orc_set_design(orc_problem* p, const double* gamma);
 int orc_set_objective(orc_problem* p, const lbmx_objective* obj);
+int orc_set_objective_terms(orc_problem* p, const lbmx_term* terms, int nterms, double time_const);
 int orc_num_dof(const orc_problem* p);
 void orc_set_threads(int n);
 /* on != 0: the cascaded reverse collision is the reference's ReverseIncompressibleCascaded2d as

@@ -43,6 +43,7 @@ def library():
         "orc_boundary_info": ([vp, C.c_int, ip, ip, ip], C.c_int),
         "orc_set_design": ([vp, dp], C.c_int),
         "orc_set_objective": ([vp, P(lbmx.Objective)], C.c_int),
+        "orc_set_objective_terms": ([vp, P(lbmx.Term), C.c_int, C.c_double], C.c_int),
         "orc_num_dof": ([vp], C.c_int),
         "orc_set_threads": ([C.c_int], None),
         "orc_set_reference_reverse": ([vp, C.c_int], None),
@@ -153,6 +154,11 @@ class Oracle:
         code = self.lib.orc_set_objective(self.h, C.byref(obj))
         if code:
             raise lbmx.LbmxError(code, "oracle set_objective")
+
+    def set_objective_terms(self, terms, time_const=1.0):
+        code = self.lib.orc_set_objective_terms(self.h, lbmx.make_terms(terms), len(terms), time_const)
+        if code:
+            raise lbmx.LbmxError(code, "oracle set_objective_terms")
 
     # node primitives
     def collide_node(self, fin, gamma):
