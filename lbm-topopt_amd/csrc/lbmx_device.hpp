@@ -236,12 +236,16 @@ __device__ __forceinline__ unsigned int mix32(unsigned int x) {
   return x;
 }
 
-// 16 random bits for direction q from the node's two hashed words (a window of one word;
-// the codecs use its low bits)
-template <int q>
-__device__ __forceinline__ unsigned int sr_bits(unsigned int r0, unsigned int r1) {
-  constexpr unsigned int sh = (unsigned int)(((q >> 1) * 3) % 17);
-  return ((q & 1) ? r1 : r0) >> sh;
+// The random bits of direction q: bits sh .. sh + B - 1 of one of the node's two hashed words
+// (q & 1), sh = ((q >> 1) * 3) % 17, moved to the top of the seed, where the conversions of the
+// 16-bit forms read their B random bits (13 for FP16, 16 for BF16). (The top bits of the hashed
+// words themselves are those of a plain multiplication, which the xor-shift does not reach:
+// as rounding bits they raised the FP16 error of the 2048 x 1024 thermal channel threefold.)
+template <int q, int B>
+__device__ __forceinline__ unsigned int sr_seed(unsigned int r0, unsigned int r1) {
+  constexpr int sh = ((q >> 1) * 3) % 17;
+  static_assert(sh + B <= 32, "window inside the word");
+  return ((q & 1) ? r1 : r0) << (32 - B - sh);
 }
 
 struct PrecF64 {
@@ -275,20 +279,32 @@ struct PrecF16 {
     return static_cast<_Float16>(__builtin_fmaf(v, iw, -c));
   }
   static constexpr bool stochastic = true;
-  // FP16 keeps 10 of the 23 fraction bits: 13 random bits added to the FP32 pattern, then a
-  // round-toward-zero conversion (two values per instruction) drops them. Exact stochastic
+  static constexpr int kRandomBits = 13;
+  // g = f/w - c rounded stochastically: the seed's top 13 bits are added to the FP32 pattern
+  // below the FP16 fraction, and a round-toward-zero conversion drops them. Exact stochastic
   // rounding for FP16-normal values; below 2^-14 the conversion truncates to the subnormal grid.
-  template <class L, int q> __device__ static float sr_ready(T v, unsigned int rnd) {
+  // (gfx950's stochastic conversion, which rounds stochastically there as well, changed the
+  // 4000-step FP16 error of the 2048 x 1024 thermal channel from 7e-4 to 2e-3 against FP64, so
+  // the FP16 form keeps this one.)
+  template <class L, int q> __device__ static float sr_ready(T v, unsigned int seed) {
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
     constexpr float c = static_cast<float>(store_shift<L, q>());
-    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -c)) + (rnd & 0x1fffu));
+    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -c)) + (seed >> 19));
   }
-  __device__ static void pack2(float a, float b, S& ea, S& eb) {
-    // the packed pair read as one 32-bit word (element-wise access of the half vector has
-    // been observed to yield the low half twice)
-    const unsigned int u = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(a, b));
-    ea = __builtin_bit_cast(_Float16, static_cast<unsigned short>(u & 0xffffu));
-    eb = __builtin_bit_cast(_Float16, static_cast<unsigned short>(u >> 16));
+  // directions q0 (low half) and q1 (high half) of one word: one conversion for both
+  template <class L, int q0, int q1>
+  __device__ static unsigned int enc_pair(T v0, T v1, unsigned int seed0, unsigned int seed1) {
+    const float t0 = sr_ready<L, q0>(v0, seed0), t1 = sr_ready<L, q1>(v1, seed1);
+    return __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(t0, t1));
+  }
+  // into half HI of `word`, the other half kept
+  template <class L, int q, bool HI> __device__ static unsigned int enc_sr(unsigned int word, T v, unsigned int seed) {
+    const float t = sr_ready<L, q>(v, seed);
+    const unsigned int u = __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(HI ? 0.0f : t, HI ? t : 0.0f));
+    return HI ? (word & 0xffffu) | (u & 0xffff0000u) : (word & 0xffff0000u) | (u & 0xffffu);
+  }
+  template <bool HI> __device__ static S half_of(unsigned int word) {
+    return __builtin_bit_cast(_Float16, static_cast<unsigned short>(HI ? word >> 16 : word & 0xffffu));
   }
 };
 struct PrecBF16 {
@@ -305,15 +321,23 @@ struct PrecBF16 {
     return f32_to_bf16(__builtin_fmaf(v, iw, -c));
   }
   static constexpr bool stochastic = true;
-  // BF16 keeps the top 16 bits of the FP32 pattern: 16 random bits added, then truncated
-  template <class L, int q> __device__ static float sr_ready(T v, unsigned int rnd) {
+  static constexpr int kRandomBits = 16;
+  // g = f/w - c rounded stochastically into half HI of `word` (the other half is kept): the
+  // gfx950 conversion adds the seed's top 16 bits to the FP32 pattern and truncates, as
+  // rounding the BF16 way in software would (checked equal on a million samples of both signs)
+  template <class L, int q, bool HI> __device__ static unsigned int enc_sr(unsigned int word, T v, unsigned int seed) {
+    typedef short short2_t __attribute__((ext_vector_type(2)));
     constexpr float iw = static_cast<float>(1.0 / L::w[q]);
     constexpr float c = static_cast<float>(store_shift<L, q>());
-    return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -c)) + (rnd & 0xffffu));
+    const short2_t r = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(short2_t, word), __builtin_fmaf(v, iw, -c), seed, HI);
+    return __builtin_bit_cast(unsigned int, r);
   }
-  __device__ static void pack2(float a, float b, S& ea, S& eb) {
-    ea = static_cast<unsigned short>(__float_as_uint(a) >> 16);
-    eb = static_cast<unsigned short>(__float_as_uint(b) >> 16);
+  template <class L, int q0, int q1>
+  __device__ static unsigned int enc_pair(T v0, T v1, unsigned int seed0, unsigned int seed1) {
+    return enc_sr<L, q1, true>(enc_sr<L, q0, false>(0u, v0, seed0), v1, seed1);
+  }
+  template <bool HI> __device__ static S half_of(unsigned int word) {
+    return static_cast<unsigned short>(HI ? word >> 16 : word & 0xffffu);
   }
 };
 

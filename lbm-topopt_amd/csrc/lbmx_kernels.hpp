@@ -123,6 +123,33 @@ __device__ __forceinline__ void pull_node(const void* src, const Layout& l, unsi
   });
 }
 
+// Halo reflections of a boundary node: p[q] of the node is what the pull of plane opp(q) at
+// this node reads from x + c_q, for every q that leaves the domain. Opposite directions share a
+// weight and a storage shift, so the node's stored encoding of q is that value. enc(q): the
+// node's stored encoding of direction q.
+template <class L, class S, class Enc>
+__device__ __forceinline__ void store_halo(void* dst, const Layout& l, unsigned int ob, int x, int y, int z, int nx,
+                                           int ny, int nz, Enc&& enc) {
+  static_for<1, L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    if constexpr (L::ex[q] != 0 || L::ey[q] != 0 || L::ez[q] != 0) {
+      static_assert(L::w[q] == L::w[L::opp[q]] && store_shift<L, q>() == store_shift<L, L::opp[q]>(),
+                    "opposite directions must share a weight and a storage shift");
+      if (leaves<L, q>(x, y, z, nx, ny, nz)) st<S>(pull_base_w<L, L::opp[q], S>(dst, l), ob, enc(qc));
+    }
+  });
+}
+
+// The node's two hashed words for the stochastic rounding: the node offset keyed by the step
+// (uniform per launch)
+__device__ __forceinline__ void sr_words(unsigned int off, int tstep, unsigned int& r0, unsigned int& r1) {
+  const unsigned int tkey = mix32((unsigned int)tstep * 0x9e3779b9u + 0x632be5abu);
+  r0 = (off ^ tkey) * 0x9e3779b1u;
+  r0 ^= r0 >> 15;
+  r1 = r0 * 0x85ebca6bu;
+  r1 ^= r1 >> 13;
+}
+
 // store post-collision populations and, at boundary nodes, the halo reflections. tstep seeds
 // the stochastic rounding of the reduced storage forms (with the node offset).
 template <class L, class P>
@@ -130,53 +157,29 @@ __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned 
                                            bool bnd, int x, int y, int z, int nx, int ny, int nz, int tstep) {
   using S = typename P::S;
   const unsigned int ob = off * (unsigned int)sizeof(S);
-  unsigned int r0 = 0, r1 = 0;
   if constexpr (P::stochastic) {
-    const unsigned int tkey = mix32((unsigned int)tstep * 0x9e3779b9u + 0x632be5abu);  // uniform per launch
-    r0 = (off ^ tkey) * 0x9e3779b1u;
-    r0 ^= r0 >> 15;
-    r1 = r0 * 0x85ebca6bu;
-    r1 ^= r1 >> 13;
-  }
-  // encoding of direction q (stochastic rounding: two directions per conversion instruction)
-  auto enc1 = [&](auto qc) -> S {
-    constexpr int q = decltype(qc)::value;
-    if constexpr (P::stochastic) {
-      S e0, e1;
-      P::pack2(P::template sr_ready<L, q>(f[q], sr_bits<q>(r0, r1)), 0.0f, e0, e1);
-      return e0;
-    } else {
-      return P::template enc<L, q>(f[q]);
-    }
-  };
-  if constexpr (P::stochastic) {
+    // two directions per 32-bit word: direction q in the low half, q + 1 in the high half
+    unsigned int r0, r1;
+    sr_words(off, tstep, r0, r1);
+    unsigned int e[(L::Q + 1) / 2];
     static_for<0, (L::Q + 1) / 2>([&](auto pc) {
-      constexpr int q = 2 * decltype(pc)::value;
-      if constexpr (q + 1 < L::Q) {
-        S e0, e1;
-        P::pack2(P::template sr_ready<L, q>(f[q], sr_bits<q>(r0, r1)),
-                 P::template sr_ready<L, q + 1>(f[q + 1], sr_bits<q + 1>(r0, r1)), e0, e1);
-        st_nt<S>(own_base_w<q, S>(dst, l), ob, e0);
-        st_nt<S>(own_base_w<q + 1, S>(dst, l), ob, e1);
-      } else {
-        st_nt<S>(own_base_w<q, S>(dst, l), ob, enc1(IC<q>{}));
-      }
+      constexpr int k = decltype(pc)::value, q = 2 * k;
+      if constexpr (q + 1 < L::Q)
+        e[k] = P::template enc_pair<L, q, q + 1>(f[q], f[q + 1], sr_seed<q, P::kRandomBits>(r0, r1),
+                                                 sr_seed<q + 1, P::kRandomBits>(r0, r1));
+      else
+        e[k] = P::template enc_sr<L, q, false>(0u, f[q], sr_seed<q, P::kRandomBits>(r0, r1));
     });
-  } else {
-    static_for<0, L::Q>([&](auto qc) { st_nt<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc1(qc)); });
-  }
-  if (bnd) {
-    static_for<1, L::Q>([&](auto qc) {
+    auto enc = [&](auto qc) -> S {
       constexpr int q = decltype(qc)::value;
-      if constexpr (L::ex[q] != 0 || L::ey[q] != 0 || L::ez[q] != 0) {
-        // p[q] of this node is what the pull of plane opp(q) at this node reads from x + c_q;
-        // opposite directions share a weight and a shift, so the encoding (recomputed: the
-        // rounding bits are deterministic) is the stored one
-        static_assert(L::w[q] == L::w[L::opp[q]] && store_shift<L, q>() == store_shift<L, L::opp[q]>(),
-                      "opposite directions must share a weight and a storage shift");
-        if (leaves<L, q>(x, y, z, nx, ny, nz)) st<S>(pull_base_w<L, L::opp[q], S>(dst, l), ob, enc1(qc));
-      }
-    });
+      return (q & 1) ? P::template half_of<true>(e[q >> 1]) : P::template half_of<false>(e[q >> 1]);
+    };
+    static_for<0, L::Q>([&](auto qc) { st_nt<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc(qc)); });
+    if (bnd) store_halo<L, S>(dst, l, ob, x, y, z, nx, ny, nz, enc);
+  } else {
+    auto enc = [&](auto qc) -> S { return P::template enc<L, decltype(qc)::value>(f[decltype(qc)::value]); };
+    static_for<0, L::Q>([&](auto qc) { st_nt<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc(qc)); });
+    if (bnd) store_halo<L, S>(dst, l, ob, x, y, z, nx, ny, nz, enc);
   }
 }
 
@@ -417,6 +420,8 @@ __device__ __forceinline__ bool in_some_term(const ObjArgs& o, int x, int y, int
 // exact zeros. The addition is an atomic without return, so the wave does not wait for the
 // accumulator's read; each segment has one adder per sweep and sweeps run in stream order, so
 // the additions into acc[seg] happen in sweep order. Every lane of the wave calls it.
+// Summation order: x-adjacent node pairs (2i, 2i+1) first, then a tree over the 32 pair sums
+// (pair i with pair i + 16, and so on), the order in which the two-node sweep sums them.
 __device__ __forceinline__ void add_segment_sum(double* acc, long long seg, double v, bool contrib) {
   LBMX_CONTRACT_IN_STATEMENT
   const unsigned long long who = __ballot(contrib);
@@ -425,8 +430,27 @@ __device__ __forceinline__ void add_segment_sum(double* acc, long long seg, doub
     if (contrib) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  v += __shfl_xor(v, 1, 64);
+  for (int off = 32; off > 1; off >>= 1) v += __shfl_down(v, off, 64);
   if (threadIdx.x == 0) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The same for a wave of the two-node sweep: lane i holds the pair sum v of nodes 2i, 2i+1 of
+// a 128-node stretch, lanes 0-31 for the segment seg0, lanes 32-63 for seg1 (inA/inB: the
+// lane's nodes that lie in a term). Each half sums as add_segment_sum does.
+__device__ __forceinline__ void add_segment_sums_x2(double* acc, long long seg0, long long seg1, double v, bool inA,
+                                                    bool inB) {
+  LBMX_CONTRACT_IN_STATEMENT
+  const unsigned long long wa = __ballot(inA), wb = __ballot(inB);
+  const bool hi = threadIdx.x >= 32;
+  const unsigned long long half = hi ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+  const int n = __builtin_popcountll(wa & half) + __builtin_popcountll(wb & half);
+  if ((wa | wb) == 0ull) return;
+  const long long seg = hi ? seg1 : seg0;
+  // a half with one contributing node: its lane adds v (the other node of the lane adds 0)
+  if (n == 1 && (inA | inB)) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (n > 1 && (threadIdx.x & 31) == 0) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The node's functional value from its pre-collision macros (the collision returns rho and
@@ -497,6 +521,134 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
   int x, y, z, tile;
   tile_coords(a.g, x, y, z, tile);
   fwd_node<L, COLL, DES, P, kTileY>(a, a.tfac, x, y, z, tile);
+}
+
+// ---- forward sweep of the 16-bit storage forms, two nodes per lane ----
+// With 2 bytes per population, a wave's load or store of one node per lane carries 128 bytes,
+// and the work done once per wave (tile coordinates, functional and boundary tests, plane
+// bases) is spread over 64 nodes. Here lane i takes the x-adjacent nodes 2i and 2i + 1 of a
+// 128-node row stretch (two row segments). x = 0 sits at an even element offset and rows have
+// an even length, so the pair's elements of a plane form one aligned 32-bit word: the planes
+// without an x shift are read, and all planes written, as such words; the x-shifted planes
+// are read as two 16-bit halves. The word of each plane holds the first node's value in its
+// low half and the second node's in its high half, and each node's result replaces its own
+// half once that node has been decoded, so one register per plane serves both nodes. Each
+// node's arithmetic is that of fwd_node (same boundary pieces, collision, functional value and
+// rounding seeds), so the stored state is bitwise the one-node sweep's, and the functional's
+// segment sums add in the same order.
+template <class L, int COLL, int DES, class P, bool HI>
+__device__ __forceinline__ void fwd_half_node(const FwdArgs& a, unsigned int (&w)[L::Q], typename P::T gam, bool bnd,
+                                              unsigned long long pieces, bool objnode, double& objv, int x, int y,
+                                              int z, unsigned int off) {
+  using T = typename P::T;
+  T f[L::Q];
+  static_for<0, L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    f[q] = P::template dec<L, q>(P::template half_of<HI>(w[q]));
+  });
+  if (bnd) apply_bcs<L, P>(f, staged_pieces(), pieces, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
+  const T temp = objnode ? node_temperature<L>(f) : T(0);
+  T rho, j[3];
+  collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+  if (objnode) objv = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
+  unsigned int r0, r1;
+  sr_words(off, a.t, r0, r1);
+  static_for<0, L::Q>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    w[q] = P::template enc_sr<L, q, HI>(w[q], f[q], sr_seed<q, P::kRandomBits>(r0, r1));
+  });
+  if (bnd) {
+    using S = typename P::S;
+    auto enc = [&](auto qc) -> S { return P::template half_of<HI>(w[decltype(qc)::value]); };
+    store_halo<L, S>(a.dst, a.ls, off * (unsigned int)sizeof(S), x, y, z, a.g.nx, a.g.ny, a.g.nz, enc);
+  }
+}
+
+template <class L, int COLL, int DES, class P>
+__device__ __forceinline__ void fwd_node_x2(const FwdArgs& a, int xa, int y, int z) {
+  using T = typename P::T;
+  using S = typename P::S;
+  static_assert(sizeof(S) == 2 && P::stochastic, "two-node sweep of the 16-bit storage forms");
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+  const int xb = xa + 1;
+  const bool actA = xa < nx && y < ny, actB = xb < nx && y < ny;
+  // the wave's two row segments start at x0 and x1 = x0 + 64 (lanes 0-31 and 32-63)
+  const int x0 = __builtin_amdgcn_readfirstlane(xa - 2 * (int)threadIdx.x);
+  const int ys = __builtin_amdgcn_readfirstlane(y), zs = __builtin_amdgcn_readfirstlane(z);
+  const int x1 = x0 + kTileX;
+  const bool hi = threadIdx.x >= 32;
+  const bool objseg0 = a.obj.acc != nullptr && (ys < ny) & segment_meets_terms(a.obj, x0, ys, zs);
+  const bool objseg1 = a.obj.acc != nullptr && (ys < ny) & (x1 < nx) & segment_meets_terms(a.obj, x1, ys, zs);
+  const bool bseg0 = a.nbc > 0 && segment_on_boundary<L>(x0, ys, zs, nx, ny, nz);
+  const bool bseg1 = a.nbc > 0 && (x1 < nx) & segment_on_boundary<L>(x1, ys, zs, nx, ny, nz);
+  const bool objseg = hi ? objseg1 : objseg0;
+  const bool objA = objseg && (actA & in_some_term(a.obj, xa, y, z));
+  const bool objB = objseg && (actB & in_some_term(a.obj, xb, y, z));
+  const unsigned int off = actA ? layout_off(a.ls, xa, y, z) : 0u;
+  const unsigned int ob = off * (unsigned int)sizeof(S);
+  const int n = (z * ny + y) * nx + xa;
+  const bool bndA = actA && on_boundary<L>(xa, y, z, nx, ny, nz);
+  const bool bndB = actB && on_boundary<L>(xb, y, z, nx, ny, nz);
+  unsigned long long rowword = 0;
+  unsigned int w[L::Q];
+  T gA = T(0), gB = T(0);
+  if (actA) {
+    rowword = bc_row_word(a.bcrows, nx, ny, xa, y, z);
+    const typename P::G* gp = static_cast<const typename P::G*>(a.gamma) + n;
+    gA = T(gp[0]);
+    if (actB) gB = T(gp[1]);
+    static_for<0, L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const char* base = pull_base<L, q, S>(a.src, a.ls) + ob;
+      if constexpr (L::ex[q] == 0) {
+        w[q] = __builtin_nontemporal_load(reinterpret_cast<const unsigned int*>(base));
+      } else {
+        const unsigned int lo = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(base));
+        const unsigned int up = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(base) + 1);
+        w[q] = lo | (up << 16);
+      }
+    });
+  }
+  // the pieces of both segments (a piece that meets only the other segment holds neither node)
+  unsigned long long pieces = 0;
+  if (bseg0 | bseg1) {
+    const unsigned int l0 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword), 0);
+    const unsigned int h0 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword >> 32), 0);
+    const unsigned int l1 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword), 32);
+    const unsigned int h1 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword >> 32), 32);
+    const unsigned long long p0 = bseg0 ? ((static_cast<unsigned long long>(h0) << 32) | l0) : 0ull;
+    const unsigned long long p1 = bseg1 ? ((static_cast<unsigned long long>(h1) << 32) | l1) : 0ull;
+    pieces = p0 | p1;
+    stage_wave_pieces(a.bcs, a.tfac, a.nbc, pieces);
+  }
+  double vA = 0.0, vB = 0.0;
+  if (actA) fwd_half_node<L, COLL, DES, P, false>(a, w, gA, bndA, pieces, objA, vA, xa, y, z, off);
+  if (actB) fwd_half_node<L, COLL, DES, P, true>(a, w, gB, bndB, pieces, objB, vB, xb, y, z, off + 1u);
+  if (actA) {
+    if (actB) {
+      static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        st_nt<unsigned int>(own_base_w<q, S>(a.dst, a.ls), ob, w[q]);
+      });
+    } else {
+      static_for<0, L::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        st_nt<S>(own_base_w<q, S>(a.dst, a.ls), ob, P::template half_of<false>(w[q]));
+      });
+    }
+  }
+  if (objseg0 | objseg1)
+    add_segment_sums_x2(a.obj.acc, bc_row_index(nx, ny, x0, ys, zs), bc_row_index(nx, ny, x1, ys, zs), vA + vB, objA,
+                        objB);
+}
+
+template <class L, int COLL, int DES, class P>
+__global__ __launch_bounds__(kTileX* kTileY) void k_fwd_x2(FwdArgs a) {
+  int x, y, z, tile;
+  tile_coords(a.g, x, y, z, tile);
+  // tile_coords places lane i at x = 64 bx + i; this sweep's tiles are 128 nodes wide
+  const int bx = (x - (int)threadIdx.x) / kTileX;
+  fwd_node_x2<L, COLL, DES, P>(a, 2 * kTileX * bx + 2 * (int)threadIdx.x, y, z);
 }
 
 // Initial state: s_0 = f_eq(rho, u[, T]) from uniform or per-node macros; writes p_0 = C(s_0)
@@ -813,6 +965,16 @@ template <class L, int COLL, int DES, class P>
 hipError_t launch_fwd_t(const FwdArgs& a, hipStream_t s) {
   size_t lds = pieces_lds_bytes(a.nbc);
   if constexpr (sizeof(typename P::T) == 8) lds = lds > kFwdF64Lds ? lds : kFwdF64Lds;
+  if constexpr (sizeof(typename P::S) == 2) {
+    if (a.mode == 0) {
+      // tiles of 128 x kTileY nodes
+      FwdArgs b = a;
+      b.g.nblk_x = (a.g.nx + 2 * kTileX - 1) / (2 * kTileX);
+      b.g.nblocks = b.g.nblk_x * b.g.nblk_y * a.g.nz;
+      hipLaunchKernelGGL((k_fwd_x2<L, COLL, DES, P>), dim3(b.g.nblocks), dim3(kTileX, kTileY, 1), lds, s, b);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((k_fwd<L, COLL, DES, P>), dim3(a.g.nblocks), dim3(kTileX, kTileY, 1), lds, s, a);
   return hipGetLastError();
 }
