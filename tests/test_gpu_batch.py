@@ -264,16 +264,17 @@ def test_batch_members_keep_their_sensitivities(lbmx_mod):
 def test_batch_shifted_storage_forms(lbmx_mod, prec):
     """The shifted-storage forms in a batch: the stochastic rounding is keyed by each member's
     own node offsets and steps, so every member's dJ/dgamma is bitwise its single-instance one.
-    A single instance sweeps forward with two x-adjacent nodes per thread, the batch with one:
-    the odd grid widths leave the last thread of a row with one node."""
+    On grids of at least 1024 one-node workgroups a single instance sweeps forward with two
+    x-adjacent nodes per thread, the batch always with one: the wide members take that path, and
+    their odd widths leave the last thread of a row with one node."""
     members = build(lbmx.CASCADED, lbmx.DESIGN_PBB, prec)
     members3 = [member3d(30, 14, 12, lbmx.MRT, lbmx.DESIGN_BRINKMAN, prec, 4000 + i, eta)
                 for i, eta in enumerate((0.4, 0.6))]
-    odd = [member(65, 20, lbmx.CASCADED, lbmx.DESIGN_PBB, prec, 4100, 0.5, True),
-           member(127, 24, lbmx.CASCADED, lbmx.DESIGN_PBB, prec, 4101, 0.4, False, lbmx.OBJ_OUTFLOW)]
-    odd3 = [member3d(33, 15, 12, lbmx.MRT, lbmx.DESIGN_BRINKMAN, prec, 4200, 0.5),
-            member3d(67, 9, 7, lbmx.MRT, lbmx.DESIGN_BRINKMAN, prec, 4201, 0.6)]
-    for group in (members, members3, odd, odd3):
+    # 17 x 64 and 3 x 11 x 32 one-node workgroups
+    wide = [member(1025, 256, lbmx.CASCADED, lbmx.DESIGN_PBB, prec, 4100, 0.5, True),
+            member(1023, 260, lbmx.CASCADED, lbmx.DESIGN_PBB, prec, 4101, 0.4, False, lbmx.OBJ_OUTFLOW)]
+    wide3 = [member3d(129, 44, 32, lbmx.MRT, lbmx.DESIGN_BRINKMAN, prec, 4200, 0.5)]
+    for group in (members, members3, wide, wide3):
         ref = singles(group, 6)
         b = lbmx.Batch(group)
         b.checkpoint_config(6)
