@@ -406,11 +406,13 @@ __device__ __forceinline__ bool segment_meets_terms(const ObjArgs& o, int x0, in
   });
   return meets;
 }
+// (per node, on the vector unit: each term is tested under the uniform condition m < nterm,
+// so a functional of one or two terms does not pay for the unused slots)
 __device__ __forceinline__ bool in_some_term(const ObjArgs& o, int x, int y, int z) {
   bool in = false;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    in = in | ((m < o.nterm) & in_box(o.tbox[m], x, y, z));
+    if (m < o.nterm) in = in | in_box(o.tbox[m], x, y, z);
   });
   return in;
 }
@@ -461,7 +463,9 @@ __device__ __forceinline__ T node_objective(const ObjArgs& o, T rho, const T (&j
   T acc = 0;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    if ((m < o.nterm) & in_box(o.tbox[m], x, y, z)) acc += T(o.tweight[m]) * term_value<COLL, T>(o.tkind[m], rho, j, temp);
+    if (m < o.nterm) {
+      if (in_box(o.tbox[m], x, y, z)) acc += T(o.tweight[m]) * term_value<COLL, T>(o.tkind[m], rho, j, temp);
+    }
   });
   return acc * T(o.time_const);
 }
