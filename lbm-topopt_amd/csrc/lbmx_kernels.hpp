@@ -444,15 +444,18 @@ __device__ __forceinline__ void add_segment_sums_x2(double* acc, long long seg0,
                                                     bool inB) {
   LBMX_CONTRACT_IN_STATEMENT
   const unsigned long long wa = __ballot(inA), wb = __ballot(inB);
-  const bool hi = threadIdx.x >= 32;
-  const unsigned long long half = hi ? 0xffffffff00000000ull : 0x00000000ffffffffull;
-  const int n = __builtin_popcountll(wa & half) + __builtin_popcountll(wb & half);
   if ((wa | wb) == 0ull) return;
+  const int n0 = __builtin_popcountll(wa & 0xffffffffull) + __builtin_popcountll(wb & 0xffffffffull);
+  const int n1 = __builtin_popcountll(wa >> 32) + __builtin_popcountll(wb >> 32);
+  const bool hi = threadIdx.x >= 32;
+  const int n = hi ? n1 : n0;
   const long long seg = hi ? seg1 : seg0;
   // a half with one contributing node: its lane adds v (the other node of the lane adds 0)
   if (n == 1 && (inA | inB)) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int off = 16; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if (n > 1 && (threadIdx.x & 31) == 0) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if ((n0 > 1) | (n1 > 1)) {  // (uniform)
+    for (int off = 16; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (n > 1 && (threadIdx.x & 31) == 0) __hip_atomic_fetch_add(acc + seg, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // The node's functional value from its pre-collision macros (the collision returns rho and
