@@ -701,6 +701,13 @@ def main():
     out["roofline"]["plane_copy_gbs"] = round(pc, 1)
     out["roofline"]["frac_of_plane_copy"] = round(head["fwd_gbs"] / pc, 4)
     log(f"19-plane copy {pc:.1f} GB/s; forward sweep at {head['fwd_gbs'] / pc:.3f} of it")
+    if "fp16" in out.get("forms", {}):
+        # the 16-bit forward sweep's pattern: 19 planes of 2-byte values, two nodes per thread
+        pc16 = lbmx.plane_copy16_gbs(19, nx * ny * nz)
+        f16 = out["forms"]["fp16"]
+        f16["plane_copy16_gbs"] = round(pc16, 1)
+        f16["fwd_frac_of_plane_copy16"] = round(f16["fwd_gbs"] / pc16, 4)
+        log(f"19-plane 16-bit copy {pc16:.1f} GB/s; FP16 forward sweep at {f16['fwd_gbs'] / pc16:.3f} of it")
     if not args.no_other_workloads:
         out["other_workloads"] = run_other_workloads(lbmx, workloads, mode)
     if not args.no_cpu_baseline:

@@ -333,6 +333,9 @@ lbmx_status lbmx_stream_copy(size_t bytes, int iters, double* gbs);
    19 FP64 planes of `nodes` values read and written at the same offset, one node per thread.
    No counterpart in the reference. */
 lbmx_status lbmx_stream_copy_planes(int planes, size_t nodes, int iters, double* gbs);
+/* Diagnostic: the same for 16-bit planes (the 16-bit storage forms), two nodes per thread, each
+   plane's pair read and written as one 32-bit word. No counterpart in the reference. */
+lbmx_status lbmx_stream_copy_planes16(int planes, size_t nodes, int iters, double* gbs);
 /* Number of visible HIP devices (0 when the runtime finds none); no handle needed. */
 lbmx_status lbmx_device_count(int* count);
 /* Bytes of device memory held by the handle (state, adjoint, checkpoints). */

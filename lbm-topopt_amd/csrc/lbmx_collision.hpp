@@ -54,13 +54,20 @@ __device__ __forceinline__ void to_pairs(const T (&f)[L::Q], Pairs<L, T>& p) {
   });
 }
 
-template <class L, class T>
+// TWICE: hand back 2 f_q for q = 1 .. QF - 1 (the caller folds the exact factor 1/2 into its
+// next multiplication)
+template <class L, class T, bool TWICE = false>
 __device__ __forceinline__ void from_pairs(const Pairs<L, T>& p, T (&f)[L::Q]) {
   f[0] = p.f0;
   static_for<1, L::HALF + 1>([&](auto dc) {
     constexpr int d = decltype(dc)::value;
-    f[d] = T(0.5) * (p.s[d] + p.a[d]);
-    f[d + L::HALF] = T(0.5) * (p.s[d] - p.a[d]);
+    if constexpr (TWICE) {
+      f[d] = p.s[d] + p.a[d];
+      f[d + L::HALF] = p.s[d] - p.a[d];
+    } else {
+      f[d] = T(0.5) * (p.s[d] + p.a[d]);
+      f[d + L::HALF] = T(0.5) * (p.s[d] - p.a[d]);
+    }
   });
 }
 
@@ -692,8 +699,9 @@ __device__ __forceinline__ void darcy_force(T (&f)[9], T rho, const T (&j)[3], T
 }
 
 // Moment-space operators (BGK, MRT): node collision in place; rho, j are the pre-collision macros.
-template <class L, int COLL, int DES, class T>
+template <class L, int COLL, int DES, class T, bool TWICE = false>
 __device__ __forceinline__ void collide_moment(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
+  static_assert(!(TWICE && DES == D_DARCY), "the Darcy force acts on the populations themselves");
   Pairs<L, T> p;
   to_pairs<L>(f, p);
   // (FP32 arithmetic only: the FP64 sweeps are bound by HBM, and the grouped form holds more
@@ -714,7 +722,7 @@ __device__ __forceinline__ void collide_moment(T (&f)[L::Q], T gamma, const Mode
     const T sc = T(1) - T(2) * imult;
     static_for<1, L::HALF + 1>([&](auto dc) { p.a[decltype(dc)::value] *= sc; });
   }
-  from_pairs<L>(p, f);
+  from_pairs<L, T, TWICE>(p, f);
   if constexpr (DES == D_DARCY) {
     static_assert(L::Q == 9 && CollTraits<COLL>::mrt && CollTraits<COLL>::inc, "DarcyMRT2d is D2Q9 MRT");
     darcy_force<T>(f, rho, j, gamma, mp);
@@ -949,11 +957,18 @@ __device__ __forceinline__ void reverse_moment(const T (&f)[L::Q], T gamma, cons
 #undef LBMX_BAS
 
 // ---- entry points -----------------------------------------------------------------------
-// Node collision in place; rho, j receive the pre-collision macros.
-template <class L, int COLL, int DES, class T>
+// The moment-space operators end by halving pair sums, which a caller that multiplies the
+// result by constants anyway (the 16-bit storage codecs) can fold into its own factor.
+template <int COLL, int DES>
+inline constexpr bool kCollideTwice = COLL != C_CASC && DES != D_DARCY;
+
+// Node collision in place; rho, j receive the pre-collision macros. TWICE (only where
+// kCollideTwice): f_q comes back doubled for q = 1 .. QF - 1.
+template <class L, int COLL, int DES, class T, bool TWICE = false>
 __device__ __forceinline__ void collide(T (&f)[L::Q], T gamma, const ModelParams& mp, T& rho, T (&j)[3]) {
+  static_assert(!TWICE || kCollideTwice<COLL, DES>, "operator without a final halving");
   if constexpr (COLL == C_CASC) collide_cascaded<L, DES>(f, gamma, mp, rho, j);
-  else collide_moment<L, COLL, DES>(f, gamma, mp, rho, j);
+  else collide_moment<L, COLL, DES, T, TWICE>(f, gamma, mp, rho, j);
 }
 
 // lam <- (dC/df)^T lam + functional source; dgam += (dC/dgamma)^T lam.

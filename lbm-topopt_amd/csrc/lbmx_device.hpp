@@ -213,6 +213,11 @@ template <class L, int q>
 __host__ __device__ constexpr double store_shift() {
   return q < L::QF ? 1.0 : 0.5;
 }
+// factor of the value handed to an encoder: 1/2 where the collision returns 2 f_q (TW)
+template <class L, int q, bool TW>
+__host__ __device__ constexpr double enc_scale() {
+  return TW && q >= 1 && q < L::QF ? 0.5 : 1.0;
+}
 __device__ __forceinline__ float bf16_to_f32(unsigned short h) {
   return __uint_as_float(static_cast<unsigned int>(h) << 16);
 }
@@ -286,23 +291,26 @@ struct PrecF16 {
   // (gfx950's stochastic conversion, which rounds stochastically there as well, changed the
   // 4000-step FP16 error of the 2048 x 1024 thermal channel from 7e-4 to 2e-3 against FP64, so
   // the FP16 form keeps this one.)
-  template <class L, int q> __device__ static float sr_ready(T v, unsigned int seed) {
-    constexpr float iw = static_cast<float>(1.0 / L::w[q]);
+  // (TW: v is 2 f_q for q = 1 .. QF - 1; the factor 1/2 joins 1/w exactly)
+  template <class L, int q, bool TW = false> __device__ static float sr_ready(T v, unsigned int seed) {
+    constexpr float iw = static_cast<float>(enc_scale<L, q, TW>() / L::w[q]);
     constexpr float c = static_cast<float>(store_shift<L, q>());
     return __uint_as_float(__float_as_uint(__builtin_fmaf(v, iw, -c)) + (seed >> 19));
   }
   // directions q0 (low half) and q1 (high half) of one word: one conversion for both
-  template <class L, int q0, int q1>
+  template <class L, int q0, int q1, bool TW = false>
   __device__ static unsigned int enc_pair(T v0, T v1, unsigned int seed0, unsigned int seed1) {
-    const float t0 = sr_ready<L, q0>(v0, seed0), t1 = sr_ready<L, q1>(v1, seed1);
+    const float t0 = sr_ready<L, q0, TW>(v0, seed0), t1 = sr_ready<L, q1, TW>(v1, seed1);
     return __builtin_bit_cast(unsigned int, __builtin_amdgcn_cvt_pkrtz(t0, t1));
   }
   // into half HI of `word`, the other half kept: gfx950's stochastic conversion with a zero
   // seed is the round-toward-zero conversion (equal on 4M samples over all FP32 exponents from
   // 2^-40 to 2^15, zeros included) and writes one half
-  template <class L, int q, bool HI> __device__ static unsigned int enc_sr(unsigned int word, T v, unsigned int seed) {
+  template <class L, int q, bool HI, bool TW = false>
+  __device__ static unsigned int enc_sr(unsigned int word, T v, unsigned int seed) {
     typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-    const half2_t r = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(half2_t, word), sr_ready<L, q>(v, seed), 0u, HI);
+    const half2_t r =
+        __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(half2_t, word), sr_ready<L, q, TW>(v, seed), 0u, HI);
     return __builtin_bit_cast(unsigned int, r);
   }
   template <bool HI> __device__ static S half_of(unsigned int word) {
@@ -327,16 +335,17 @@ struct PrecBF16 {
   // g = f/w - c rounded stochastically into half HI of `word` (the other half is kept): the
   // gfx950 conversion adds the seed's top 16 bits to the FP32 pattern and truncates, as
   // rounding the BF16 way in software would (checked equal on a million samples of both signs)
-  template <class L, int q, bool HI> __device__ static unsigned int enc_sr(unsigned int word, T v, unsigned int seed) {
+  template <class L, int q, bool HI, bool TW = false>
+  __device__ static unsigned int enc_sr(unsigned int word, T v, unsigned int seed) {
     typedef short short2_t __attribute__((ext_vector_type(2)));
-    constexpr float iw = static_cast<float>(1.0 / L::w[q]);
+    constexpr float iw = static_cast<float>(enc_scale<L, q, TW>() / L::w[q]);
     constexpr float c = static_cast<float>(store_shift<L, q>());
     const short2_t r = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(short2_t, word), __builtin_fmaf(v, iw, -c), seed, HI);
     return __builtin_bit_cast(unsigned int, r);
   }
-  template <class L, int q0, int q1>
+  template <class L, int q0, int q1, bool TW = false>
   __device__ static unsigned int enc_pair(T v0, T v1, unsigned int seed0, unsigned int seed1) {
-    return enc_sr<L, q1, true>(enc_sr<L, q0, false>(0u, v0, seed0), v1, seed1);
+    return enc_sr<L, q1, true, TW>(enc_sr<L, q0, false, TW>(0u, v0, seed0), v1, seed1);
   }
   template <bool HI> __device__ static S half_of(unsigned int word) {
     return static_cast<unsigned short>(HI ? word >> 16 : word & 0xffffu);

@@ -140,6 +140,11 @@ __device__ __forceinline__ void store_halo(void* dst, const Layout& l, unsigned 
   });
 }
 
+// The 16-bit forms take the moment-space collisions' doubled populations (collide<..., true>):
+// their encoders fold the factor 1/2 into 1/w exactly, so the stored bits are unchanged.
+template <class P, int COLL, int DES>
+inline constexpr bool kStoreTwice = P::shifted && kCollideTwice<COLL, DES>;
+
 // The node's two hashed words for the stochastic rounding: the node offset keyed by the step
 // (uniform per launch)
 __device__ __forceinline__ void sr_words(unsigned int off, int tstep, unsigned int& r0, unsigned int& r1) {
@@ -151,8 +156,9 @@ __device__ __forceinline__ void sr_words(unsigned int off, int tstep, unsigned i
 }
 
 // store post-collision populations and, at boundary nodes, the halo reflections. tstep seeds
-// the stochastic rounding of the reduced storage forms (with the node offset).
-template <class L, class P>
+// the stochastic rounding of the reduced storage forms (with the node offset). TW: f holds the
+// collision's doubled populations (collide<..., true>), which only the 16-bit codecs take.
+template <class L, class P, bool TW = false>
 __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned int off, const typename P::T (&f)[L::Q],
                                            bool bnd, int x, int y, int z, int nx, int ny, int nz, int tstep) {
   using S = typename P::S;
@@ -165,10 +171,10 @@ __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned 
     static_for<0, (L::Q + 1) / 2>([&](auto pc) {
       constexpr int k = decltype(pc)::value, q = 2 * k;
       if constexpr (q + 1 < L::Q)
-        e[k] = P::template enc_pair<L, q, q + 1>(f[q], f[q + 1], sr_seed<q, P::kRandomBits>(r0, r1),
-                                                 sr_seed<q + 1, P::kRandomBits>(r0, r1));
+        e[k] = P::template enc_pair<L, q, q + 1, TW>(f[q], f[q + 1], sr_seed<q, P::kRandomBits>(r0, r1),
+                                                     sr_seed<q + 1, P::kRandomBits>(r0, r1));
       else
-        e[k] = P::template enc_sr<L, q, false>(0u, f[q], sr_seed<q, P::kRandomBits>(r0, r1));
+        e[k] = P::template enc_sr<L, q, false, TW>(0u, f[q], sr_seed<q, P::kRandomBits>(r0, r1));
     });
     auto enc = [&](auto qc) -> S {
       constexpr int q = decltype(qc)::value;
@@ -177,6 +183,7 @@ __device__ __forceinline__ void store_node(void* dst, const Layout& l, unsigned 
     static_for<0, L::Q>([&](auto qc) { st_nt<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc(qc)); });
     if (bnd) store_halo<L, S>(dst, l, ob, x, y, z, nx, ny, nz, enc);
   } else {
+    static_assert(!TW, "doubled populations go to the 16-bit codecs only");
     auto enc = [&](auto qc) -> S { return P::template enc<L, decltype(qc)::value>(f[decltype(qc)::value]); };
     static_for<0, L::Q>([&](auto qc) { st_nt<S>(own_base_w<decltype(qc)::value, S>(dst, l), ob, enc(qc)); });
     if (bnd) store_halo<L, S>(dst, l, ob, x, y, z, nx, ny, nz, enc);
@@ -516,9 +523,10 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   if (active && a.mode != 1) {
     const T temp = objnode ? node_temperature<L>(f) : T(0);
     T rho, j[3];
-    collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+    constexpr bool tw = kStoreTwice<P, COLL, DES>;
+    collide<L, COLL, DES, T, tw>(f, gam, a.m, rho, j);
     if (objnode) objv = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
-    store_node<L, P>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
+    store_node<L, P, tw>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
   }
   if (objseg) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), objv, objnode);
 }
@@ -556,13 +564,14 @@ __device__ __forceinline__ void fwd_half_node(const FwdArgs& a, unsigned int (&w
   if (bnd) apply_bcs<L, P>(f, staged_pieces(), pieces, staged_tfac(a.nbc), x, y, z, a.src, a.ls);
   const T temp = objnode ? node_temperature<L>(f) : T(0);
   T rho, j[3];
-  collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
+  constexpr bool tw = kStoreTwice<P, COLL, DES>;
+  collide<L, COLL, DES, T, tw>(f, gam, a.m, rho, j);
   if (objnode) objv = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
   unsigned int r0, r1;
   sr_words(off, a.t, r0, r1);
   static_for<0, L::Q>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    w[q] = P::template enc_sr<L, q, HI>(w[q], f[q], sr_seed<q, P::kRandomBits>(r0, r1));
+    w[q] = P::template enc_sr<L, q, HI, tw>(w[q], f[q], sr_seed<q, P::kRandomBits>(r0, r1));
   });
   if (bnd) {
     using S = typename P::S;
@@ -705,8 +714,9 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_init(FwdArgs a, const double
     static_for<0, L::Q>([&](auto qc) { f[decltype(qc)::value] = T(fd[decltype(qc)::value]); });
     const T gam = T(static_cast<const typename P::G*>(a.gamma)[n]);
     T rho, j[3];
-    collide<L, COLL, DES, T>(f, gam, a.m, rho, j);
-    store_node<L, P>(a.dst, a.ls, layout_off(a.ls, x, y, z), f, on_boundary<L>(x, y, z, nx, ny, nz), x, y, z, nx,
+    constexpr bool tw = kStoreTwice<P, COLL, DES>;
+    collide<L, COLL, DES, T, tw>(f, gam, a.m, rho, j);
+    store_node<L, P, tw>(a.dst, a.ls, layout_off(a.ls, x, y, z), f, on_boundary<L>(x, y, z, nx, ny, nz), x, y, z, nx,
                      ny, nz, 0);
   }
 }
@@ -925,15 +935,15 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   T rho, j[3];
   if (act1) {
     const T temp = on1 ? node_temperature<L>(f1) : T(0);
-    collide<L, COLL, DES, T>(f1, g1, a.m, rho, j);
+    collide<L, COLL, DES, T, kStoreTwice<P, COLL, DES>>(f1, g1, a.m, rho, j);
     if (on1) obj1 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
-    store_node<L, P>(a.dst, a.ls, off1, f1, bnd1, x, y, z, nx, ny, a.g.nz, a.t);
+    store_node<L, P, kStoreTwice<P, COLL, DES>>(a.dst, a.ls, off1, f1, bnd1, x, y, z, nx, ny, a.g.nz, a.t);
   }
   if (act2) {
     const T temp = on2 ? node_temperature<L>(f2) : T(0);
-    collide<L, COLL, DES, T>(f2, g2, a.m, rho, j);
+    collide<L, COLL, DES, T, kStoreTwice<P, COLL, DES>>(f2, g2, a.m, rho, j);
     if (on2) obj2 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y2, z));
-    store_node<L, P>(a.dst, a.ls, off2, f2, bnd2, x, y2, z, nx, ny, a.g.nz, a.t);
+    store_node<L, P, kStoreTwice<P, COLL, DES>>(a.dst, a.ls, off2, f2, bnd2, x, y2, z, nx, ny, a.g.nz, a.t);
   }
   if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1, on1);
   if (objseg2) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y2, z), obj2, on2);

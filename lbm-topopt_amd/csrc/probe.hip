@@ -32,7 +32,73 @@ __global__ __launch_bounds__(256) void k_plane_copy(const double* __restrict__ i
   for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(v[q], out + q * plane + i);
 }
 
+// The same for 16-bit planes as the 16-bit forms' two-node forward sweep moves them: two nodes
+// per thread, each plane's pair read and written as one 32-bit word.
+template <int Q>
+__global__ __launch_bounds__(256) void k_plane_copy16(const unsigned int* __restrict__ in, unsigned int* __restrict__ out,
+                                                      long long plane_words, long long nwords) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwords) return;
+  unsigned int v[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) v[q] = __builtin_nontemporal_load(in + q * plane_words + i);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(v[q], out + q * plane_words + i);
+}
+
 }  // namespace lbmx
+
+extern "C" lbmx_status lbmx_stream_copy_planes16(int planes, size_t nodes, int iters, double* gbs) {
+  if (!gbs || iters < 1 || nodes < 512 || (planes != 9 && planes != 14 && planes != 19)) {
+    lbmx::g_last_error = "stream copy planes16: needs gbs, iters >= 1, nodes >= 512 and planes 9, 14 or 19";
+    return LBMX_ERR_ARG;
+  }
+  const long long nw = (long long)nodes / 2;           // 32-bit words (two nodes each)
+  const long long plane = (nw + 63) / 64 * 64;         // 256-byte aligned planes
+  const size_t bytes = (size_t)planes * plane * 4;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) {
+    lbmx::g_last_error = "device allocation failed";
+    return LBMX_ERR_MEM;
+  }
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    lbmx::g_last_error = "device allocation failed";
+    return LBMX_ERR_MEM;
+  }
+  (void)hipMemset(a, 0, bytes);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const unsigned blocks = (unsigned)((nw + 255) / 256);
+  auto launch = [&]() {
+    const unsigned int* in = static_cast<const unsigned int*>(a);
+    unsigned int* out = static_cast<unsigned int*>(b);
+    if (planes == 19)
+      hipLaunchKernelGGL(lbmx::k_plane_copy16<19>, dim3(blocks), dim3(256), 0, nullptr, in, out, plane, nw);
+    else if (planes == 14)
+      hipLaunchKernelGGL(lbmx::k_plane_copy16<14>, dim3(blocks), dim3(256), 0, nullptr, in, out, plane, nw);
+    else
+      hipLaunchKernelGGL(lbmx::k_plane_copy16<9>, dim3(blocks), dim3(256), 0, nullptr, in, out, plane, nw);
+  };
+  launch();
+  (void)hipEventRecord(e0, nullptr);
+  for (int k = 0; k < iters; ++k) launch();
+  (void)hipEventRecord(e1, nullptr);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) {
+    lbmx::g_last_error = hipGetErrorString(err);
+    return LBMX_ERR_DEVICE;
+  }
+  *gbs = 2.0 * planes * nw * 4.0 * iters / (ms * 1e-3) / 1e9;
+  return LBMX_OK;
+}
 
 extern "C" lbmx_status lbmx_stream_copy_planes(int planes, size_t nodes, int iters, double* gbs) {
   if (!gbs || iters < 1 || nodes < 256 || (planes != 9 && planes != 14 && planes != 19)) {

@@ -179,6 +179,7 @@ def library():
         "lbmx_time_sweeps": [C.c_void_p, C.c_int, C.c_int, dp],
         "lbmx_stream_copy": [C.c_size_t, C.c_int, dp],
         "lbmx_stream_copy_planes": [C.c_int, C.c_size_t, C.c_int, dp],
+        "lbmx_stream_copy_planes16": [C.c_int, C.c_size_t, C.c_int, dp],
         "lbmx_device_count": [P(C.c_int)],
         "lbmx_batch_create": [P(C.c_void_p), C.c_int, P(C.c_void_p)],
         "lbmx_batch_destroy": [C.c_void_p],
@@ -245,7 +246,7 @@ EXPORTED = ["lbmx_create", "lbmx_destroy", "lbmx_set_stream", "lbmx_last_error",
             "lbmx_init_equilibrium_field", "lbmx_forward", "lbmx_get_state", "lbmx_set_state",
             "lbmx_get_macros", "lbmx_get_timestep", "lbmx_checkpoint_config", "lbmx_gradient",
             "lbmx_get_sensitivity", "lbmx_get_adjoint", "lbmx_get_stats", "lbmx_device_bytes",
-            "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps", "lbmx_stream_copy", "lbmx_stream_copy_planes",
+            "lbmx_describe_boundaries", "lbmx_checkpoint_simulate", "lbmx_time_sweeps", "lbmx_stream_copy", "lbmx_stream_copy_planes", "lbmx_stream_copy_planes16",
             "lbmx_device_count", "lbmx_batch_create", "lbmx_batch_destroy", "lbmx_batch_checkpoint_config",
             "lbmx_batch_set_tile_rows", "lbmx_batch_set_graph", "lbmx_batch_autotune", "lbmx_batch_gradient", "lbmx_batch_get_stats",
             "lbmx_batch_device_bytes", "lbmx_filter_create", "lbmx_filter_destroy", "lbmx_filter_set_projection",
@@ -326,6 +327,13 @@ def plane_copy_gbs(planes, nodes, iters=20):
     """Measured copy bandwidth (GB/s) with a sweep's access pattern: planes FP64 planes in and out."""
     g = C.c_double(0.0)
     _check(library().lbmx_stream_copy_planes(planes, nodes, iters, C.byref(g)))
+    return g.value
+
+
+def plane_copy16_gbs(planes, nodes, iters=20):
+    """The same for 16-bit planes, two nodes per thread (the 16-bit forms' forward sweep)."""
+    g = C.c_double(0.0)
+    _check(library().lbmx_stream_copy_planes16(planes, nodes, iters, C.byref(g)))
     return g.value
 
 
