@@ -580,11 +580,48 @@ __device__ __forceinline__ void fwd_half_node(const FwdArgs& a, unsigned int (&w
   }
 }
 
-template <class L, int COLL, int DES, class P>
-__device__ __forceinline__ void fwd_node_x2(const FwdArgs& a, int xa, int y, int z) {
+// What a two-node lane loads before it computes: each plane's word for its pair, the pair's
+// design values and its segment's boundary row word.
+template <class L, class P>
+struct X2Loads {
+  unsigned int w[L::Q];
+  typename P::T gA, gB;
+  unsigned long long rowword;
+};
+
+template <class L, class P>
+__device__ __forceinline__ void x2_load(const FwdArgs& a, int xa, int y, int z, X2Loads<L, P>& ld) {
   using T = typename P::T;
   using S = typename P::S;
-  static_assert(sizeof(S) == 2 && P::stochastic, "two-node sweep of the 16-bit storage forms");
+  const int nx = a.g.nx, ny = a.g.ny;
+  const bool actA = xa < nx && y < ny, actB = xa + 1 < nx && y < ny;
+  ld.rowword = 0;
+  ld.gA = ld.gB = T(0);
+  if (actA) {
+    const unsigned int ob = layout_off(a.ls, xa, y, z) * (unsigned int)sizeof(S);
+    ld.rowword = bc_row_word(a.bcrows, nx, ny, xa, y, z);
+    const typename P::G* gp = static_cast<const typename P::G*>(a.gamma) + ((z * ny + y) * nx + xa);
+    ld.gA = T(gp[0]);
+    if (actB) ld.gB = T(gp[1]);
+    static_for<0, L::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const char* base = pull_base<L, q, S>(a.src, a.ls) + ob;
+      if constexpr (L::ex[q] == 0) {
+        ld.w[q] = __builtin_nontemporal_load(reinterpret_cast<const unsigned int*>(base));
+      } else {
+        const unsigned int lo = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(base));
+        const unsigned int up = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(base) + 1);
+        ld.w[q] = lo | (up << 16);
+      }
+    });
+  }
+}
+
+// Everything after the loads: boundary pieces, both nodes, the stores and the functional's
+// segment sums.
+template <class L, int COLL, int DES, class P>
+__device__ __forceinline__ void x2_finish(const FwdArgs& a, int xa, int y, int z, X2Loads<L, P>& ld) {
+  using S = typename P::S;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const int xb = xa + 1;
   const bool actA = xa < nx && y < ny, actB = xb < nx && y < ny;
@@ -602,60 +639,47 @@ __device__ __forceinline__ void fwd_node_x2(const FwdArgs& a, int xa, int y, int
   const bool objB = objseg && (actB & in_some_term(a.obj, xb, y, z));
   const unsigned int off = actA ? layout_off(a.ls, xa, y, z) : 0u;
   const unsigned int ob = off * (unsigned int)sizeof(S);
-  const int n = (z * ny + y) * nx + xa;
   const bool bndA = actA && on_boundary<L>(xa, y, z, nx, ny, nz);
   const bool bndB = actB && on_boundary<L>(xb, y, z, nx, ny, nz);
-  unsigned long long rowword = 0;
-  unsigned int w[L::Q];
-  T gA = T(0), gB = T(0);
-  if (actA) {
-    rowword = bc_row_word(a.bcrows, nx, ny, xa, y, z);
-    const typename P::G* gp = static_cast<const typename P::G*>(a.gamma) + n;
-    gA = T(gp[0]);
-    if (actB) gB = T(gp[1]);
-    static_for<0, L::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const char* base = pull_base<L, q, S>(a.src, a.ls) + ob;
-      if constexpr (L::ex[q] == 0) {
-        w[q] = __builtin_nontemporal_load(reinterpret_cast<const unsigned int*>(base));
-      } else {
-        const unsigned int lo = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(base));
-        const unsigned int up = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(base) + 1);
-        w[q] = lo | (up << 16);
-      }
-    });
-  }
   // the pieces of both segments (a piece that meets only the other segment holds neither node)
   unsigned long long pieces = 0;
   if (bseg0 | bseg1) {
-    const unsigned int l0 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword), 0);
-    const unsigned int h0 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword >> 32), 0);
-    const unsigned int l1 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword), 32);
-    const unsigned int h1 = __builtin_amdgcn_readlane(static_cast<unsigned int>(rowword >> 32), 32);
+    const unsigned int l0 = __builtin_amdgcn_readlane(static_cast<unsigned int>(ld.rowword), 0);
+    const unsigned int h0 = __builtin_amdgcn_readlane(static_cast<unsigned int>(ld.rowword >> 32), 0);
+    const unsigned int l1 = __builtin_amdgcn_readlane(static_cast<unsigned int>(ld.rowword), 32);
+    const unsigned int h1 = __builtin_amdgcn_readlane(static_cast<unsigned int>(ld.rowword >> 32), 32);
     const unsigned long long p0 = bseg0 ? ((static_cast<unsigned long long>(h0) << 32) | l0) : 0ull;
     const unsigned long long p1 = bseg1 ? ((static_cast<unsigned long long>(h1) << 32) | l1) : 0ull;
     pieces = p0 | p1;
     stage_wave_pieces(a.bcs, a.tfac, a.nbc, pieces);
   }
   double vA = 0.0, vB = 0.0;
-  if (actA) fwd_half_node<L, COLL, DES, P, false>(a, w, gA, bndA, pieces, objA, vA, xa, y, z, off);
-  if (actB) fwd_half_node<L, COLL, DES, P, true>(a, w, gB, bndB, pieces, objB, vB, xb, y, z, off + 1u);
+  if (actA) fwd_half_node<L, COLL, DES, P, false>(a, ld.w, ld.gA, bndA, pieces, objA, vA, xa, y, z, off);
+  if (actB) fwd_half_node<L, COLL, DES, P, true>(a, ld.w, ld.gB, bndB, pieces, objB, vB, xb, y, z, off + 1u);
   if (actA) {
     if (actB) {
       static_for<0, L::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        st_nt<unsigned int>(own_base_w<q, S>(a.dst, a.ls), ob, w[q]);
+        st_nt<unsigned int>(own_base_w<q, S>(a.dst, a.ls), ob, ld.w[q]);
       });
     } else {
       static_for<0, L::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        st_nt<S>(own_base_w<q, S>(a.dst, a.ls), ob, P::template half_of<false>(w[q]));
+        st_nt<S>(own_base_w<q, S>(a.dst, a.ls), ob, P::template half_of<false>(ld.w[q]));
       });
     }
   }
   if (objseg0 | objseg1)
     add_segment_sums_x2(a.obj.acc, bc_row_index(nx, ny, x0, ys, zs), bc_row_index(nx, ny, x1, ys, zs), vA + vB, objA,
                         objB);
+}
+
+template <class L, int COLL, int DES, class P>
+__device__ __forceinline__ void fwd_node_x2(const FwdArgs& a, int xa, int y, int z) {
+  static_assert(sizeof(typename P::S) == 2 && P::stochastic, "two-node sweep of the 16-bit storage forms");
+  X2Loads<L, P> ld;
+  x2_load<L, P>(a, xa, y, z, ld);
+  x2_finish<L, COLL, DES, P>(a, xa, y, z, ld);
 }
 
 template <class L, int COLL, int DES, class P>
