@@ -14,6 +14,8 @@
 //          sweep's m_t (at x + c_q; the own node's opposite plane where x + c_q is outside),
 //          reverse collision, functional source, transposed boundaries (reverse order), store
 //          m_{t-1} at the own node; dJ/dgamma accumulated in FP64.
+//  k_fwd_x2 : k_fwd for the 16-bit storage forms on large grids, two x-adjacent nodes per
+//          thread (same results).
 // Workgroups are 64 (x) x 4 (y) threads; the linear workgroup id is remapped so that each
 // XCD (workgroups are dealt round-robin to the 8 XCDs) receives a contiguous run of tiles and
 // x-neighbouring tiles share that XCD's L2 for the lines straddling their edge.
@@ -545,12 +547,15 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
 // 128-node row stretch (two row segments). x = 0 sits at an even element offset and rows have
 // an even length, so the pair's elements of a plane form one aligned 32-bit word: the planes
 // without an x shift are read, and all planes written, as such words; the x-shifted planes
-// are read as two 16-bit halves. The word of each plane holds the first node's value in its
+// are read as two 16-bit halves (which the compiler merges into one unaligned 32-bit load).
+// The word of each plane holds the first node's value in its
 // low half and the second node's in its high half, and each node's result replaces its own
 // half once that node has been decoded, so one register per plane serves both nodes. Each
 // node's arithmetic is that of fwd_node (same boundary pieces, collision, functional value and
 // rounding seeds), so the stored state is bitwise the one-node sweep's, and the functional's
-// segment sums add in the same order.
+// segment sums add in the same order. (A boundary node's halo reflections go to cells outside
+// the domain, which no thread's own stores touch, so the first node may write them before
+// the pair's words are stored.)
 template <class L, int COLL, int DES, class P, bool HI>
 __device__ __forceinline__ void fwd_half_node(const FwdArgs& a, unsigned int (&w)[L::Q], typename P::T gam, bool bnd,
                                               unsigned long long pieces, bool objnode, double& objv, int x, int y,
