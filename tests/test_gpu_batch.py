@@ -222,6 +222,8 @@ def test_batch_thermal_equals_single_instances(lbmx_mod, coll, design, prec):
     """Thermal realisations (D2Q9 + D2Q5, heat-flux objective) in one batch: each member's
     dJ/dgamma bitwise its single-instance gradient from a moving, warm initial state."""
     spec = [((60, 26), 0.3), ((83, 30), 0.5), ((48, 21), 0.7)]
+    if prec in (lbmx.FP16_SHIFTED, lbmx.BF16_SHIFTED):
+        spec.append(((1025, 260), 0.5))  # wide enough for the single instance's two-node forward sweep
     members = [member_thermal(nx, ny, coll, design, prec, 3000 + i, eta) for i, ((nx, ny), eta) in enumerate(spec)]
     u0, T0 = (0.03, 0.0, 0.0), 0.5
     ref = []
@@ -274,7 +276,9 @@ def test_batch_shifted_storage_forms(lbmx_mod, prec):
     wide = [member(1025, 256, lbmx.CASCADED, lbmx.DESIGN_PBB, prec, 4100, 0.5, True),
             member(1023, 260, lbmx.CASCADED, lbmx.DESIGN_PBB, prec, 4101, 0.4, False, lbmx.OBJ_OUTFLOW)]
     wide3 = [member3d(129, 44, 32, lbmx.MRT, lbmx.DESIGN_BRINKMAN, prec, 4200, 0.5)]
-    for group in (members, members3, wide, wide3):
+    # a moment-space operator, whose doubled populations go straight to the codecs
+    wide_mrt = [member(1025, 256, lbmx.MRT, lbmx.DESIGN_BRINKMAN, prec, 4300, 0.5)]
+    for group in (members, members3, wide, wide3, wide_mrt):
         ref = singles(group, 6)
         b = lbmx.Batch(group)
         b.checkpoint_config(6)
