@@ -415,13 +415,21 @@ __device__ __forceinline__ bool segment_meets_terms(const ObjArgs& o, int x0, in
   });
   return meets;
 }
-// (per node, on the vector unit: each term is tested under the uniform condition m < nterm,
-// so a functional of one or two terms does not pay for the unused slots)
+// Per node, on the vector unit. UNIFORM_COUNT: each term is tested under the uniform condition
+// m < nterm, so a functional of one or two terms does not pay for the unused slots (the
+// single-instance sweeps, whose term count arrives with their arguments). Otherwise every slot
+// is tested branch-free, which the batched sweeps keep: their term count comes from the member
+// table, and branching on it made the robust ensemble's gradient 1.1% slower.
+template <bool UNIFORM_COUNT>
 __device__ __forceinline__ bool in_some_term(const ObjArgs& o, int x, int y, int z) {
   bool in = false;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    if (m < o.nterm) in = in | in_box(o.tbox[m], x, y, z);
+    if constexpr (UNIFORM_COUNT) {
+      if (m < o.nterm) in = in | in_box(o.tbox[m], x, y, z);
+    } else {
+      in = in | ((m < o.nterm) & in_box(o.tbox[m], x, y, z));
+    }
   });
   return in;
 }
@@ -469,14 +477,20 @@ __device__ __forceinline__ void add_segment_sums_x2(double* acc, long long seg0,
 
 // The node's functional value from its pre-collision macros (the collision returns rho and
 // j, so they are not summed again); the sweeps add the segment's sum after the node's stores.
-template <int COLL, class T>
+// (UNIFORM_COUNT as for in_some_term)
+template <int COLL, class T, bool UNIFORM_COUNT>
 __device__ __forceinline__ T node_objective(const ObjArgs& o, T rho, const T (&j)[3], T temp, int x, int y, int z) {
   LBMX_CONTRACT_IN_STATEMENT
   T acc = 0;
   static_for<0, kMaxTerms>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    if (m < o.nterm) {
-      if (in_box(o.tbox[m], x, y, z)) acc += T(o.tweight[m]) * term_value<COLL, T>(o.tkind[m], rho, j, temp);
+    if constexpr (UNIFORM_COUNT) {
+      if (m < o.nterm) {
+        if (in_box(o.tbox[m], x, y, z)) acc += T(o.tweight[m]) * term_value<COLL, T>(o.tkind[m], rho, j, temp);
+      }
+    } else {
+      if ((m < o.nterm) & in_box(o.tbox[m], x, y, z))
+        acc += T(o.tweight[m]) * term_value<COLL, T>(o.tkind[m], rho, j, temp);
     }
   });
   return acc * T(o.time_const);
@@ -484,7 +498,8 @@ __device__ __forceinline__ T node_objective(const ObjArgs& o, T rho, const T (&j
 
 // One forward sweep over the node (x,y,z) of workgroup tile `tile`; called by every thread of
 // the workgroup (the objective reduction synchronises it). tfac: time factor per boundary piece.
-template <class L, int COLL, int DES, class P, int TY>
+// BATCH: called by the batched sweep (member table; see in_some_term)
+template <class L, int COLL, int DES, class P, int TY, bool BATCH = false>
 __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, int x, int y, int z, int tile) {
   using T = typename P::T;
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
@@ -495,7 +510,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
   const bool bseg = a.nbc > 0 && a.mode != 2 && segment_on_boundary<L>(x0, ys, zs, nx, ny, nz);
   double objv = 0.0;
   // (objseg is the wave's: the per-node test is skipped by whole waves)
-  const bool objnode = objseg && (active & in_some_term(a.obj, x, y, z));
+  const bool objnode = objseg && (active & in_some_term<!BATCH>(a.obj, x, y, z));
   const unsigned int off = active ? layout_off(a.ls, x, y, z) : 0u;
   const int n = (z * ny + y) * nx + x;
   const bool bnd = active && on_boundary<L>(x, y, z, nx, ny, nz);
@@ -527,7 +542,7 @@ __device__ __forceinline__ void fwd_node(const FwdArgs& a, const double* tfac, i
     T rho, j[3];
     constexpr bool tw = kStoreTwice<P, COLL, DES>;
     collide<L, COLL, DES, T, tw>(f, gam, a.m, rho, j);
-    if (objnode) objv = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
+    if (objnode) objv = double(node_objective<COLL, T, !BATCH>(a.obj, rho, j, temp, x, y, z));
     store_node<L, P, tw>(a.dst, a.ls, off, f, bnd, x, y, z, nx, ny, nz, a.t);
   }
   if (objseg) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), objv, objnode);
@@ -571,7 +586,7 @@ __device__ __forceinline__ void fwd_half_node(const FwdArgs& a, unsigned int (&w
   T rho, j[3];
   constexpr bool tw = kStoreTwice<P, COLL, DES>;
   collide<L, COLL, DES, T, tw>(f, gam, a.m, rho, j);
-  if (objnode) objv = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
+  if (objnode) objv = double(node_objective<COLL, T, true>(a.obj, rho, j, temp, x, y, z));
   unsigned int r0, r1;
   sr_words(off, a.t, r0, r1);
   static_for<0, L::Q>([&](auto qc) {
@@ -640,8 +655,8 @@ __device__ __forceinline__ void x2_finish(const FwdArgs& a, int xa, int y, int z
   const bool bseg0 = a.nbc > 0 && segment_on_boundary<L>(x0, ys, zs, nx, ny, nz);
   const bool bseg1 = a.nbc > 0 && (x1 < nx) & segment_on_boundary<L>(x1, ys, zs, nx, ny, nz);
   const bool objseg = hi ? objseg1 : objseg0;
-  const bool objA = objseg && (actA & in_some_term(a.obj, xa, y, z));
-  const bool objB = objseg && (actB & in_some_term(a.obj, xb, y, z));
+  const bool objA = objseg && (actA & in_some_term<true>(a.obj, xa, y, z));
+  const bool objB = objseg && (actB & in_some_term<true>(a.obj, xb, y, z));
   const unsigned int off = actA ? layout_off(a.ls, xa, y, z) : 0u;
   const unsigned int ob = off * (unsigned int)sizeof(S);
   const bool bndA = actA && on_boundary<L>(xa, y, z, nx, ny, nz);
@@ -751,7 +766,8 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_init(FwdArgs a, const double
 }
 
 // One adjoint sweep over node (x,y,z); tfac: time factor per boundary piece at step a.t.
-template <class L, int COLL, int DES, class P, int TY>
+// BATCH: called by the batched sweep (member table; see in_some_term)
+template <class L, int COLL, int DES, class P, int TY, bool BATCH = false>
 __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, int x, int y, int z) {
   using T = typename P::T;
   using LS = typename P::LS;
@@ -797,7 +813,7 @@ __device__ __forceinline__ void adj_node(const AdjArgs& a, const double* tfac, i
     });
   }
   T srcA = 0, srcB[3] = {0, 0, 0}, srcAT = 0;
-  if (objseg && in_some_term(a.obj, x, y, z)) {
+  if (objseg && in_some_term<!BATCH>(a.obj, x, y, z)) {
     T rho, j[3];
     flow_macros<L>(f, rho, j);
     T temp = 0;
@@ -887,7 +903,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_fwd_batch(BatchStep st) {
   a.m = from_const(I.m);
   a.obj = from_const(I.obj);
   if (!st.with_obj) a.obj.acc = nullptr;
-  fwd_node<L, COLL, DES, P, TY>(a, st.tfac + (long long)i * st.tf_stride, x, y, z, tile);
+  fwd_node<L, COLL, DES, P, TY, true>(a, st.tfac + (long long)i * st.tf_stride, x, y, z, tile);
 }
 
 template <class L, int COLL, int DES, class P, int TY>
@@ -913,7 +929,7 @@ __global__ __launch_bounds__(kTileX* TY) void k_adj_batch(BatchStep st) {
   a.obj = from_const(I.obj);
   a.obj.acc = nullptr;
   if (!st.with_obj) a.obj.nterm = 0;
-  adj_node<L, COLL, DES, P, TY>(a, st.tfac + (long long)i * st.tf_stride, x, y, z);
+  adj_node<L, COLL, DES, P, TY, true>(a, st.tfac + (long long)i * st.tf_stride, x, y, z);
 }
 
 // Batched forward sweep with two nodes per thread: a tile of 2H rows is swept by H waves, each
@@ -934,8 +950,8 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   const int ys2 = ys + H;
   const bool objseg1 = a.obj.acc != nullptr && (ys < ny) & segment_meets_terms(a.obj, x0, ys, zs);
   const bool objseg2 = a.obj.acc != nullptr && (ys2 < ny) & segment_meets_terms(a.obj, x0, ys2, zs);
-  const bool on1 = objseg1 && (act1 & in_some_term(a.obj, x, y, z));
-  const bool on2 = objseg2 && (act2 & in_some_term(a.obj, x, y2, z));
+  const bool on1 = objseg1 && (act1 & in_some_term<false>(a.obj, x, y, z));
+  const bool on2 = objseg2 && (act2 & in_some_term<false>(a.obj, x, y2, z));
   const typename P::G* gp = static_cast<const typename P::G*>(a.gamma);
   T f1[L::Q], f2[L::Q], g1 = 0, g2 = 0;
   unsigned long long rw1 = 0, rw2 = 0;
@@ -965,13 +981,13 @@ __device__ __forceinline__ void fwd_pair(const FwdArgs& a, const double* tfac, i
   if (act1) {
     const T temp = on1 ? node_temperature<L>(f1) : T(0);
     collide<L, COLL, DES, T, kStoreTwice<P, COLL, DES>>(f1, g1, a.m, rho, j);
-    if (on1) obj1 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y, z));
+    if (on1) obj1 = double(node_objective<COLL, T, false>(a.obj, rho, j, temp, x, y, z));
     store_node<L, P, kStoreTwice<P, COLL, DES>>(a.dst, a.ls, off1, f1, bnd1, x, y, z, nx, ny, a.g.nz, a.t);
   }
   if (act2) {
     const T temp = on2 ? node_temperature<L>(f2) : T(0);
     collide<L, COLL, DES, T, kStoreTwice<P, COLL, DES>>(f2, g2, a.m, rho, j);
-    if (on2) obj2 = double(node_objective<COLL, T>(a.obj, rho, j, temp, x, y2, z));
+    if (on2) obj2 = double(node_objective<COLL, T, false>(a.obj, rho, j, temp, x, y2, z));
     store_node<L, P, kStoreTwice<P, COLL, DES>>(a.dst, a.ls, off2, f2, bnd2, x, y2, z, nx, ny, a.g.nz, a.t);
   }
   if (objseg1) add_segment_sum(a.obj.acc, bc_row_index(nx, ny, x0, y, z), obj1, on1);
