@@ -242,10 +242,8 @@ __device__ __forceinline__ unsigned int mix32(unsigned int x) {
 }
 
 // The random bits of direction q: bits sh .. sh + B - 1 of one of the node's two hashed words
-// (q & 1), sh = ((q >> 1) * 3) % 17, moved to the top of the seed, where the conversions of the
-// 16-bit forms read their B random bits (13 for FP16, 16 for BF16). (The top bits of the hashed
-// words themselves are those of a plain multiplication, which the xor-shift does not reach:
-// as rounding bits they raised the FP16 error of the 2048 x 1024 thermal channel threefold.)
+// (q & 1), sh = ((q >> 1) * 3) % 17, moved to the top of the seed, where the 16-bit codecs read
+// their B random bits (13 for FP16, 16 for BF16).
 template <int q, int B>
 __device__ __forceinline__ unsigned int sr_seed(unsigned int r0, unsigned int r1) {
   constexpr int sh = ((q >> 1) * 3) % 17;
@@ -340,7 +338,8 @@ struct PrecBF16 {
     typedef short short2_t __attribute__((ext_vector_type(2)));
     constexpr float iw = static_cast<float>(enc_scale<L, q, TW>() / L::w[q]);
     constexpr float c = static_cast<float>(store_shift<L, q>());
-    const short2_t r = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(short2_t, word), __builtin_fmaf(v, iw, -c), seed, HI);
+    const short2_t r =
+        __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(short2_t, word), __builtin_fmaf(v, iw, -c), seed, HI);
     return __builtin_bit_cast(unsigned int, r);
   }
   template <class L, int q0, int q1, bool TW = false>
