@@ -563,9 +563,9 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_fwd(FwdArgs a) {
 // an even length, so the pair's elements of a plane form one aligned 32-bit word: the planes
 // without an x shift are read, and all planes written, as such words; the x-shifted planes
 // are read as two 16-bit halves (which the compiler merges into one unaligned 32-bit load).
-// The word of each plane holds the first node's value in its
-// low half and the second node's in its high half, and each node's result replaces its own
-// half once that node has been decoded, so one register per plane serves both nodes. Each
+// The word of each plane holds the first node's value in its low half and the second node's
+// in its high half, and each node's result replaces its own half once that node has been
+// decoded, so one register per plane serves both nodes. Each
 // node's arithmetic is that of fwd_node (same boundary pieces, collision, functional value and
 // rounding seeds), so the stored state is bitwise the one-node sweep's, and the functional's
 // segment sums add in the same order. (A boundary node's halo reflections go to cells outside
